@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Regenerate the committed fixtures under tests/golden/.
+
+Runs only in the development container (it reads the reference tree and
+needs h5dump and the reference build oracle/_ref/libgli_ref.so):
+
+  1. the reference's own HDF5 test data (test/data/*.hdf5,
+     examples/eri_3x3x3.hdf5) -> .npz via `h5dump -b LE` (data only);
+  2. the seeded synthetic gausslet coefficient sets of the workloads;
+  3. golden vectors of the reference build for the synthetic workloads:
+     full results on the small grids, seeded samples of canonical indices
+     on the large ones.
+
+Usage: python tests/golden/make_fixtures.py [--skip-ref-runs]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from gausslectronitegrate_amd import _abi  # noqa: E402
+from gausslectronitegrate_amd.coeffs import synthetic_coefficients  # noqa: E402
+
+REF = "/root/reference"
+H5DUMP = "/opt/conda/bin/h5dump"
+
+
+def h5_read(path, obj, dtype, shape, attribute=False):
+    """Read one dataset/attribute as raw little-endian bytes."""
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "x.bin")
+        flag = "-a" if attribute else "-d"
+        subprocess.run([H5DUMP, "-b", "LE", flag, obj, "-o", out, path],
+                       check=True, stdout=subprocess.DEVNULL, cwd=td)
+        a = np.fromfile(out, dtype=dtype)
+    return a.reshape(shape)
+
+
+def convert_reference_fixtures():
+    d = os.path.join(REF, "test", "data")
+    f = os.path.join(d, "test_eri_integrals.hdf5")
+    np.savez_compressed(os.path.join(HERE, "ref_eri.npz"),
+                        coeffs=h5_read(f, "/gausslet_coeffs", "<f8", (135,), True),
+                        tol=h5_read(f, "/tol", "<f8", (), True),
+                        eri=h5_read(f, "/eri", "<f8", (12, 12, 12, 12)))
+    f = os.path.join(d, "test_erida_integrals.hdf5")
+    np.savez_compressed(os.path.join(HERE, "ref_erida.npz"),
+                        coeffs=h5_read(f, "/gausslet_coeffs", "<f8", (135,), True),
+                        tol=h5_read(f, "/tol", "<f8", (), True),
+                        erida=h5_read(f, "/erida", "<f8", (12, 12)))
+    f = os.path.join(d, "test_kinetic_integrals.hdf5")
+    np.savez_compressed(os.path.join(HERE, "ref_kinetic.npz"),
+                        coeffs=h5_read(f, "/gausslet_coeffs", "<f8", (135,), True),
+                        kgi=h5_read(f, "/kgi", "<f8", (18, 18)))
+    f = os.path.join(d, "test_nuclear_integrals.hdf5")
+    nn = int(h5_read(f, "/num_nuclei", "<i8", (), True))
+    pos = np.stack([h5_read(f, "/nuclear_pos_%d" % k, "<f8", (3,), True) for k in range(nn)])
+    chg = np.array([int(h5_read(f, "/nuclear_charge_%d" % k, "<i8", (), True)) for k in range(nn)])
+    np.savez_compressed(os.path.join(HERE, "ref_nuclear.npz"),
+                        coeffs=h5_read(f, "/gausslet_coeffs", "<f8", (135,), True),
+                        tol=h5_read(f, "/tol", "<f8", (), True),
+                        pos=pos, charges=chg,
+                        ngi=h5_read(f, "/ngi", "<f8", (12, 12)))
+    f = os.path.join(REF, "examples", "eri_3x3x3.hdf5")
+    np.savez_compressed(os.path.join(HERE, "ref_eri_3x3x3.npz"),
+                        grid_length=h5_read(f, "/grid_length", "<i8", (), True),
+                        tol=h5_read(f, "/tol", "<f8", (), True),
+                        coeffs=h5_read(f, "/gausslet_coeffs", "<f8", (135,)),
+                        four_indices=h5_read(f, "/four_indices", "<i8", (1027,)),
+                        values=h5_read(f, "/integral_values", "<f8", (1027,)))
+
+
+# ---------------------------------------------------------------------------
+# the reference build, driven through the C ABI
+
+def load_ref():
+    lib = C.CDLL(os.path.join(REPO, "oracle", "_ref", "libgli_ref.so"))
+    return _abi.bind(lib)
+
+
+def gdata_of(c):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    g = _abi.GaussletData()
+    g.coefficients = c.ctypes.data_as(_abi.P_double)
+    g.indices.istart = -(len(c) - 1) // 2
+    g.indices.num = len(c)
+    return g, c
+
+
+def sym_grid(n):
+    h = (n - 1) // 2
+    return ((-h, n), (-h, n), (-h, n))
+
+
+def ref_enumerate(lib, n):
+    g = _abi.Grid.of(sym_grid(n))
+    out = _abi.SparseEriIndices()
+    lib.enumerate_symmetry_reduced_eri_indices(C.byref(g), C.byref(out))
+    a = np.ctypeslib.as_array(out.four_indices, shape=(out.num,)).copy()
+    lib.delete_sparse_eri_indices(C.byref(out))
+    return a
+
+
+def ref_enumerate_erida(lib, n):
+    g = _abi.Grid.of(sym_grid(n))
+    out = _abi.SparseEridaIndices()
+    lib.enumerate_symmetry_reduced_erida_indices(C.byref(g), C.byref(out))
+    a = np.ctypeslib.as_array(out.two_indices, shape=(out.num,)).copy()
+    lib.delete_sparse_erida_indices(C.byref(out))
+    return a
+
+
+def ref_eri_values(lib, c, n, idx, tol):
+    gd, keep = gdata_of(c)
+    g = _abi.Grid.of(sym_grid(n))
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    out = _abi.SparseEri()
+    lib.compute_sparse_eri_gausslet_integrals(C.byref(gd), C.byref(g), idx.ctypes.data_as(_abi.P_glong),
+                                              len(idx), tol, C.byref(out))
+    v = np.ctypeslib.as_array(out.integral_values, shape=(out.num_entries,)).copy()
+    lib.delete_sparse_eri_gausslet_integrals(C.byref(out))
+    return v
+
+
+def ref_erida_values(lib, c, n, idx, tol):
+    gd, keep = gdata_of(c)
+    g = _abi.Grid.of(sym_grid(n))
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    out = _abi.SparseErida()
+    lib.compute_sparse_erida_gausslet_integrals(C.byref(gd), C.byref(g), idx.ctypes.data_as(_abi.P_glong),
+                                                len(idx), tol, C.byref(out))
+    v = np.ctypeslib.as_array(out.integral_values, shape=(out.num_entries,)).copy()
+    lib.delete_sparse_erida_gausslet_integrals(C.byref(out))
+    return v
+
+
+def ref_nuclear(lib, c, grid, pos, chg, tol):
+    gd, keep = gdata_of(c)
+    g = _abi.Grid.of(grid)
+    nuc = (_abi.Nucleus * len(chg))()
+    for k in range(len(chg)):
+        for a in range(3):
+            nuc[k].pos[a] = float(pos[k][a])
+        nuc[k].charge = int(chg[k])
+    out = _abi.NuclearIntegrals()
+    lib.compute_nuclear_gausslet_integrals(C.byref(gd), C.byref(g), nuc, len(chg), tol, C.byref(out))
+    N = grid[0][1] * grid[1][1] * grid[2][1]
+    v = np.ctypeslib.as_array(out.integral_values, shape=(N, N)).copy()
+    lib.delete_nuclear_gausslet_integrals(C.byref(out))
+    return v
+
+
+def ref_kinetic(lib, c, grid):
+    gd, keep = gdata_of(c)
+    g = _abi.Grid.of(grid)
+    out = _abi.KineticIntegrals()
+    lib.compute_kinetic_gausslet_integrals(C.byref(gd), C.byref(g), C.byref(out))
+    N = grid[0][1] * grid[1][1] * grid[2][1]
+    v = np.ctypeslib.as_array(out.integral_values, shape=(N, N)).copy()
+    lib.delete_kinetic_gausslet_integrals(C.byref(out))
+    return v
+
+
+def ref_perms(lib, n):
+    g = _abi.Grid.of(sym_grid(n))
+    arr = (_abi.P_glong * 48)()
+    lib.evaluate_octahedral_grid_permutations(C.byref(g), arr)
+    return g, arr
+
+
+def sample_canonical(lib, n, count, seed):
+    """Seeded sample of canonical ERI four-indices: random 4-point tuples are
+    centred and canonicalised with the reference's own orbit minimum, the
+    same route its lookups take (pymodule.c:710-780)."""
+    N = n ** 3
+    h = (n - 1) // 2
+    g, perm = ref_perms(lib, n)
+    rng = np.random.default_rng(seed)
+    keys = set()
+    while len(keys) < count:
+        pts = rng.integers(-h, h + 1, size=(4, 3))
+        lo, hi = pts.min(axis=0), pts.max(axis=0)
+        pts = pts - (lo + hi) // 2          # centre: lo + hi in {0, 1}
+        c = pts.min(axis=0) + pts.max(axis=0)
+        best = None
+        for bx in range(abs(int(c[0])) + 1):
+            for by in range(abs(int(c[1])) + 1):
+                for bz in range(abs(int(c[2])) + 1):
+                    q = pts - np.array([bx, by, bz]) * c
+                    lin = [((int(p[0]) + h) * n + int(p[1]) + h) * n + int(p[2]) + h for p in q]
+                    v = lib.minimum_octahedral_orbit_eri_tensor_index(N, perm, *lin)
+                    best = v if best is None else min(best, v)
+        keys.add(int(best))
+    return np.array(sorted(keys), dtype=np.int64)
+
+
+def golden_synthetic(lib):
+    tol = 1e-16
+    c57s0 = synthetic_coefficients(57, 0)
+    c57s1 = synthetic_coefficients(57, 1)
+    c43s0 = synthetic_coefficients(43, 0)
+    c29s0 = synthetic_coefficients(29, 0)
+    # small grid: everything in full (3^3, K=57, seed 0)
+    n = 3
+    idx = ref_enumerate(lib, n)
+    vals = ref_eri_values(lib, c57s0, n, idx, tol)
+    kin = ref_kinetic(lib, c57s0, sym_grid(n))
+    nuc = ref_nuclear(lib, c57s0, sym_grid(n), [[0.0, 0.0, 0.0]], [1], tol)
+    eda_idx = ref_enumerate_erida(lib, n)
+    eda = ref_erida_values(lib, c57s0, n, eda_idx, tol)
+    np.savez_compressed(os.path.join(HERE, "golden_n3_K57s0.npz"), coeffs=c57s0, tol=tol,
+                        eri_indices=idx, eri_values=vals, kinetic=kin, nuclear=nuc,
+                        erida_indices=eda_idx, erida_values=eda)
+    # ERI enumeration lists of the reference at 5^3 (bit-exact pin for the GPU enumeration)
+    idx5 = ref_enumerate(lib, 5)
+    np.savez_compressed(os.path.join(HERE, "golden_enum_n5.npz"), eri_indices=idx5)
+    # 7^3: ERIDA list + values, nuclear matrix with two unit charges at (0, 0, +-0.7)
+    n = 7
+    eda_idx = ref_enumerate_erida(lib, n)
+    eda = ref_erida_values(lib, c57s0, n, eda_idx, tol)
+    pos = [[0.0, 0.0, 0.7], [0.0, 0.0, -0.7]]
+    nuc = ref_nuclear(lib, c57s0, sym_grid(n), pos, [1, 1], tol)
+    rng = np.random.default_rng(7)
+    N = n ** 3
+    rows = rng.integers(0, N, size=2048)
+    cols = rng.integers(0, N, size=2048)
+    np.savez_compressed(os.path.join(HERE, "golden_n7_K57s0.npz"), coeffs=c57s0, tol=tol,
+                        erida_indices=eda_idx, erida_values=eda, nuc_pos=np.array(pos),
+                        nuc_charges=np.array([1, 1]), nuc_rows=rows, nuc_cols=cols,
+                        nuc_sample=nuc[rows, cols], nuc_rowsum=nuc.sum(axis=1), nuc_trace=np.trace(nuc))
+    # sampled canonical ERI indices with reference values at the large grids
+    cases = [(5, 29, c29s0, "K29s0", 256), (7, 43, c43s0, "K43s0", 256), (9, 57, c57s1, "K57s1", 192),
+             (11, 57, c57s0, "K57s0", 128), (13, 57, c57s0, "K57s0", 64)]
+    for n, K, c, tag, count in cases:
+        idx = sample_canonical(lib, n, count, seed=1000 + n)
+        vals = ref_eri_values(lib, c, n, idx, tol)
+        np.savez_compressed(os.path.join(HERE, "golden_sample_n%d_%s.npz" % (n, tag)), coeffs=c, tol=tol,
+                            eri_indices=idx, eri_values=vals)
+        print("sample n=%d K=%d: %d values, max |v| %.3g" % (n, K, len(vals), np.abs(vals).max()))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--skip-ref-runs", action="store_true")
+    args = ap.parse_args()
+    convert_reference_fixtures()
+    for K, seed in ((29, 0), (43, 0), (57, 0), (57, 1)):
+        np.save(os.path.join(HERE, "coeffs_K%d_s%d.npy" % (K, seed)), synthetic_coefficients(K, seed))
+    if not args.skip_ref_runs:
+        golden_synthetic(load_ref())
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
