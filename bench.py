@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Benchmark: unique ERI elements/s on an 11^3 gausslet grid (K = 57, seed 0, tol 1e-16).
+
+One step = the ERI values of every symmetry-unique four-index of the grid
+(261,244,251 elements), from the coefficients to the values in HBM: the
+factor/product/fold tables, the key-tensor contraction and the per-element
+gather.  The index list is enumerated on the GPU beforehand (timed and
+reported separately) and stays resident in HBM, as do the values.
+
+Printed: one JSON line with the throughput, the roofline of the dominant
+kernel (HIP events on the library stream), the per-element FP64 path on a
+seeded sample, the error of the evaluated path against the per-element FP64
+path and the reference C build, and a CPU baseline timed on the host cores.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from gausslectronitegrate_amd import _abi, _lib  # noqa: E402
+from gausslectronitegrate_amd.coeffs import synthetic_coefficients  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+METHODS = {"tensor": 0, "mixed": 1, "direct": 2}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sym_grid6(n):
+    h = (n - 1) // 2
+    return np.array([-h, n, -h, n, -h, n], dtype=np.int64)
+
+
+class Dev:
+    def __init__(self, L, nbytes):
+        self.L = L
+        self.ptr = C.c_void_p()
+        _lib.check(L.gli_device_malloc(C.byref(self.ptr), max(8, int(nbytes))), "device malloc")
+
+    def __del__(self):
+        if self.ptr:
+            self.L.gli_device_free(self.ptr)
+
+
+def eri_step(L, c, g, d_idx, num, tol, method, d_val):
+    _lib.check(L.gli_eri_values_device(c.ctypes.data_as(_abi.P_double), len(c), g.ctypes.data_as(_abi.P_glong),
+                                       d_idx, num, tol, method, d_val), "gli_eri_values_device")
+
+
+def phase_times(L):
+    ms = (C.c_double * 16)()
+    names = (C.c_char_p * 16)()
+    n = L.gli_phase_times(ms, names, 16)
+    return {names[i].decode(): ms[i] for i in range(max(n, 0))}
+
+
+def cpu_baseline(c, n, sample_idx, tol, target_s):
+    """Reference C build (oracle/_ref) if present, else the oracle restatement, on all host threads."""
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    kind = "port"
+    run = None
+    if os.path.exists(orc.REF_LIB_PATH):
+        ref = _abi.bind(C.CDLL(orc.REF_LIB_PATH))
+        kind = "reference"
+
+        def run(idx):
+            gd = _abi.GaussletData()
+            cc = np.ascontiguousarray(c)
+            gd.coefficients = cc.ctypes.data_as(_abi.P_double)
+            gd.indices.istart = -(len(cc) - 1) // 2
+            gd.indices.num = len(cc)
+            h = (n - 1) // 2
+            grid = _abi.Grid.of(((-h, n), (-h, n), (-h, n)))
+            ii = np.ascontiguousarray(idx)
+            out = _abi.SparseEri()
+            ref.compute_sparse_eri_gausslet_integrals(C.byref(gd), C.byref(grid), ii.ctypes.data_as(_abi.P_glong),
+                                                      len(ii), tol, C.byref(out))
+            v = np.ctypeslib.as_array(out.integral_values, shape=(len(ii),)).copy()
+            ref.delete_sparse_eri_gausslet_integrals(C.byref(out))
+            return v
+    else:
+        h = (n - 1) // 2
+
+        def run(idx):
+            return orc.eri_sparse(c, ((-h, n), (-h, n), (-h, n)), idx, tol)
+    # calibrate on a small slice, then size the timed sample to about target_s
+    probe = sample_idx[: max(threads * 2, 32)]
+    t0 = time.perf_counter()
+    run(probe)
+    dt = time.perf_counter() - t0
+    per = dt / len(probe)
+    count = int(min(len(sample_idx), max(len(probe), target_s / max(per, 1e-9))))
+    sub = sample_idx[:count]
+    t0 = time.perf_counter()
+    vals = run(sub)
+    dt = time.perf_counter() - t0
+    return {"value": count / dt, "unit": "elements/s", "cores": threads, "kind": kind,
+            "sample": "%d seeded symmetry-unique 11^3 four-indices (K=57, seed 0, tol 1e-16), %.1f s" % (count, dt)}, sub, vals
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=11)
+    ap.add_argument("--K", type=int, default=57)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tol", type=float, default=1e-16)
+    ap.add_argument("--method", choices=sorted(METHODS), default="tensor")
+    ap.add_argument("--direct-sample", type=int, default=16384)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would take a contiguous, "
+              "cost-balanced slice of the sorted index list); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # device memory and events are plumbing here; the work runs in libgli_amd.so
+    L = _lib.require_gpu()
+    stream = torch.cuda.ExternalStream(L.gli_stream())
+    method = METHODS[args.method]
+    c = synthetic_coefficients(args.K, args.seed)
+    g = sym_grid6(args.n)
+
+    # enumeration (grid only), resident in HBM
+    log("enumerating %d^3 ..." % args.n)
+    p = C.c_void_p()
+    num = C.c_int64()
+    t0 = time.perf_counter()
+    _lib.check(L.gli_eri_enumerate_device(args.n, C.byref(p), C.byref(num)), "enumeration")
+    enum_ms = (time.perf_counter() - t0) * 1e3
+    num = num.value
+    d_idx = p
+    log("unique four-indices: %d (%.1f ms)" % (num, enum_ms))
+    d_val = Dev(L, 8 * num)
+
+    # warmup, then K timed steps bracketed by synchronisation
+    for _ in range(args.warmup):
+        eri_step(L, c, g, d_idx, num, args.tol, method, d_val.ptr)
+    L.gli_synchronize()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        eri_step(L, c, g, d_idx, num, args.tol, method, d_val.ptr)
+    ev1.record(stream)
+    L.gli_synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = ev0.elapsed_time(ev1) / args.steps
+    log("step: %.3f ms (events), wall %.3f ms/step" % (step_ms, wall * 1e3 / args.steps))
+
+    # per-phase HIP events over a few extra steps: dominant kernel duration
+    L.gli_enable_phase_timing(1)
+    phases = {}
+    reps = 5
+    for _ in range(reps):
+        eri_step(L, c, g, d_idx, num, args.tol, method, d_val.ptr)
+        for k, v in phase_times(L).items():
+            phases[k] = phases.get(k, 0.0) + v / reps
+    L.gli_enable_phase_timing(0)
+    log("phases (ms):", {k: round(v, 4) for k, v in phases.items()})
+
+    # per-element FP64 path on a seeded sample, and errors against it
+    rng = np.random.default_rng(12345)
+    host_idx = np.empty(num, dtype=np.int64)
+    _lib.check(L.gli_memcpy(host_idx.ctypes.data, d_idx, 8 * num, 1), "copy indices")
+    vals = np.empty(num)
+    _lib.check(L.gli_memcpy(vals.ctypes.data, d_val.ptr, 8 * num, 1), "copy values")
+    samp = np.sort(rng.choice(num, size=min(args.direct_sample, num), replace=False))
+    s_idx = np.ascontiguousarray(host_idx[samp])
+    ds_idx = Dev(L, 8 * len(s_idx))
+    ds_val = Dev(L, 8 * len(s_idx))
+    _lib.check(L.gli_memcpy(ds_idx.ptr, s_idx.ctypes.data, 8 * len(s_idx), 0), "copy sample")
+    eri_step(L, c, g, ds_idx.ptr, len(s_idx), args.tol, 2, ds_val.ptr)
+    L.gli_synchronize()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    eri_step(L, c, g, ds_idx.ptr, len(s_idx), args.tol, 2, ds_val.ptr)
+    e1.record(stream)
+    L.gli_synchronize()
+    torch.cuda.synchronize()
+    direct_ms = e0.elapsed_time(e1)
+    direct_vals = np.empty(len(s_idx))
+    _lib.check(L.gli_memcpy(direct_vals.ctypes.data, ds_val.ptr, 8 * len(s_idx), 1), "copy direct")
+    err_abs = float(np.max(np.abs(vals[samp] - direct_vals)))
+    err_rel = float(np.max(np.abs(vals[samp] - direct_vals) / np.maximum(np.abs(direct_vals), 1e-300)))
+    direct_rate = len(s_idx) / (direct_ms / 1e3)
+    log("direct FP64 sample: %d elements in %.2f ms; max abs err %.3g" % (len(s_idx), direct_ms, err_abs))
+
+    # roofline of the dominant kernel
+    dom = max(phases.items(), key=lambda kv: kv[1]) if phases else ("step", step_ms)
+    name, dom_ms = dom
+    M = 6 * (args.n - 1) + 2 * (args.K - 1)
+    Mp = (M + 1 + 15) // 16 * 16
+    C1 = L.gli_eri_key_count(args.n, int(np.array_equal(c, c[::-1])))
+    Cp = (C1 + 63) // 64 * 64
+    if name == "gather":
+        achieved = 16.0 * num / (dom_ms / 1e3) / 1e9   # index in + value out, bytes per element
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_gather"}
+    else:
+        flops = {"mode1": 2.0 * Mp * Mp * Cp * Mp, "mode2": 2.0 * Mp * Cp * Cp * Mp / 2,
+                 "mode3": 2.0 * Cp * Cp * Cp * Mp / 6}.get(name, 0.0)
+        achieved = flops / (dom_ms / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFS, "traffic": None, "kernel": "k_mode_product:" + name}
+    roof["kernel_ms"] = dom_ms
+
+    cpu = None
+    cpu_err = None
+    if not args.no_cpu:
+        log("CPU baseline ...")
+        cpu, sub, cpu_vals = cpu_baseline(c, args.n, s_idx, args.tol, args.cpu_seconds)
+        pos = np.searchsorted(s_idx, sub)
+        cpu_err = float(np.max(np.abs(vals[samp][pos] - cpu_vals)))
+        log("cpu:", cpu, "max abs err vs cpu", cpu_err)
+
+    value = num / (step_ms / 1e3)
+    line = {
+        "metric": "unique ERI elements/s at 11^3 grid, mixed vs FP64, err<=1e-10 vs FP64",
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64"}[args.method],
+        "data": "synthetic (seeded gausslet coefficients)",
+        "config": {"workload": "eri_%dx%dx%d_K%d_seed%d" % (args.n, args.n, args.n, args.K, args.seed),
+                   "grid": args.n, "K": args.K, "seed": args.seed, "tol": args.tol, "method": args.method,
+                   "unique_elements": num, "axis_keys": C1},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "phases_ms": phases,
+        "enumeration_ms": enum_ms,
+        "fp64_direct": {"elements_per_s": direct_rate, "sample": len(s_idx), "ms": direct_ms},
+        "max_abs_err_vs_fp64_direct": err_abs,
+        "max_rel_err_vs_fp64_direct": err_rel,
+        "max_abs_err_vs_cpu_reference": cpu_err,
+        "speedup_vs_fp64_direct": value / direct_rate,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    L.gli_device_free(d_idx)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+"""Build the C-ABI library lib/libgli_amd.so (HIP kernels for gfx950 + host C).
+
+    python -m gausslectronitegrate_amd.build      (or build() from Python)
+
+Every HIP source is compiled with hipcc for gfx950 only; the host C sources
+(the reference-compatible API) with gcc; everything links into one shared
+library whose exported symbols are the declarations of include/*.h.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB_DIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIB_DIR, "libgli_amd.so")
+INCLUDE = os.path.join(REPO, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CC = os.environ.get("CC", "gcc")
+
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result"]
+C_FLAGS = ["-O3", "-fPIC", "-std=gnu11", "-fopenmp", "-Wall", "-Wno-unused-function"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build step failed:\n  %s\n%s" % (" ".join(cmd), r.stdout))
+    return r.stdout
+
+
+def sources():
+    hip = sorted(os.path.join(CSRC, "hip", f) for f in os.listdir(os.path.join(CSRC, "hip")) if f.endswith(".hip"))
+    c = sorted(os.path.join(CSRC, "host", f) for f in os.listdir(os.path.join(CSRC, "host")) if f.endswith(".c"))
+    return hip, c
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    _run([sys.executable, os.path.join(CSRC, "host", "gen_symmetry.py"), os.path.join(BUILD, "oh_tables.inc")])
+    incs = ["-I" + INCLUDE, "-I" + CSRC, "-I" + BUILD]
+    hip, c = sources()
+    jobs = []
+    for s in hip:
+        o = os.path.join(BUILD, os.path.basename(s) + ".o")
+        jobs.append(([HIPCC] + HIP_FLAGS + incs + ["-c", s, "-o", o], o))
+    for s in c:
+        o = os.path.join(BUILD, os.path.basename(s) + ".o")
+        jobs.append(([CC] + C_FLAGS + incs + ["-c", s, "-o", o], o))
+    workers = min(8, max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        outs = list(ex.map(lambda j: _run(j[0]), jobs))
+    if verbose:
+        for o in outs:
+            if o.strip():
+                print(o)
+    objs = [o for _, o in jobs]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-Wl,-Bsymbolic", "-Wl,--no-undefined", "-lgomp", "-lm"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,42 @@
+/*
+ * Thin C-ABI between the host C library (the reference-compatible API) and
+ * the HIP kernels.  Plain C types only; every function returns 0 on success
+ * and a negative status on failure (message: gli_last_error()).
+ * Host arrays returned through int64_t** are 64-byte aligned and released
+ * with free(), like every array of the public API.
+ * A grid is passed as g[6] = {istart_x, num_x, istart_y, num_y, istart_z, num_z}.
+ */
+#pragma once
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ERI value methods */
+#define GLI_ERI_TENSOR_FP64  0  /* key-tensor contraction, FP64 throughout (default) */
+#define GLI_ERI_TENSOR_MIXED 1  /* key-tensor contraction, split-precision MFMA bulk, FP64 final sums */
+#define GLI_ERI_DIRECT_FP64  2  /* per-element FP64 sum, no cross-element reuse */
+
+int gli_hip_eri_enumerate(int64_t n, int64_t** out, int64_t* num);
+int gli_hip_erida_enumerate(int64_t n, int64_t** out, int64_t* num);
+
+int gli_hip_eri_values(const double* coeffs, int64_t K, const int64_t g[6],
+                       const int64_t* indices, int64_t num, double tol, int method, double* values);
+int gli_hip_erida_values(const double* coeffs, int64_t K, const int64_t g[6],
+                         const int64_t* indices, int64_t num, double tol, double* values);
+
+int gli_hip_nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6],
+                           const double* pos, const int* charges, int num_nuclei, double tol, double* out);
+int gli_hip_kinetic_matrix(const double* coeffs, int64_t K, const int64_t g[6], double* out);
+
+/* dense reconstruction and contractions of a sparse ERI / ERIDA set */
+int gli_hip_eri_fill_dense(const int64_t gs[6], const int64_t* keys, const double* vals, int64_t num,
+                           const int64_t gd[6], double* out);
+int gli_hip_erida_fill_dense(const int64_t gs[6], const int64_t* keys, const double* vals, int64_t num,
+                             const int64_t gd[6], double* out);
+
+#ifdef __cplusplus
+}
+#endif

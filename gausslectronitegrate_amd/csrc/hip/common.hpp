@@ -1,0 +1,121 @@
+// Shared device context, error handling and small helpers of the HIP side.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace gli {
+
+// ---------------------------------------------------------------------------
+// errors: every failure records a message; the C entry points turn it into a
+// status code, the reference-compatible void entry points into an abort.
+
+void set_error(const char* file, int line, const std::string& msg);
+
+struct Failure {
+	std::string msg;
+};
+
+#define GLI_HIP(call)                                                                          \
+	do {                                                                                       \
+		hipError_t e_ = (call);                                                                \
+		if (e_ != hipSuccess) {                                                                \
+			::gli::set_error(__FILE__, __LINE__, std::string(#call) + ": " + hipGetErrorString(e_)); \
+			throw ::gli::Failure{hipGetErrorString(e_)};                                       \
+		}                                                                                      \
+	} while (0)
+
+#define GLI_REQUIRE(cond, text)                                                 \
+	do {                                                                        \
+		if (!(cond)) {                                                          \
+			::gli::set_error(__FILE__, __LINE__, text);                         \
+			throw ::gli::Failure{text};                                         \
+		}                                                                       \
+	} while (0)
+
+// ---------------------------------------------------------------------------
+// device buffers
+
+template <typename T>
+struct DevBuf {
+	T* p = nullptr;
+	size_t n = 0;
+	DevBuf() = default;
+	explicit DevBuf(size_t count) { alloc(count); }
+	DevBuf(const DevBuf&) = delete;
+	DevBuf& operator=(const DevBuf&) = delete;
+	DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+	DevBuf& operator=(DevBuf&& o) noexcept
+	{
+		if (this != &o) {
+			release();
+			p = o.p; n = o.n;
+			o.p = nullptr; o.n = 0;
+		}
+		return *this;
+	}
+	~DevBuf() { release(); }
+	void alloc(size_t count)
+	{
+		release();
+		if (count > 0) { GLI_HIP(hipMalloc(&p, count * sizeof(T))); }
+		n = count;
+	}
+	// grow-only reuse
+	void reserve(size_t count)
+	{
+		if (count > n) { alloc(count); }
+	}
+	void release()
+	{
+		if (p != nullptr) { (void)hipFree(p); }
+		p = nullptr;
+		n = 0;
+	}
+};
+
+// ---------------------------------------------------------------------------
+// process-wide context: one stream at default priority, cached Coulomb tables
+
+struct Context {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::recursive_mutex mu;
+	// ERI kernel table T[rho], w = 1/3, rho = 36 r^2 (grows on demand)
+	std::vector<double> eri_table_host;
+	DevBuf<double> eri_table_dev;
+	// ERIDA kernel table, w = sqrt(2)/3, rho = 9 r^2
+	std::vector<double> erida_table_host;
+	DevBuf<double> erida_table_dev;
+	// events of the last ERI value evaluation, one per phase boundary
+	static constexpr int kMaxPhases = 16;
+	hipEvent_t phase_ev[kMaxPhases + 1] = {};
+	const char* phase_name[kMaxPhases] = {};
+	int num_phases = 0;
+	bool phases_on = false;
+};
+
+Context& ctx();
+
+// ERI / ERIDA Coulomb tables of at least n entries, device resident
+const double* eri_table(int64_t n);
+const double* erida_table(int64_t n);
+
+// phase timing (records an event on the library stream when enabled)
+void phase_begin();
+void phase_mark(const char* name);
+
+// Y[p][k] = sum_{mu < Kd} X[mu * ldx + p] * B[mu * ldb + k] for p < P, k < C (FP64 MFMA);
+// with Cp > 0 and tri_cols / tri_rows, tiles outside k >= r / outer <= r (p = outer*Cp + r) are skipped
+void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
+                  double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
+
+}  // namespace gli

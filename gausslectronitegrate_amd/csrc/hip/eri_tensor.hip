@@ -1,0 +1,611 @@
+// ERI values by the key-tensor contraction, plus the per-element FP64 path.
+//
+// For a four-index (ij|kl) with grid points p_i..p_l the reference evaluates
+// (eri_integrals.c:608-669)
+//     sum_{tx,ty,tz} Px[tx] Py[ty] Pz[tz] T[dx^2 + dy^2 + dz^2],  d_a = 3 cd_a + t_a,
+// where P_a = P_{s01,s23} is the product table of the axis shifts
+// s01 = (p_i - p_j)_a, s23 = (p_k - p_l)_a, cd_a = ((p_i + p_j) - (p_k + p_l))_a,
+// and entries with |P| <= tol contribute nothing (eri_integrals.c:77-103, 629-658).
+//
+// T depends on |d_a| only, so each axis reduces to the folded column
+//     B[mu][key] = sum_{t : |3 cd + t| = mu} P'_{s01,s23}[t],   mu = 0 .. M,
+// with P' = P * [|P| > tol].  Because f_{-s} = f_s for every coefficient set,
+// P_{s01,s23} depends on (|s01|, |s23|) only; for mirror-symmetric coefficients
+// P is also even in t and symmetric in (s01, s23), so B depends on
+// (min, max of |s01|, |s23|; |cd|).  The integral is then
+//     V[k1][k2][k3] = sum_{mu1,mu2,mu3} B[mu1][k1] B[mu2][k2] B[mu3][k3] T[mu1^2 + mu2^2 + mu3^2],
+// a fully symmetric 3-tensor over the distinct axis keys, built by three mode
+// products of the kernel tensor T3 with the one factor matrix B (FP64 MFMA),
+// and each element gathers V at its sorted key triple.  This is an exact
+// re-association of the reference sum: same terms, different order.
+#include "common.hpp"
+#include "../gli_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <tuple>
+
+namespace gli {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// 1D tables
+
+// f[s][m] = (sqrt(pi)/3) sum_{a+b=m} c_a c_b exp(-(0.5 (3 s + a - b))^2), s = 0 .. R-1
+// (gausslet_factors.c:12-50; f_{-s} = f_s).  Same ascending-a summation order.
+__global__ void k_factors(const double* __restrict__ c, int K, double* __restrict__ f)
+{
+	const int L = 2 * K - 1;
+	const int s = blockIdx.y;
+	const int m = blockIdx.x * blockDim.x + threadIdx.x;
+	if (m >= L) { return; }
+	const double prefac = 0.590817950301838675766;
+	double acc = 0;
+	const int ilo = max(0, m - (K - 1)), ihi = min(K - 1, m);
+	for (int i = ilo; i <= ihi; ++i) {
+		const int j = m - i;
+		const double x = 0.5 * (3 * s + (i - j));
+		acc += prefac * c[i] * c[j] * exp(-x * x);
+	}
+	f[(int64_t)s * L + m] = acc;
+}
+
+// P[a][b][t] = sum_m f_a[m] f_b[m - (t - (2K-2))], zeroed where |P| <= tol
+// (eri_integrals.c:59-103 with the tolerance skips of 629/637/658 folded in)
+__global__ void k_products(const double* __restrict__ f, int K, int R, double tol, double* __restrict__ P)
+{
+	const int L = 2 * K - 1, LP = 4 * K - 3;
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int ab = blockIdx.y;
+	if (t >= LP) { return; }
+	const int a = ab / R, b = ab % R;
+	const int d = t - (2 * K - 2);  // m - m'
+	const double* fa = f + (int64_t)a * L;
+	const double* fb = f + (int64_t)b * L;
+	double acc = 0;
+	const int mlo = max(0, d), mhi = min(L - 1, L - 1 + d);
+	for (int m = mlo; m <= mhi; ++m) { acc += fa[m] * fb[m - d]; }
+	P[(int64_t)ab * LP + t] = (fabs(acc) > tol) ? acc : 0.0;
+}
+
+// B[mu][q] = P'[a][b][mu - 3cd] + (mu > 0 ? P'[a][b][-mu - 3cd] : 0)
+__global__ void k_fold(const double* __restrict__ P, const int4* __restrict__ keys, int C1, int K, int R,
+                       int M, int Cp, double* __restrict__ B)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	const int mu = blockIdx.y;
+	if (q >= Cp) { return; }
+	double v = 0;
+	if (q < C1 && mu <= M) {
+		const int4 k = keys[q];
+		const int LP = 4 * K - 3, off = 2 * K - 2;
+		const double* p = P + ((int64_t)k.x * R + k.y) * LP;
+		const int t1 = mu - 3 * k.z + off;
+		if (t1 >= 0 && t1 < LP) { v += p[t1]; }
+		if (mu > 0) {
+			const int t2 = -mu - 3 * k.z + off;
+			if (t2 >= 0 && t2 < LP) { v += p[t2]; }
+		}
+	}
+	B[(int64_t)mu * Cp + q] = v;
+}
+
+// T3[m1][m2][m3] = T[m1^2 + m2^2 + m3^2] for m <= M, zero padding up to Mp
+__global__ void k_kernel_tensor(const double* __restrict__ T, int M, int Mp, double* __restrict__ T3)
+{
+	const int m3 = blockIdx.x * blockDim.x + threadIdx.x;
+	const int m2 = blockIdx.y, m1 = blockIdx.z;
+	if (m3 >= Mp) { return; }
+	double v = 0;
+	if (m1 <= M && m2 <= M && m3 <= M) { v = T[m1 * m1 + m2 * m2 + m3 * m3]; }
+	T3[((int64_t)m1 * Mp + m2) * Mp + m3] = v;
+}
+
+// ---------------------------------------------------------------------------
+// mode product  Y[p][k] = sum_{mu < Kd} X[mu][p] B[mu][k]   (FP64 MFMA 16x16x4)
+//
+// 64x64 output tile per workgroup of 4 waves (each 32x32 = 2x2 MFMA blocks),
+// mu staged through LDS 16 at a time.  Optional triangular skipping for the
+// symmetric tensor: with p = outer * Cp + r,
+//   tri_cols: only k >= r is needed;  tri_rows: only outer <= r is needed.
+// Tiles entirely outside the needed region are not computed (never read later).
+
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+__global__ __launch_bounds__(256) void k_mode_product(
+	const double* __restrict__ X, int64_t ldx, int64_t P, const double* __restrict__ B, int64_t ldb, int C, int Kd,
+	double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base)
+{
+	__shared__ double As[GBK][GBM];
+	__shared__ double Bs[GBK][GBN];
+	const int tid = threadIdx.x;
+	const int lane = tid & 63, wave = tid >> 6;
+	const int64_t p0 = (int64_t)blockIdx.x * GBM;
+	const int k0 = blockIdx.y * GBN;
+
+	if (tri_cols || tri_rows) {
+		// global row index (with p_base) decomposes as outer * Cp + r; Cp % GBM == 0 keeps outer fixed per tile
+		const int64_t gp = p_base + p0;
+		const int64_t outer = gp / Cp;
+		const int64_t rlo = gp % Cp, rhi = rlo + GBM - 1;
+		if (tri_cols && k0 + GBN - 1 < rlo) { return; }
+		if (tri_rows && rhi < outer) { return; }
+	}
+
+	const int wm = wave >> 1, wn = wave & 1;
+	d4 acc[2][2];
+	for (int i = 0; i < 2; ++i) {
+		for (int j = 0; j < 2; ++j) { acc[i][j] = d4{0, 0, 0, 0}; }
+	}
+
+	for (int mu0 = 0; mu0 < Kd; mu0 += GBK) {
+#pragma unroll
+		for (int r = 0; r < (GBK * GBM) / 256; ++r) {
+			const int id = tid + 256 * r;
+			const int mu = id / GBM, p = id % GBM;
+			double v = 0;
+			if (mu0 + mu < Kd && p0 + p < P) { v = X[(int64_t)(mu0 + mu) * ldx + p0 + p]; }
+			As[mu][p] = v;
+		}
+#pragma unroll
+		for (int r = 0; r < (GBK * GBN) / 256; ++r) {
+			const int id = tid + 256 * r;
+			const int mu = id / GBN, k = id % GBN;
+			double v = 0;
+			if (mu0 + mu < Kd && k0 + k < C) { v = B[(int64_t)(mu0 + mu) * ldb + k0 + k]; }
+			Bs[mu][k] = v;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int kk = 0; kk < GBK; kk += 4) {
+			double a[2], b[2];
+			for (int i = 0; i < 2; ++i) { a[i] = As[kk + lane / 16][wm * 32 + i * 16 + lane % 16]; }
+			for (int j = 0; j < 2; ++j) { b[j] = Bs[kk + lane / 16][wn * 32 + j * 16 + lane % 16]; }
+			for (int i = 0; i < 2; ++i) {
+				for (int j = 0; j < 2; ++j) {
+					acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+				}
+			}
+		}
+		__syncthreads();
+	}
+
+	// accumulator layout of the 16x16 FP64 block: lane holds column lane%16, rows 4*(lane/16)+v
+	for (int i = 0; i < 2; ++i) {
+		for (int j = 0; j < 2; ++j) {
+			const int col = k0 + wn * 32 + j * 16 + lane % 16;
+			for (int v = 0; v < 4; ++v) {
+				const int64_t row = p0 + wm * 32 + i * 16 + 4 * (lane / 16) + v;
+				if (row < P && col < C) { Y[row * ldy + col] = acc[i][j][v]; }
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// per-element decoding
+
+struct GridDev {
+	int n[3];       // points per axis
+	int N;          // total points
+	int R;          // max points per axis (key tables are built for spans < R)
+	int64_t N2;     // N * N
+	int64_t N4;     // N^4: four-indices at or above it are outside the grid
+};
+
+// linear grid index -> offsets (x, y, z) from the range starts
+__device__ inline void decode_point(const GridDev& g, int idx, int& x, int& y, int& z)
+{
+	z = idx % g.n[2];
+	const int r = idx / g.n[2];
+	y = r % g.n[1];
+	x = r / g.n[1];
+}
+
+// four-index -> (i, j, k, l); exact for four-indices below 2^53 via a corrected FP64 quotient
+__device__ inline void decode_four(const GridDev& g, int64_t t, int& i, int& j, int& k, int& l)
+{
+	int64_t q = (int64_t)((double)t / (double)g.N2);
+	int64_t r = t - q * g.N2;
+	if (r < 0) { q -= 1; r += g.N2; }
+	else if (r >= g.N2) { q += 1; r -= g.N2; }
+	i = (int)(q / g.N);
+	j = (int)(q % g.N);
+	k = (int)(r / g.N);
+	l = (int)(r % g.N);
+}
+
+// key id of one axis: a = |s01|, b = |s23|, cd = (pi + pj) - (pk + pl)
+__device__ inline int axis_key(const int* __restrict__ lut, int R, int pi, int pj, int pk, int pl)
+{
+	const int a = abs(pi - pj), b = abs(pk - pl), cd = (pi + pj) - (pk + pl);
+	return lut[(a * R + b) * (4 * R - 3) + cd + 2 * (R - 1)];
+}
+
+__device__ inline void element_keys(const GridDev& g, const int* __restrict__ lut, int64_t t, int& k1, int& k2, int& k3)
+{
+	int i, j, k, l;
+	decode_four(g, t, i, j, k, l);
+	int pi[3], pj[3], pk[3], pl[3];
+	decode_point(g, i, pi[0], pi[1], pi[2]);
+	decode_point(g, j, pj[0], pj[1], pj[2]);
+	decode_point(g, k, pk[0], pk[1], pk[2]);
+	decode_point(g, l, pl[0], pl[1], pl[2]);
+	int kk[3];
+	for (int a = 0; a < 3; ++a) { kk[a] = axis_key(lut, g.R, pi[a], pj[a], pk[a], pl[a]); }
+	// sort ascending (V is symmetric in its three arguments)
+	int x = kk[0], y = kk[1], z = kk[2], s;
+	if (x > y) { s = x; x = y; y = s; }
+	if (y > z) { s = y; y = z; z = s; }
+	if (x > y) { s = x; x = y; y = s; }
+	k1 = x; k2 = y; k3 = z;
+}
+
+// values[e] = V[k1][k2][k3] for elements whose smallest key lies in [klo, khi)
+__global__ void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g, const int* __restrict__ lut,
+                         const double* __restrict__ V, int Cp, int klo, int khi, double* __restrict__ out)
+{
+	const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= num) { return; }
+	const int64_t t = idx[e];
+	if (t < 0 || t >= g.N4) {
+		if (klo == 0) { out[e] = __builtin_nan(""); }
+		return;
+	}
+	int k1, k2, k3;
+	element_keys(g, lut, t, k1, k2, k3);
+	if (k1 < klo || k1 >= khi) { return; }
+	out[e] = V[((int64_t)(k1 - klo) * Cp + k2) * Cp + k3];
+}
+
+// per-element FP64 sum over the folded columns, no reuse across elements:
+// value = sum_{m1,m2} B1[m1] B2[m2] sum_{m3} B3[m3] T[m1^2 + m2^2 + m3^2]
+__global__ __launch_bounds__(256) void k_eri_direct(const int64_t* __restrict__ idx, int64_t num, GridDev g,
+	const int* __restrict__ lut, const double* __restrict__ B, int Cp, int M, int Mp, const double* __restrict__ T,
+	double* __restrict__ out)
+{
+	extern __shared__ double smem[];
+	double* col = smem;               // 3 x Mp folded columns
+	double* red = smem + 3 * Mp;      // 256 partial sums
+	__shared__ int win[3][2];
+	const int tid = threadIdx.x;
+	for (int64_t e = blockIdx.x; e < num; e += gridDim.x) {
+		const int64_t t = idx[e];
+		if (t < 0 || t >= g.N4) {
+			if (tid == 0) { out[e] = __builtin_nan(""); }
+			continue;
+		}
+		int k[3];
+		element_keys(g, lut, t, k[0], k[1], k[2]);
+		for (int a = 0; a < 3; ++a) {
+			for (int m = tid; m < Mp; m += blockDim.x) { col[a * Mp + m] = (m <= M) ? B[(int64_t)m * Cp + k[a]] : 0.0; }
+		}
+		__syncthreads();
+		if (tid < 3) {
+			int lo = M + 1, hi = -1;
+			for (int m = 0; m <= M; ++m) {
+				if (col[tid * Mp + m] != 0) { lo = min(lo, m); hi = m; }
+			}
+			win[tid][0] = lo;
+			win[tid][1] = hi;
+		}
+		__syncthreads();
+		const int n0 = win[0][1] - win[0][0] + 1, n1 = win[1][1] - win[1][0] + 1;
+		double acc = 0;
+		if (n0 > 0 && n1 > 0 && win[2][1] >= win[2][0]) {
+			const double* c0 = col;
+			const double* c1 = col + Mp;
+			const double* c2 = col + 2 * Mp;
+			for (int pq = tid; pq < n0 * n1; pq += blockDim.x) {
+				const int m1 = win[0][0] + pq / n1, m2 = win[1][0] + pq % n1;
+				const double w = c0[m1] * c1[m2];
+				if (w == 0) { continue; }
+				const int base = m1 * m1 + m2 * m2;
+				double s = 0;
+				for (int m3 = win[2][0]; m3 <= win[2][1]; ++m3) { s += c2[m3] * T[base + m3 * m3]; }
+				acc += w * s;
+			}
+		}
+		red[tid] = acc;
+		__syncthreads();
+		for (int w = 128; w > 0; w >>= 1) {
+			if (tid < w) { red[tid] += red[tid + w]; }
+			__syncthreads();
+		}
+		if (tid == 0) { out[e] = red[0]; }
+		__syncthreads();
+	}
+}
+
+// ---------------------------------------------------------------------------
+// host side: key tables, workspace, orchestration
+
+struct KeySet {
+	int R = 0;
+	bool symmetric = false;
+	int C1 = 0;
+	std::vector<int4> keys;  // (a, b, cd, 0) of each distinct folded column
+	std::vector<int> lut;    // (a * R + b) * (4R - 3) + cd + 2(R - 1) -> key id
+	DevBuf<int4> keys_dev;
+	DevBuf<int> lut_dev;
+};
+
+static KeySet& key_set(int R, bool symmetric)
+{
+	static std::map<std::pair<int, bool>, KeySet*> cache;
+	auto it = cache.find({R, symmetric});
+	if (it != cache.end()) { return *it->second; }
+	KeySet* ks = new KeySet();
+	ks->R = R;
+	ks->symmetric = symmetric;
+	const int W = 4 * R - 3;
+	ks->lut.assign((size_t)R * R * W, -1);
+	std::map<std::tuple<int, int, int>, int> ids;
+	// every per-axis 4-tuple with span < R (a translation class per key)
+	for (int i = 0; i < R; ++i) {
+		for (int j = 0; j < R; ++j) {
+			for (int k = 0; k < R; ++k) {
+				for (int l = 0; l < R; ++l) {
+					const int a = std::abs(i - j), b = std::abs(k - l), cd = (i + j) - (k + l);
+					std::tuple<int, int, int> canon = symmetric ? std::make_tuple(std::min(a, b), std::max(a, b), std::abs(cd))
+					                                            : std::make_tuple(a, b, cd);
+					auto f = ids.find(canon);
+					int id;
+					if (f == ids.end()) {
+						id = (int)ks->keys.size();
+						ids[canon] = id;
+						ks->keys.push_back(int4{std::get<0>(canon), std::get<1>(canon), std::get<2>(canon), 0});
+					}
+					else {
+						id = f->second;
+					}
+					ks->lut[(a * R + b) * W + cd + 2 * (R - 1)] = id;
+				}
+			}
+		}
+	}
+	ks->C1 = (int)ks->keys.size();
+	ks->keys_dev.alloc(ks->keys.size());
+	ks->lut_dev.alloc(ks->lut.size());
+	GLI_HIP(hipMemcpy(ks->keys_dev.p, ks->keys.data(), ks->keys.size() * sizeof(int4), hipMemcpyHostToDevice));
+	GLI_HIP(hipMemcpy(ks->lut_dev.p, ks->lut.data(), ks->lut.size() * sizeof(int), hipMemcpyHostToDevice));
+	cache[{R, symmetric}] = ks;
+	return *ks;
+}
+
+struct EriWorkspace {
+	DevBuf<double> coeffs, f, P, B, Y1, Y2, V;
+	DevBuf<double> T3;
+	int T3_M = -1;
+};
+
+static EriWorkspace& eri_ws()
+{
+	static EriWorkspace* w = new EriWorkspace();
+	return *w;
+}
+
+static bool mirror_symmetric(const double* c, int64_t K)
+{
+	for (int64_t i = 0; i < K; ++i) {
+		if (c[i] != c[K - 1 - i]) { return false; }
+	}
+	return true;
+}
+
+struct EriSetup {
+	int K, R, M, Mp, C1, Cp;
+	GridDev g;
+	const KeySet* ks;
+	const double* T;
+};
+
+// tables f, P, B (and T3) for a coefficient set on a grid; all on the library stream
+static EriSetup eri_tables(const double* coeffs, int64_t K, const int64_t grid[6], double tol)
+{
+	GLI_REQUIRE(K >= 3 && (K % 2) == 1, "the number of gausslet coefficients must be odd and at least 3");
+	GLI_REQUIRE(grid[1] > 0 && grid[3] > 0 && grid[5] > 0, "grid axes must have a positive number of points");
+	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
+	Context& c = ctx();
+	EriWorkspace& w = eri_ws();
+	EriSetup s;
+	s.K = (int)K;
+	s.R = (int)std::max(grid[1], std::max(grid[3], grid[5]));
+	s.M = 6 * (s.R - 1) + 2 * (s.K - 1);
+	s.Mp = (int)round_up(s.M + 1, 16);
+	GLI_REQUIRE(s.Mp <= 2048, "grid too long: the folded kernel range 6(n-1) + 2(K-1) must stay below 2048");
+	const bool sym = mirror_symmetric(coeffs, K);
+	const KeySet& ks = key_set(s.R, sym);
+	s.ks = &ks;
+	s.C1 = ks.C1;
+	s.Cp = (int)round_up(s.C1, GBM);
+	s.g.n[0] = (int)grid[1];
+	s.g.n[1] = (int)grid[3];
+	s.g.n[2] = (int)grid[5];
+	s.g.N = (int)(grid[1] * grid[3] * grid[5]);
+	s.g.R = s.R;
+	s.g.N2 = (int64_t)s.g.N * s.g.N;
+	s.g.N4 = s.g.N2 * s.g.N2;
+	const int64_t nT = 3 * (int64_t)s.M * s.M + 1;
+	s.T = eri_table(nT);
+
+	const int L = 2 * s.K - 1, LP = 4 * s.K - 3;
+	w.coeffs.reserve(K);
+	w.f.reserve((size_t)s.R * L);
+	w.P.reserve((size_t)s.R * s.R * LP);
+	w.B.reserve((size_t)s.Mp * s.Cp);
+	GLI_HIP(hipMemcpyAsync(w.coeffs.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, c.stream));
+	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, 0, c.stream>>>(w.coeffs.p, s.K, w.f.p);
+	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, 0, c.stream>>>(w.f.p, s.K, s.R, tol, w.P.p);
+	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, c.stream>>>(w.P.p, ks.keys_dev.p, s.C1, s.K, s.R, s.M, s.Cp, w.B.p);
+	GLI_HIP(hipGetLastError());
+	return s;
+}
+
+void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
+                         double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base)
+{
+	if (P <= 0 || C <= 0) { return; }
+	dim3 grid((unsigned)ceil_div(P, GBM), (unsigned)ceil_div(C, GBN));
+	GLI_REQUIRE(grid.x < 2147483647u, "mode product too large");
+	k_mode_product<<<grid, 256, 0, ctx().stream>>>(X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows, p_base);
+	GLI_HIP(hipGetLastError());
+}
+
+// values of device-resident four-indices into device-resident values
+static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_idx, int64_t num,
+                           double tol, int method, double* d_vals)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	GLI_REQUIRE(num >= 0, "negative number of indices");
+	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64 || method == GLI_ERI_DIRECT_FP64 || method == GLI_ERI_TENSOR_MIXED,
+	            "unknown ERI method");
+	{
+		const double N = (double)grid[1] * grid[3] * grid[5];
+		GLI_REQUIRE(N * N * N * N < 9007199254740992.0, "grid too large: four-indices must stay below 2^53");
+	}
+	phase_begin();
+	EriSetup s = eri_tables(coeffs, K, grid, tol);
+	EriWorkspace& w = eri_ws();
+	phase_mark("tables");
+	if (num == 0) { return; }
+	const int64_t Mp = s.Mp, Cp = s.Cp;
+
+	if (method == GLI_ERI_DIRECT_FP64) {
+		const int64_t blocks = std::min<int64_t>(num, 65536);
+		const size_t shm = (3 * (size_t)s.Mp + 256) * sizeof(double);
+		k_eri_direct<<<(unsigned)blocks, 256, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.B.p, s.Cp, s.M, s.Mp, s.T, d_vals);
+		GLI_HIP(hipGetLastError());
+		phase_mark("direct");
+		return;
+	}
+	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64, "the split-precision ERI method is not available in this build");
+
+	// kernel tensor, cached per folded range
+	if (w.T3_M != s.M) {
+		w.T3.reserve((size_t)Mp * Mp * Mp);
+		k_kernel_tensor<<<dim3((unsigned)ceil_div(Mp, 64), (unsigned)Mp, (unsigned)Mp), 64, 0, c.stream>>>(s.T, s.M, s.Mp, w.T3.p);
+		GLI_HIP(hipGetLastError());
+		w.T3_M = s.M;
+	}
+	phase_mark("kernel_tensor");
+	// Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]
+	w.Y1.reserve((size_t)Mp * Mp * Cp);
+	mode_product(w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0);
+	phase_mark("mode1");
+	// Y2[m3][k1][k2] = sum_m2 Y1[m2][m3][k1] B[m2][k2]   (k2 >= k1 only)
+	w.Y2.reserve((size_t)Mp * Cp * Cp);
+	mode_product(w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0);
+	phase_mark("mode2");
+	// V[k1][k2][k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3]    (k1 <= k2 <= k3 only), in row chunks of k1
+	size_t free_b = 0, total_b = 0;
+	GLI_HIP(hipMemGetInfo(&free_b, &total_b));
+	const int64_t row_bytes = Cp * Cp * (int64_t)sizeof(double);
+	const int64_t have = (int64_t)w.V.n * (int64_t)sizeof(double);
+	int64_t budget = std::max<int64_t>(have, (int64_t)(free_b * 0.6) + have);
+	int64_t rows = std::min<int64_t>(s.C1, std::max<int64_t>(1, budget / row_bytes));
+	w.V.reserve((size_t)(rows * Cp * Cp));
+	for (int64_t lo = 0; lo < s.C1; lo += rows) {
+		const int64_t hi = std::min<int64_t>(s.C1, lo + rows);
+		mode_product(w.Y2.p + lo * Cp, Cp * Cp, (hi - lo) * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, Cp, s.Cp, 1, 1, lo * Cp);
+		if (lo == 0) { phase_mark("mode3"); }
+		k_gather<<<(unsigned)ceil_div(num, 256), 256, 0, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.V.p, s.Cp,
+		                                                             (int)lo, (int)hi, d_vals);
+		GLI_HIP(hipGetLastError());
+		if (lo == 0) { phase_mark("gather"); }
+	}
+}
+
+}  // namespace gli
+
+using namespace gli;
+
+extern "C" {
+
+int gli_eri_values_device(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_idx, int64_t num,
+                          double tol, int method, double* d_values)
+{
+	try {
+		eri_values_dev(coeffs, K, grid, d_idx, num, tol, method, d_values);
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_hip_eri_values(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* indices, int64_t num,
+                       double tol, int method, double* values)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		if (num == 0) { return 0; }
+		DevBuf<int64_t> di(num);
+		DevBuf<double> dv(num);
+		GLI_HIP(hipMemcpyAsync(di.p, indices, num * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+		eri_values_dev(coeffs, K, g, di.p, num, tol, method, dv.p);
+		GLI_HIP(hipMemcpyAsync(values, dv.p, num * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,
+                                const int64_t* const* d_idx, const int64_t* num, double tol, int method, double* const* d_values)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		for (int b = 0; b < num_grids; ++b) {
+			eri_values_dev(coeffs[b], K[b], grids + 6 * b, d_idx[b], num[b], tol, method, d_values[b]);
+		}
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_mode_product_device(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int64_t C, int64_t Kd,
+                            double* Y, int64_t ldy)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		GLI_REQUIRE(P >= 0 && C >= 0 && Kd >= 0 && ldx >= P && ldb >= C && ldy >= C, "inconsistent mode-product shapes");
+		mode_product(X, ldx, P, B, ldb, (int)C, (int)Kd, Y, ldy, 0, 0, 0, 0);
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+// number of distinct folded columns (axis keys) the contraction uses on an R-point axis
+int gli_eri_key_count(int64_t R, int symmetric)
+{
+	try {
+		return key_set((int)R, symmetric != 0).C1;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+void gli_release_workspace(void)
+{
+	EriWorkspace& w = eri_ws();
+	w.coeffs.release(); w.f.release(); w.P.release(); w.B.release();
+	w.Y1.release(); w.Y2.release(); w.V.release(); w.T3.release();
+	w.T3_M = -1;
+}
+
+}  // extern "C"

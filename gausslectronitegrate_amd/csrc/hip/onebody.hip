@@ -1,0 +1,379 @@
+// Nuclear attraction and kinetic matrices, and ERIDA values.
+//
+// Nuclear (nuclear_integrals.c:133-266): the reference sums, per orbital pair,
+// f_x f_y f_z * sum_n Z_n erf(|r - R_n| / w) / |r - R_n| at r = (p_i + p_j)/2 + m/6.
+// Every such r lies on the lattice u/6 (u = 3(p_i + p_j) + m per axis), so the
+// potential W[u] is evaluated once per lattice point and the matrix is the
+// three mode products  V = W x1 Fx^T x2 Fy^T x3 Fz^T  with
+// F_a[u][(i_a, j_a)] = f'_{i_a - j_a}[u - 3(p_i + p_j)_a]  (f' = f * [|f| > tol]).
+// The lower triangle mirrors the upper one (nuclear_integrals.c:257-263).
+//
+// Kinetic (kinetic_integrals.c:14-121): K x I x I + I x K x I + I x I x K from
+// the 1D table t(s) = 1/2 sum c_a c_b (3 sqrt(pi)/2)(1 - d^2/2) e^{-d^2/4}, d = 3s + a - b.
+//
+// ERIDA (erida_integrals.c:44-100, 429-542): per element, the folded columns
+// b_a[mu] = F'[mu - 3 s_a] + F'[-mu - 3 s_a] of F[d] = (2 pi / 9) sum_{a-b=d} c_a c_b
+// contracted with the ERIDA kernel table, one workgroup per element.
+#include "common.hpp"
+#include "../gli_device.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace gli {
+
+// Euclidean norm scaled by the largest component (util.h:69-117)
+__device__ inline double norm3(double v0, double v1, double v2)
+{
+	const double a = fabs(v0), b = fabs(v1), c = fabs(v2);
+	double big, s0, s1;
+	if (b <= c) {
+		if (a <= c) {
+			if (c == 0) { return 0; }
+			big = c; s0 = a; s1 = b;
+		}
+		else { big = a; s0 = b; s1 = c; }
+	}
+	else {
+		if (a <= b) { big = b; s0 = a; s1 = c; }
+		else { big = a; s0 = b; s1 = c; }
+	}
+	const double r0 = s0 / big, r1 = s1 / big;
+	return big * sqrt(1 + r0 * r0 + r1 * r1);
+}
+
+struct Nuclei {
+	int num;
+	double pos[3 * 64];
+	double charge[64];
+};
+
+__global__ void k_nuclear_potential(Nuclei nuc, int ux0, int uy0, int uz0, int Ux, int Uy, int Uz, double* __restrict__ W)
+{
+	const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+	const int iy = blockIdx.y, ix = blockIdx.z;
+	if (iz >= Uz) { return; }
+	const double rx = (ux0 + ix) / 6.0, ry = (uy0 + iy) / 6.0, rz = (uz0 + iz) / 6.0;
+	const double w = 1. / 3;
+	double val = 0;
+	for (int k = 0; k < nuc.num; ++k) {
+		const double d = norm3(rx - nuc.pos[3 * k], ry - nuc.pos[3 * k + 1], rz - nuc.pos[3 * k + 2]);
+		const double g = (d > 0) ? erf(d / w) / d : 2 / (sqrt(M_PI) * w);
+		val += nuc.charge[k] * g;
+	}
+	W[((int64_t)ix * Uy + iy) * Uz + iz] = val;
+}
+
+// f'[s][m] for s = 0..S-1, zeroed where |f| <= tol (gausslet_factors.c:12-77 with the skips of
+// nuclear_integrals.c:210-228 folded in)
+__global__ void k_factors_tol(const double* __restrict__ c, int K, double tol, double* __restrict__ f)
+{
+	const int L = 2 * K - 1;
+	const int s = blockIdx.y;
+	const int m = blockIdx.x * blockDim.x + threadIdx.x;
+	if (m >= L) { return; }
+	const double prefac = 0.590817950301838675766;
+	double acc = 0;
+	const int ilo = max(0, m - (K - 1)), ihi = min(K - 1, m);
+	for (int i = ilo; i <= ihi; ++i) {
+		const int j = m - i;
+		const double x = 0.5 * (3 * s + (i - j));
+		acc += prefac * c[i] * c[j] * exp(-x * x);
+	}
+	f[(int64_t)s * L + m] = (fabs(acc) > tol) ? acc : 0.0;
+}
+
+// F_a[u][q], q = i*n + j over axis offsets, u over the lattice of the axis
+__global__ void k_pair_factors(const double* __restrict__ f, int K, int n, int g0, int u0, int U, double* __restrict__ F)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	const int u = blockIdx.y;
+	const int C = n * n;
+	if (q >= C) { return; }
+	const int i = q / n, j = q % n;
+	const int m = (u0 + u) - 3 * ((g0 + i) + (g0 + j));
+	double v = 0;
+	if (m >= -(K - 1) && m <= K - 1) { v = f[(int64_t)abs(i - j) * (2 * K - 1) + m + (K - 1)]; }
+	F[(int64_t)u * C + q] = v;
+}
+
+// ngi[I][J] from V[(qx*Cy + qy)*Cz + qz], mirrored from the upper triangle
+__global__ void k_nuclear_scatter(const double* __restrict__ V, int nx, int ny, int nz, double* __restrict__ out)
+{
+	const int64_t N = (int64_t)nx * ny * nz;
+	const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= N * N) { return; }
+	int64_t I = e / N, J = e % N;
+	if (I > J) { const int64_t t = I; I = J; J = t; }
+	const int iz = (int)(I % nz), iy = (int)((I / nz) % ny), ix = (int)(I / ((int64_t)nz * ny));
+	const int jz = (int)(J % nz), jy = (int)((J / nz) % ny), jx = (int)(J / ((int64_t)nz * ny));
+	const int64_t qx = ix * nx + jx, qy = iy * ny + jy, qz = iz * nz + jz;
+	out[e] = V[(qx * ((int64_t)ny * ny) + qy) * ((int64_t)nz * nz) + qz];
+}
+
+static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], const double* pos, const int* charges,
+                           int num_nuclei, double tol, double* out)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	GLI_REQUIRE(K >= 3 && (K % 2) == 1, "the number of gausslet coefficients must be odd and at least 3");
+	GLI_REQUIRE(g[1] > 0 && g[3] > 0 && g[5] > 0, "grid axes must have a positive number of points");
+	GLI_REQUIRE(num_nuclei >= 0 && num_nuclei <= 64, "between 0 and 64 nuclei are supported");
+	hipStream_t st = c.stream;
+	const int n[3] = { (int)g[1], (int)g[3], (int)g[5] };
+	const int g0[3] = { (int)g[0], (int)g[2], (int)g[4] };
+	const int R = std::max(n[0], std::max(n[1], n[2]));
+	const int L = 2 * (int)K - 1;
+	int u0[3], U[3], C[3];
+	for (int a = 0; a < 3; ++a) {
+		u0[a] = 6 * g0[a] - ((int)K - 1);
+		U[a] = 6 * (n[a] - 1) + 2 * ((int)K - 1) + 1;
+		C[a] = n[a] * n[a];
+	}
+	Nuclei nuc;
+	nuc.num = num_nuclei;
+	for (int k = 0; k < num_nuclei; ++k) {
+		for (int a = 0; a < 3; ++a) { nuc.pos[3 * k + a] = pos[3 * k + a]; }
+		nuc.charge[k] = (double)charges[k];
+	}
+	DevBuf<double> dc(K), f((size_t)R * L), W((size_t)U[0] * U[1] * U[2]);
+	DevBuf<double> F0((size_t)U[0] * C[0]), F1((size_t)U[1] * C[1]), F2((size_t)U[2] * C[2]);
+	DevBuf<double> Y1((size_t)U[1] * U[2] * C[0]), Y2((size_t)U[2] * C[0] * C[1]), V((size_t)C[0] * C[1] * C[2]);
+	const int64_t N = (int64_t)n[0] * n[1] * n[2];
+	DevBuf<double> M(N * N);
+	GLI_HIP(hipMemcpyAsync(dc.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, st));
+	k_factors_tol<<<dim3((unsigned)ceil_div(L, 128), R), 128, 0, st>>>(dc.p, (int)K, tol, f.p);
+	k_nuclear_potential<<<dim3((unsigned)ceil_div(U[2], 64), U[1], U[0]), 64, 0, st>>>(nuc, u0[0], u0[1], u0[2], U[0], U[1], U[2], W.p);
+	double* Fs[3] = { F0.p, F1.p, F2.p };
+	for (int a = 0; a < 3; ++a) {
+		k_pair_factors<<<dim3((unsigned)ceil_div(C[a], 64), U[a]), 64, 0, st>>>(f.p, (int)K, n[a], g0[a], u0[a], U[a], Fs[a]);
+	}
+	GLI_HIP(hipGetLastError());
+	// W[ux][uy][uz] -> Y1[uy][uz][qx] -> Y2[uz][qx][qy] -> V[qx][qy][qz]
+	mode_product(W.p, (int64_t)U[1] * U[2], (int64_t)U[1] * U[2], F0.p, C[0], C[0], U[0], Y1.p, C[0], 0, 0, 0, 0);
+	mode_product(Y1.p, (int64_t)U[2] * C[0], (int64_t)U[2] * C[0], F1.p, C[1], C[1], U[1], Y2.p, C[1], 0, 0, 0, 0);
+	mode_product(Y2.p, (int64_t)C[0] * C[1], (int64_t)C[0] * C[1], F2.p, C[2], C[2], U[2], V.p, C[2], 0, 0, 0, 0);
+	k_nuclear_scatter<<<(unsigned)ceil_div(N * N, 256), 256, 0, st>>>(V.p, n[0], n[1], n[2], M.p);
+	GLI_HIP(hipGetLastError());
+	GLI_HIP(hipMemcpyAsync(out, M.p, N * N * sizeof(double), hipMemcpyDeviceToHost, st));
+	GLI_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// kinetic
+
+__global__ void k_kinetic_1d(const double* __restrict__ c, int K, double* __restrict__ t)
+{
+	__shared__ double red[256];
+	const int s = blockIdx.x;
+	double acc = 0;
+	for (int q = threadIdx.x; q < K * K; q += blockDim.x) {
+		const int i = q / K, j = q % K;
+		const int d = 3 * s + (i - j);
+		const double d2 = (double)d * d;
+		acc += 0.5 * c[i] * c[j] * (1.5 * sqrt(M_PI) * (1 - 0.5 * d2) * exp(-0.25 * d2));
+	}
+	red[threadIdx.x] = acc;
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if (threadIdx.x < (unsigned)w) { red[threadIdx.x] += red[threadIdx.x + w]; }
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) { t[s] = red[0]; }
+}
+
+__global__ void k_kinetic_fill(const double* __restrict__ t, int nx, int ny, int nz, double* __restrict__ out)
+{
+	const int64_t N = (int64_t)nx * ny * nz;
+	const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= N * N) { return; }
+	const int64_t I = e / N, J = e % N;
+	const int iz = (int)(I % nz), iy = (int)((I / nz) % ny), ix = (int)(I / ((int64_t)nz * ny));
+	const int jz = (int)(J % nz), jy = (int)((J / nz) % ny), jx = (int)(J / ((int64_t)nz * ny));
+	const double ex = (ix == jx) ? 1 : 0, ey = (iy == jy) ? 1 : 0, ez = (iz == jz) ? 1 : 0;
+	out[e] = t[abs(ix - jx)] * ey * ez + ex * t[abs(iy - jy)] * ez + ex * ey * t[abs(iz - jz)];
+}
+
+static void kinetic_matrix(const double* coeffs, int64_t K, const int64_t g[6], double* out)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	GLI_REQUIRE(K >= 1, "at least one gausslet coefficient is required");
+	GLI_REQUIRE(g[1] > 0 && g[3] > 0 && g[5] > 0, "grid axes must have a positive number of points");
+	hipStream_t st = c.stream;
+	const int R = (int)std::max(g[1], std::max(g[3], g[5]));
+	const int64_t N = g[1] * g[3] * g[5];
+	DevBuf<double> dc(K), t(R), M(N * N);
+	GLI_HIP(hipMemcpyAsync(dc.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, st));
+	k_kinetic_1d<<<R, 256, 0, st>>>(dc.p, (int)K, t.p);
+	k_kinetic_fill<<<(unsigned)ceil_div(N * N, 256), 256, 0, st>>>(t.p, (int)g[1], (int)g[3], (int)g[5], M.p);
+	GLI_HIP(hipGetLastError());
+	GLI_HIP(hipMemcpyAsync(out, M.p, N * N * sizeof(double), hipMemcpyDeviceToHost, st));
+	GLI_HIP(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// ERIDA
+
+// F[d + K - 1] = (2 pi / 9) sum_{i - j = d} c_i c_j, zeroed where |F| <= tol
+__global__ void k_erida_factors(const double* __restrict__ c, int K, double tol, double* __restrict__ F)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= 2 * K - 1) { return; }
+	const int d = q - (K - 1);
+	const double prefac = 0.69813170079773183077;
+	double acc = 0;
+	for (int i = max(0, d); i <= min(K - 1, K - 1 + d); ++i) { acc += prefac * c[i] * c[i - d]; }
+	F[q] = (fabs(acc) > tol) ? acc : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_erida_values(const int64_t* __restrict__ idx, int64_t num, int nx, int ny, int nz,
+	const double* __restrict__ F, int K, int M, int Mp, const double* __restrict__ T, double* __restrict__ out)
+{
+	extern __shared__ double smem[];
+	double* col = smem;           // 3 x Mp
+	double* red = smem + 3 * Mp;  // 256
+	__shared__ int win[3][2];
+	const int tid = threadIdx.x;
+	const int64_t N = (int64_t)nx * ny * nz;
+	for (int64_t e = blockIdx.x; e < num; e += gridDim.x) {
+		const int64_t i = idx[e] / N, j = idx[e] % N;
+		int s[3];
+		s[2] = (int)(i % nz) - (int)(j % nz);
+		s[1] = (int)((i / nz) % ny) - (int)((j / nz) % ny);
+		s[0] = (int)(i / ((int64_t)nz * ny)) - (int)(j / ((int64_t)nz * ny));
+		for (int a = 0; a < 3; ++a) {
+			for (int m = tid; m < Mp; m += blockDim.x) {
+				double v = 0;
+				if (m <= M) {
+					const int t1 = m - 3 * s[a] + (K - 1);
+					if (t1 >= 0 && t1 < 2 * K - 1) { v += F[t1]; }
+					if (m > 0) {
+						const int t2 = -m - 3 * s[a] + (K - 1);
+						if (t2 >= 0 && t2 < 2 * K - 1) { v += F[t2]; }
+					}
+				}
+				col[a * Mp + m] = v;
+			}
+		}
+		__syncthreads();
+		if (tid < 3) {
+			int lo = M + 1, hi = -1;
+			for (int m = 0; m <= M; ++m) {
+				if (col[tid * Mp + m] != 0) { lo = min(lo, m); hi = m; }
+			}
+			win[tid][0] = lo;
+			win[tid][1] = hi;
+		}
+		__syncthreads();
+		const int n0 = win[0][1] - win[0][0] + 1, n1 = win[1][1] - win[1][0] + 1;
+		double acc = 0;
+		if (n0 > 0 && n1 > 0 && win[2][1] >= win[2][0]) {
+			for (int pq = tid; pq < n0 * n1; pq += blockDim.x) {
+				const int m1 = win[0][0] + pq / n1, m2 = win[1][0] + pq % n1;
+				const double w = col[m1] * col[Mp + m2];
+				if (w == 0) { continue; }
+				const int base = m1 * m1 + m2 * m2;
+				double sum = 0;
+				for (int m3 = win[2][0]; m3 <= win[2][1]; ++m3) { sum += col[2 * Mp + m3] * T[base + m3 * m3]; }
+				acc += w * sum;
+			}
+		}
+		red[tid] = acc;
+		__syncthreads();
+		for (int w = 128; w > 0; w >>= 1) {
+			if (tid < w) { red[tid] += red[tid + w]; }
+			__syncthreads();
+		}
+		if (tid == 0) { out[e] = red[0]; }
+		__syncthreads();
+	}
+}
+
+static void erida_values_dev(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* d_idx, int64_t num, double tol, double* d_vals)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	GLI_REQUIRE(K >= 1 && (K % 2) == 1, "the number of gausslet coefficients must be odd");
+	GLI_REQUIRE(g[1] > 0 && g[3] > 0 && g[5] > 0, "grid axes must have a positive number of points");
+	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
+	hipStream_t st = c.stream;
+	const int R = (int)std::max(g[1], std::max(g[3], g[5]));
+	const int M = 3 * (R - 1) + ((int)K - 1);
+	const int Mp = (int)round_up(M + 1, 16);
+	const double* T = erida_table(3 * (int64_t)M * M + 1);
+	DevBuf<double> dc(K), F(2 * K - 1);
+	GLI_HIP(hipMemcpyAsync(dc.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, st));
+	k_erida_factors<<<(unsigned)ceil_div(2 * K - 1, 128), 128, 0, st>>>(dc.p, (int)K, tol, F.p);
+	if (num > 0) {
+		const size_t shm = (3 * (size_t)Mp + 256) * sizeof(double);
+		k_erida_values<<<(unsigned)std::min<int64_t>(num, 65536), 256, shm, st>>>(d_idx, num, (int)g[1], (int)g[3], (int)g[5],
+		                                                                        F.p, (int)K, M, Mp, T, d_vals);
+	}
+	GLI_HIP(hipGetLastError());
+	GLI_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace gli
+
+using namespace gli;
+
+extern "C" {
+
+int gli_hip_nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], const double* pos, const int* charges,
+                           int num_nuclei, double tol, double* out)
+{
+	try {
+		nuclear_matrix(coeffs, K, g, pos, charges, num_nuclei, tol, out);
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_hip_kinetic_matrix(const double* coeffs, int64_t K, const int64_t g[6], double* out)
+{
+	try {
+		kinetic_matrix(coeffs, K, g, out);
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_erida_values_device(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* d_idx, int64_t num,
+                            double tol, double* d_values)
+{
+	try {
+		erida_values_dev(coeffs, K, g, d_idx, num, tol, d_values);
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_hip_erida_values(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* indices, int64_t num,
+                         double tol, double* values)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		DevBuf<int64_t> di(std::max<int64_t>(num, 1));
+		DevBuf<double> dv(std::max<int64_t>(num, 1));
+		if (num > 0) { GLI_HIP(hipMemcpyAsync(di.p, indices, num * sizeof(int64_t), hipMemcpyHostToDevice, c.stream)); }
+		erida_values_dev(coeffs, K, g, di.p, num, tol, dv.p);
+		if (num > 0) {
+			GLI_HIP(hipMemcpyAsync(values, dv.p, num * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+			GLI_HIP(hipStreamSynchronize(c.stream));
+		}
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+#pragma once
+/*
+ * Additive API of the MI355X build.  None of these entry points exists in the
+ * reference; they select the evaluation method, report errors of the GPU path
+ * and expose device-resident variants for callers that keep their index lists
+ * and values in HBM.  The reference entry points keep their behaviour: they
+ * use FP64 by default and abort with a message when the GPU path fails.
+ *
+ * Device pointers below are HIP device memory of the current device; the
+ * library works on one internal stream at default priority (gli_stream()).
+ */
+#include <stdint.h>
+#include "grid.h"
+#include "gausslet.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ERI value methods */
+#define GLI_ERI_TENSOR_FP64  0  /* key-tensor contraction in FP64 (default, exact re-association) */
+#define GLI_ERI_TENSOR_MIXED 1  /* key-tensor contraction, split-precision MFMA bulk, FP64 final sums */
+#define GLI_ERI_DIRECT_FP64  2  /* per-element FP64 evaluation, no reuse across elements */
+
+/* method used by compute_sparse_eri_gausslet_integrals / compute_eri_gausslet_integral;
+ * the environment variable GLI_ERI_METHOD (tensor | mixed | direct) sets the initial value */
+int gli_set_eri_method(int method);
+int gli_get_eri_method(void);
+
+/* error reporting of the GPU path: status of the last failing call on this thread */
+const char* gli_last_error(void);
+int gli_last_status(void);
+void gli_clear_error(void);
+
+/* the library stream (hipStream_t) and a full synchronisation of it */
+void* gli_stream(void);
+int gli_synchronize(void);
+
+/* phase timing of the last ERI evaluation (HIP events on the library stream) */
+int gli_enable_phase_timing(int on);
+int gli_phase_times(double* ms, const char** names, int max);
+
+/* device memory helpers; kind: 0 host->device, 1 device->host, 2 device->device */
+int gli_device_malloc(void** p, int64_t bytes);
+void gli_device_free(void* p);
+int gli_memcpy(void* dst, const void* src, int64_t bytes, int kind);
+
+/* device-resident enumeration of a symmetric n^3 grid; *d_out is released with gli_device_free */
+int gli_eri_enumerate_device(int64_t n, int64_t** d_out, int64_t* num);
+int gli_erida_enumerate_device(int64_t n, int64_t** d_out, int64_t* num);
+
+/* device-resident values; grid = {istart_x, num_x, istart_y, num_y, istart_z, num_z};
+ * coeffs is host memory (K values); indices outside the grid produce NaN */
+int gli_eri_values_device(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_indices,
+                          int64_t num, double tol, int method, double* d_values);
+int gli_erida_values_device(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_indices,
+                            int64_t num, double tol, double* d_values);
+
+/* several grids in one call (for example 5^3, 7^3, 9^3 and 11^3 with different K) */
+int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,
+                                const int64_t* const* d_indices, const int64_t* num, double tol, int method,
+                                double* const* d_values);
+
+/* Y[p][k] = sum_{mu < Kd} X[mu*ldx + p] * B[mu*ldb + k] (FP64 MFMA; device pointers, synchronous) */
+int gli_mode_product_device(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int64_t C, int64_t Kd,
+                            double* Y, int64_t ldy);
+
+/* number of distinct folded axis columns of an R-point axis (symmetric: mirror-symmetric coefficients) */
+int gli_eri_key_count(int64_t R, int symmetric);
+
+/* release cached device workspace */
+void gli_release_workspace(void);
+
+#ifdef __cplusplus
+}
+#endif

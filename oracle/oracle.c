@@ -859,3 +859,43 @@ int orc_num_threads(void)
 	return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------- */
+/* canonical key of each four-index of a symmetric grid: minimum over the bias
+ * placements of its (centred) box and the O_h x label orbit
+ * (pymodule.c:700-780 lookup route, eri_integrals.c:240-291)                 */
+
+void orc_eri_canonical_keys(const glong* g, const glong* idx, glong num, glong* out)
+{
+	const glong N = g_num(g);
+	glong* perm = malloc(48 * N * sizeof(glong));
+	orc_octahedral_perms(g, perm);
+	#pragma omp parallel for schedule(static)
+	for (glong e = 0; e < num; ++e) {
+		glong r = idx[e];
+		const glong lin[4] = { r / (N * N * N), (r / (N * N)) % N, (r / N) % N, r % N };
+		glong pt[4][3];
+		for (int p = 0; p < 4; ++p) { g_pt(g, lin[p], pt[p]); }
+		glong tr[3], bi[3];
+		for (int a = 0; a < 3; ++a) {
+			const glong c = lmn(lmn(pt[0][a], pt[1][a]), lmn(pt[2][a], pt[3][a])) + lmx(lmx(pt[0][a], pt[1][a]), lmx(pt[2][a], pt[3][a]));
+			tr[a] = c / 2;
+			bi[a] = c - 2 * tr[a];
+		}
+		glong best = INT64_MAX;
+		for (glong bx = 0; bx <= labs(bi[0]); ++bx)
+		for (glong by = 0; by <= labs(bi[1]); ++by)
+		for (glong bz = 0; bz <= labs(bi[2]); ++bz) {
+			const glong b[3] = { bx, by, bz };
+			glong q[4];
+			for (int p = 0; p < 4; ++p) {
+				glong o[3];
+				for (int a = 0; a < 3; ++a) { o[a] = pt[p][a] - tr[a] - b[a] * bi[a] - g[2 * a]; }
+				q[p] = g_lin(g, o[0], o[1], o[2]);
+			}
+			best = lmn(best, orc_min_orbit_eri(N, perm, q[0], q[1], q[2], q[3]));
+		}
+		out[e] = best;
+	}
+	free(perm);
+}

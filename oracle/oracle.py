@@ -61,6 +61,7 @@ def lib():
         L.orc_fill_dense_eri.argtypes = [_pl, _pl, _pd, _i64, _pl, _pd]
         L.orc_fill_dense_erida.argtypes = [_pl, _pl, _pd, _i64, _pl, _pd]
         L.orc_num_threads.restype = C.c_int
+        L.orc_eri_canonical_keys.argtypes = [_pl, _pl, _i64, _pl]
         _lib = L
     return _lib
 
@@ -230,3 +231,10 @@ def fill_dense_erida(grid_sparse, keys, vals, grid_dense):
 
 def num_threads():
     return int(lib().orc_num_threads())
+
+
+def eri_canonical_keys(grid, indices):
+    idx = np.ascontiguousarray(indices, dtype=np.int64)
+    out = np.zeros(len(idx), dtype=np.int64)
+    lib().orc_eri_canonical_keys(_g(grid), idx, len(idx), out)
+    return out

@@ -1,0 +1,303 @@
+"""GPU parity of the HIP path against the reference fixtures, the golden
+vectors of the reference build, the oracle, and size-independent invariants
+at the full workload sizes.  Every call goes through the C ABI of
+lib/libgli_amd.so.
+
+Tolerances: integer results (index lists) are bit-exact; FP64 values are
+compared with the reference tests' own bounds (1e-13 ERI/nuclear/kinetic,
+1e-12 ERIDA dense, test/test_*.c).
+"""
+import ctypes as C
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import gausslectronitegrate_amd as gli
+from gausslectronitegrate_amd import _abi, _lib
+from tests import burnside
+from tests.conftest import golden
+from tests.gpu_helpers import DevArray, eri_enumerate_device, eri_values_device
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def L():
+    return _lib.require_gpu()
+
+
+def sym(n):
+    h = (n - 1) // 2
+    return ((-h, n), (-h, n), (-h, n))
+
+
+# ---------------------------------------------------------------------------
+# the MFMA mode product (pins the FP64 16x16x4 accumulator layout)
+
+@pytest.mark.parametrize("Kd,P,Cc", [(4, 16, 16), (17, 70, 33), (176, 300, 130)])
+def test_mode_product_vs_numpy(L, Kd, P, Cc):
+    say("mode product", Kd, P, Cc)
+    rng = np.random.default_rng(Kd + P)
+    X = rng.standard_normal((Kd, P))
+    B = rng.standard_normal((Kd, Cc))
+    dX, dB, dY = DevArray.from_numpy(X), DevArray.from_numpy(B), DevArray(8 * P * Cc)
+    _lib.check(L.gli_mode_product_device(dX.ptr, P, P, dB.ptr, Cc, Cc, Kd, dY.ptr, Cc), "mode product")
+    Y = dY.to_numpy(np.float64, P * Cc).reshape(P, Cc)
+    assert np.max(np.abs(Y - X.T @ B)) <= 1e-12 * max(1.0, np.abs(X.T @ B).max())
+
+
+# ---------------------------------------------------------------------------
+# enumeration
+
+def test_eri_enumeration_bit_exact_small(L):
+    say("enumerate 3, 5")
+    g = gli.SparseERI  # noqa: F841 (surface import check)
+    ref3 = golden("ref_eri_3x3x3.npz")["four_indices"]
+    d, n = eri_enumerate_device(3)
+    assert np.array_equal(d.to_numpy(np.int64, n), ref3)
+    ref5 = golden("golden_enum_n5.npz")["eri_indices"]
+    d, n = eri_enumerate_device(5)
+    assert np.array_equal(d.to_numpy(np.int64, n), ref5)
+
+
+def test_eri_enumeration_7_complete_and_canonical(L, orc):
+    say("enumerate 7")
+    d, n = eri_enumerate_device(7)
+    got = d.to_numpy(np.int64, n)
+    assert n == burnside.orbit_count(7)
+    assert np.all(np.diff(got) > 0)
+    assert np.array_equal(orc.eri_canonical_keys(sym(7), got), got)
+
+
+@pytest.mark.parametrize("n", [9, 11])
+def test_eri_enumeration_large_counts(L, orc, n):
+    """Full-size lists: Burnside count, strictly ascending, every sampled entry canonical."""
+    say("enumerate", n)
+    d, num = eri_enumerate_device(n)
+    assert num == burnside.orbit_count(n)
+    got = d.to_numpy(np.int64, num)
+    d.free()
+    assert got[0] >= 0 and got[-1] < (n ** 3) ** 4
+    assert np.all(np.diff(got) > 0)
+    rng = np.random.default_rng(n)
+    sample = np.sort(rng.choice(got, size=20000, replace=False))
+    assert np.array_equal(orc.eri_canonical_keys(sym(n), sample), sample)
+
+
+def test_eri_enumeration_13_count(L):
+    say("enumerate 13")
+    d, num = eri_enumerate_device(13)
+    assert num == burnside.orbit_count(13) == 1250296187
+    # spot-check order on a strided slice without copying 10 GB to the host
+    head = d.to_numpy(np.int64, 1 << 20)
+    assert np.all(np.diff(head) > 0)
+    d.free()
+
+
+def test_erida_enumeration(L):
+    for n in (3, 5, 7, 9, 11, 13):
+        e = gli.SparseERIDA(golden("coeffs_K29_s0.npy"), sym(n), 1e-3) if n == 3 else None
+        out = _abi.SparseEridaIndices()
+        g = _abi.Grid.of(sym(n))
+        L.enumerate_symmetry_reduced_erida_indices(C.byref(g), C.byref(out))
+        got = np.ctypeslib.as_array(out.two_indices, shape=(out.num,)).copy()
+        L.delete_sparse_erida_indices(C.byref(out))
+        assert len(got) == math.comb(n + 2, 3) == burnside.orbit_count(n, 2)
+        assert np.all(np.diff(got) > 0)
+        if n == 7:
+            assert np.array_equal(got, golden("golden_n7_K57s0.npz")["erida_indices"])
+
+
+# ---------------------------------------------------------------------------
+# ERI values
+
+def test_eri_3x3x3_fixture_through_python_surface(L):
+    """examples/eri_3x3x3.hdf5: G10, tol 1e-16, 1027 values; notebook values (eri_integrals.ipynb:59-192)."""
+    say("eri 3x3x3")
+    ref = golden("ref_eri_3x3x3.npz")
+    eri = gli.compute_sparse_eri_gausslet_integrals(ref["coeffs"], sym(3), 1e-16)
+    assert eri.num_entries == 1027
+    assert np.array_equal(eri.four_indices, ref["four_indices"])
+    assert np.max(np.abs(eri.integral_values - ref["values"])) <= 1e-13
+    assert eri.integral_value(([0, 0, 0], [0, 0, 0], [0, 0, 0], [0, 0, 0])) == pytest.approx(1.6252009000829197, abs=1e-13)
+    pts = ([1, 0, 0], [0, 0, -1], [0, 0, 1], [0, 0, 0])
+    assert eri.integral_value(pts) == pytest.approx(-0.0005864045861050727, abs=1e-13)
+    assert gli.compute_eri_gausslet_integral(ref["coeffs"], pts, 1e-16) == pytest.approx(-0.0005864045861050708, abs=1e-13)
+
+
+@pytest.mark.parametrize("method", [0, 2])
+def test_eri_dense_fixture(L, orc, method):
+    """test_eri_integrals.hdf5: sparse 3^3 (tol 1e-3) -> dense 12^4, <= 1e-13 (test_eri_integrals.c:12-85)."""
+    say("eri dense fixture, method", method)
+    ref = golden("ref_eri.npz")
+    tol = float(ref["tol"])
+    idx = golden("ref_eri_3x3x3.npz")["four_indices"]
+    vals = eri_values_device(ref["coeffs"], sym(3), idx, tol, method)
+    dense = orc.fill_dense_eri(sym(3), idx, vals, ((-1, 2), (0, 2), (-1, 3)))
+    assert np.max(np.abs(dense - ref["eri"])) <= 1e-13
+
+
+def test_eri_fill_tensor_and_single(L):
+    """fill_tensor of the GPU set vs the fixture, and single-element evaluation vs dense (test_eri_integrals.c:87-112)."""
+    ref = golden("ref_eri.npz")
+    eri = gli.compute_sparse_eri_gausslet_integrals(ref["coeffs"], sym(3), float(ref["tol"]))
+    dense = eri.fill_tensor(((-1, 2), (0, 2), (-1, 3)))
+    assert np.max(np.abs(dense - ref["eri"])) <= 1e-13
+    pts = [(x, y, z) for x in range(-1, 1) for y in range(0, 2) for z in range(-1, 2)]
+    rng = np.random.default_rng(2)
+    for _ in range(12):
+        q = rng.integers(0, 12, size=4)
+        v = gli.compute_eri_gausslet_integral(ref["coeffs"], [pts[i] for i in q], float(ref["tol"]))
+        assert abs(v - dense[tuple(q)]) <= 1e-13
+
+
+def test_eri_synthetic_3_full(L):
+    g = golden("golden_n3_K57s0.npz")
+    for method in (0, 2):
+        got = eri_values_device(g["coeffs"], sym(3), g["eri_indices"], float(g["tol"]), method)
+        assert np.max(np.abs(got - g["eri_values"])) <= 1e-13
+
+
+@pytest.mark.parametrize("name,n", [("golden_sample_n5_K29s0.npz", 5), ("golden_sample_n7_K43s0.npz", 7),
+                                    ("golden_sample_n9_K57s1.npz", 9), ("golden_sample_n11_K57s0.npz", 11),
+                                    ("golden_sample_n13_K57s0.npz", 13)])
+@pytest.mark.parametrize("method", [0, 2])
+def test_eri_golden_samples(L, name, n, method):
+    """seeded canonical samples with values of the reference build (tol 1e-16)."""
+    say("eri golden", name, "method", method)
+    g = golden(name)
+    got = eri_values_device(g["coeffs"], sym(n), g["eri_indices"], float(g["tol"]), method)
+    assert np.max(np.abs(got - g["eri_values"])) <= 1e-13
+
+
+def test_eri_nonsymmetric_coefficients(L, orc):
+    """coefficients without mirror symmetry use the general key set; compare with the oracle on a 5^3 sample."""
+    rng = np.random.default_rng(9)
+    c = rng.standard_normal(21) * np.exp(-(np.arange(21) - 10) ** 2 / 60.0)
+    idx = golden("golden_enum_n5.npz")["eri_indices"][::97]
+    ref = orc.eri_sparse(c, sym(5), idx, 1e-12)
+    for method in (0, 2):
+        got = eri_values_device(c, sym(5), idx, 1e-12, method)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def test_eri_arbitrary_grid_and_indices(L, orc):
+    """compute_sparse_* accepts any index list on any grid (eri_integrals.h:50-52)."""
+    c = golden("coeffs_K29_s0.npy")
+    grid = ((-2, 4), (0, 3), (1, 5))
+    N = 4 * 3 * 5
+    rng = np.random.default_rng(4)
+    idx = np.unique(rng.integers(0, N ** 4, size=300))
+    ref = orc.eri_sparse(c, grid, idx, 1e-10)
+    got = eri_values_device(c, grid, idx, 1e-10, 0)
+    assert np.max(np.abs(got - ref)) <= 1e-13
+    bad = eri_values_device(c, grid, np.array([-1, N ** 4], dtype=np.int64), 1e-10, 0)
+    assert np.all(np.isnan(bad))
+
+
+def test_eri_7_full_list_tensor_vs_direct_sample(L, orc):
+    """full 7^3 list on the tensor path; a seeded sample checked against the oracle."""
+    say("eri 7 full")
+    c = golden("coeffs_K57_s0.npy")
+    d, n = eri_enumerate_device(7)
+    idx = d.to_numpy(np.int64, n)
+    vals = eri_values_device(c, sym(7), idx, 1e-16, 0)
+    assert np.all(np.isfinite(vals))
+    rng = np.random.default_rng(77)
+    s = np.sort(rng.choice(n, size=300, replace=False))
+    ref = orc.eri_sparse(c, sym(7), idx[s], 1e-16)
+    assert np.max(np.abs(vals[s] - ref)) <= 1e-13
+
+
+# ---------------------------------------------------------------------------
+# ERIDA
+
+def test_erida_fixture_and_notebook(L, orc):
+    say("erida")
+    ref = golden("ref_erida.npz")
+    e = gli.compute_sparse_erida_gausslet_integrals(ref["coeffs"], sym(3), float(ref["tol"]))
+    dense = e.fill_matrix(((0, 2), (-1, 3), (-1, 2)))
+    assert np.max(np.abs(dense - ref["erida"])) <= 1e-12
+    g10 = golden("ref_eri.npz")["coeffs"]
+    assert gli.compute_erida_gausslet_integral(g10, ([0, 0, 0], [0, 0, 0]), 1e-16) == pytest.approx(2.4104836730129793, abs=1e-13)
+    assert gli.compute_erida_gausslet_integral(g10, ([-1, 0, 2], [0, 1, -1]), 1e-16) == pytest.approx(0.3024908490984703, abs=1e-13)
+    e7 = gli.compute_sparse_erida_gausslet_integrals(g10, sym(7), 1e-16)
+    assert e7.num_entries == 84
+    assert e7.integral_value(([-1, 0, 2], [0, 1, -1])) == pytest.approx(0.3024908490984723, abs=1e-13)
+
+
+def test_erida_golden_7(L):
+    g = golden("golden_n7_K57s0.npz")
+    e = gli.compute_sparse_erida_gausslet_integrals(g["coeffs"], sym(7), float(g["tol"]))
+    assert np.array_equal(e.two_indices, g["erida_indices"])
+    assert np.max(np.abs(e.integral_values - g["erida_values"])) <= 1e-13
+
+
+# ---------------------------------------------------------------------------
+# one-body
+
+def test_kinetic(L):
+    say("kinetic")
+    ref = golden("ref_kinetic.npz")
+    k = gli.compute_kinetic_gausslet_integrals(ref["coeffs"], ((0, 2), (1, 3), (-1, 3)))
+    assert np.max(np.abs(k - ref["kgi"])) <= 1e-13
+    g10 = golden("ref_eri.npz")["coeffs"]
+    kg = gli.compute_kinetic_gausslet_integrals(g10, ((-1, 3), (-2, 5), (-2, 4)))
+    assert kg[7, 7] == pytest.approx(5.218509050588766, abs=1e-13)
+    assert gli.compute_kinetic_gausslet_integral(g10, ([0, -2, -1], [0, -2, 1])) == pytest.approx(0.3010258456920169, abs=1e-13)
+    assert np.array_equal(kg, kg.T)
+    g = golden("golden_n3_K57s0.npz")
+    assert np.max(np.abs(gli.compute_kinetic_gausslet_integrals(g["coeffs"], sym(3)) - g["kinetic"])) <= 1e-13
+
+
+def test_nuclear(L):
+    say("nuclear")
+    ref = golden("ref_nuclear.npz")
+    m = gli.compute_nuclear_gausslet_integrals(ref["coeffs"], ((0, 2), (2, 2), (-1, 3)), ref["pos"], ref["charges"], float(ref["tol"]))
+    assert np.max(np.abs(m - ref["ngi"])) <= 1e-13
+    g10 = golden("ref_eri.npz")["coeffs"]
+    pos = [(-0.2, 1.3, 0.5), (0.7, 0, -2.1)]
+    ngi = gli.compute_nuclear_gausslet_integrals(g10, ((-1, 3), (0, 2), (-2, 4)), pos, [2, 1], 1e-16)
+    assert ngi[10, 10] == pytest.approx(1.8699450271662796, abs=1e-13)
+    assert np.array_equal(ngi, ngi.T)
+    v = gli.compute_nuclear_gausslet_integral(g10, ([-1, 0, -2], [0, 1, 1]), pos, [2, 1], 1e-16)
+    assert v == pytest.approx(-0.0003044157658974286, abs=1e-13)
+    g = golden("golden_n3_K57s0.npz")
+    m3 = gli.compute_nuclear_gausslet_integrals(g["coeffs"], sym(3), [[0, 0, 0]], [1], float(g["tol"]))
+    assert np.max(np.abs(m3 - g["nuclear"])) <= 1e-13
+
+
+def test_nuclear_golden_7(L):
+    """7^3, two unit charges at (0, 0, +-0.7): sampled entries, row sums and trace of the reference build."""
+    say("nuclear 7")
+    g = golden("golden_n7_K57s0.npz")
+    m = gli.compute_nuclear_gausslet_integrals(g["coeffs"], sym(7), g["nuc_pos"], g["nuc_charges"], float(g["tol"]))
+    assert np.max(np.abs(m[g["nuc_rows"], g["nuc_cols"]] - g["nuc_sample"])) <= 1e-13
+    assert np.max(np.abs(m.sum(axis=1) - g["nuc_rowsum"])) <= 1e-11
+    assert abs(np.trace(m) - g["nuc_trace"]) <= 1e-11
+    assert np.array_equal(m, m.T)
+
+
+# ---------------------------------------------------------------------------
+# post-processing through the surface (notebook checks, eri_integrals.ipynb:17-21)
+
+def test_project_and_apply(L):
+    ref = golden("ref_eri_3x3x3.npz")
+    eri = gli.compute_sparse_eri_gausslet_integrals(ref["coeffs"], sym(3), 1e-16)
+    rng = np.random.default_rng(293)
+    basis = np.linalg.qr(rng.standard_normal((27, 27)))[0][:, :4]
+    dense = eri.fill_tensor(sym(3))
+    proj = eri.project(basis)
+    ref_proj = np.einsum("ip,jq,kr,ls,ijkl->pqrs", basis, basis, basis, basis, dense, optimize=True)
+    assert np.linalg.norm(proj - ref_proj) <= 1e-12
+    states = rng.normal(size=(27 * 27, 3))
+    out = eri.apply(states)
+    vint = dense.transpose((0, 2, 1, 3)).reshape(27 * 27, 27 * 27)
+    assert np.linalg.norm(out - vint @ states) <= 1e-12

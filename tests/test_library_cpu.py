@@ -1,0 +1,86 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every entry point
+declared in include/*.h, keeps the reference struct layouts and its group
+tables agree with the O_h axioms.  No GPU computation is made here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from gausslectronitegrate_amd import _abi, _lib
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(REPO, "include")):
+        text = open(os.path.join(REPO, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "for", "while", "return", "sizeof"):
+                names.add(name)
+        for m in re.finditer(r"^extern const [^;]*?\b(\w+)\s*\[", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert "compute_sparse_eri_gausslet_integrals" in names and "gli_eri_values_device" in names
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_abi_struct_sizes():
+    for cls, size in _abi.ABI_SIZES.items():
+        assert ctypes.sizeof(cls) == size, cls.__name__
+
+
+def test_exported_group_tables():
+    L = _lib.lib()
+    M = (_abi.IMat3x3 * 48).in_dll(L, "octahedral_matrep")
+    T = ((ctypes.c_int * 48) * 48).in_dll(L, "octahedral_multiplication_table")
+    mats = np.array([[[M[e].data[r][c] for c in range(3)] for r in range(3)] for e in range(48)])
+    index = {m.tobytes(): e for e, m in enumerate(mats)}
+    assert len(index) == 48 and np.array_equal(mats[0], np.eye(3, dtype=np.int64))
+    for i in range(48):
+        for j in range(48):
+            assert index[(mats[i] @ mats[j]).tobytes()] == T[i][j]
+
+
+def test_group_tables_match_reference_build(ref_lib):
+    L = _lib.lib()
+    for name, typ in (("octahedral_matrep", _abi.IMat3x3 * 48), ("octahedral_multiplication_table", (ctypes.c_int * 48) * 48)):
+        a = bytes(typ.in_dll(L, name))
+        b = bytes(typ.in_dll(ref_lib, name))
+        assert a == b, name
+
+
+def test_host_helpers_match_reference_semantics(orc):
+    """grid permutation, orbit minimum and permutation helpers run on the host; compare with the oracle."""
+    L = _lib.lib()
+    g = _abi.Grid.of(orc.sym_grid(5))
+    arr = (_abi.P_glong * 48)()
+    L.evaluate_octahedral_grid_permutations(ctypes.byref(g), arr)
+    P = np.array([np.ctypeslib.as_array(arr[e], shape=(125,)) for e in range(48)])
+    assert np.array_equal(P, orc.octahedral_perms(orc.sym_grid(5)))
+    assert L.is_permutation(arr[17], 125)
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        i, j, k, l = (int(x) for x in rng.integers(0, 125, size=4))
+        assert L.minimum_octahedral_orbit_eri_tensor_index(125, arr, i, j, k, l) == orc.min_orbit_eri(orc.sym_grid(5), P, i, j, k, l)
+        assert L.minimum_octahedral_orbit_erida_tensor_index(125, arr, i, j) == orc.min_orbit_erida(orc.sym_grid(5), P, i, j)
+
+
+def test_no_gpu_raises_cleanly():
+    import subprocess, sys
+    code = ("import os; os.environ['HIP_VISIBLE_DEVICES']='-1'\n"
+            "from gausslectronitegrate_amd import _lib\n"
+            "try:\n    _lib.require_gpu()\nexcept RuntimeError as e:\n    print('raised', e)\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=REPO, timeout=120)
+    assert "raised" in out.stdout, out.stdout + out.stderr
