@@ -41,6 +41,8 @@ EXTRA = {
     "gli_eri_sparse_key": (C.c_int64, [C.POINTER(_abi.Grid), _abi.P_glong * 48, C.POINTER(_abi.GridPoint)]),
     "gli_erida_sparse_key": (C.c_int64, [C.POINTER(_abi.Grid), _abi.P_glong * 48, C.POINTER(_abi.GridPoint)]),
     "gli_release_workspace": (None, []),
+    "gli_mode_product_split_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
+                                                C.c_int64, C.c_void_p, C.c_int64]),
     "gli_mode_product_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int64,
                                           C.c_int64, C.c_void_p, C.c_int64]),
 }

@@ -115,6 +115,13 @@ void phase_mark(const char* name);
 void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
                   double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base);
 
+// split-precision (BF16 digit slices, exact FP32 slice sums, FP64 combination) variant:
+// split_factor() prepares B once; mode_product_split() then contracts X with it
+void split_factor(const double* B, int64_t ldb, int C, int Kd);
+void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, double* Y, int64_t ldy,
+                        int Cp, int tri_cols, int tri_rows, int64_t p_base, bool cache_x);
+void split_invalidate_cache();
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int64_t round_up(int64_t a, int64_t b) { return ceil_div(a, b) * b; }
 

@@ -172,12 +172,13 @@ __global__ __launch_bounds__(256) void k_mode_product(
 		__syncthreads();
 	}
 
-	// accumulator layout of the 16x16 FP64 block: lane holds column lane%16, rows 4*(lane/16)+v
+	// accumulator layout of the 16x16 FP64 block (v_mfma_f64_16x16x4f64): lane holds
+	// column lane%16 and rows lane/16 + 4 v, v = 0..3
 	for (int i = 0; i < 2; ++i) {
 		for (int j = 0; j < 2; ++j) {
 			const int col = k0 + wn * 32 + j * 16 + lane % 16;
 			for (int v = 0; v < 4; ++v) {
-				const int64_t row = p0 + wm * 32 + i * 16 + 4 * (lane / 16) + v;
+				const int64_t row = p0 + wm * 32 + i * 16 + (lane / 16) + 4 * v;
 				if (row < P && col < C) { Y[row * ldy + col] = acc[i][j][v]; }
 			}
 		}
@@ -482,7 +483,8 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 		phase_mark("direct");
 		return;
 	}
-	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64, "the split-precision ERI method is not available in this build");
+	const bool mixed = (method == GLI_ERI_TENSOR_MIXED);
+	GLI_REQUIRE(!mixed || s.Mp <= 256, "the split-precision method needs 6(n-1) + 2(K-1) < 256; use the FP64 method");
 
 	// kernel tensor, cached per folded range
 	if (w.T3_M != s.M) {
@@ -490,15 +492,22 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 		k_kernel_tensor<<<dim3((unsigned)ceil_div(Mp, 64), (unsigned)Mp, (unsigned)Mp), 64, 0, c.stream>>>(s.T, s.M, s.Mp, w.T3.p);
 		GLI_HIP(hipGetLastError());
 		w.T3_M = s.M;
+		split_invalidate_cache();
 	}
 	phase_mark("kernel_tensor");
+	if (mixed) {
+		split_factor(w.B.p, Cp, s.Cp, s.Mp);
+		phase_mark("split_factor");
+	}
 	// Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]
 	w.Y1.reserve((size_t)Mp * Mp * Cp);
-	mode_product(w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0);
+	if (mixed) { mode_product_split(w.T3.p, Mp * Mp, Mp * Mp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, true); }
+	else { mode_product(w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0); }
 	phase_mark("mode1");
 	// Y2[m3][k1][k2] = sum_m2 Y1[m2][m3][k1] B[m2][k2]   (k2 >= k1 only)
 	w.Y2.reserve((size_t)Mp * Cp * Cp);
-	mode_product(w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0);
+	if (mixed) { mode_product_split(w.Y1.p, Mp * Cp, Mp * Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, false); }
+	else { mode_product(w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0); }
 	phase_mark("mode2");
 	// V[k1][k2][k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3]    (k1 <= k2 <= k3 only), in row chunks of k1
 	size_t free_b = 0, total_b = 0;
@@ -510,7 +519,12 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	w.V.reserve((size_t)(rows * Cp * Cp));
 	for (int64_t lo = 0; lo < s.C1; lo += rows) {
 		const int64_t hi = std::min<int64_t>(s.C1, lo + rows);
-		mode_product(w.Y2.p + lo * Cp, Cp * Cp, (hi - lo) * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, Cp, s.Cp, 1, 1, lo * Cp);
+		if (mixed) {
+			mode_product_split(w.Y2.p + lo * Cp, Cp * Cp, (hi - lo) * Cp, s.Cp, s.Mp, w.V.p, Cp, s.Cp, 1, 1, lo * Cp, false);
+		}
+		else {
+			mode_product(w.Y2.p + lo * Cp, Cp * Cp, (hi - lo) * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, Cp, s.Cp, 1, 1, lo * Cp);
+		}
 		if (lo == 0) { phase_mark("mode3"); }
 		k_gather<<<(unsigned)ceil_div(num, 256), 256, 0, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.V.p, s.Cp,
 		                                                             (int)lo, (int)hi, d_vals);

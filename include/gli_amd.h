@@ -65,6 +65,11 @@ int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, cons
 int gli_mode_product_device(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int64_t C, int64_t Kd,
                             double* Y, int64_t ldy);
 
+/* the same contraction from BF16 digit slices with exact FP32 slice sums and an FP64 combination
+ * (needs Kd <= 256; error below 2^-40 of the column scales) */
+int gli_mode_product_split_device(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int64_t C,
+                                  int64_t Kd, double* Y, int64_t ldy);
+
 /* number of distinct folded axis columns of an R-point axis (symmetric: mirror-symmetric coefficients) */
 int gli_eri_key_count(int64_t R, int symmetric);
 

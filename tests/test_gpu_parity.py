@@ -301,3 +301,41 @@ def test_project_and_apply(L):
     out = eri.apply(states)
     vint = dense.transpose((0, 2, 1, 3)).reshape(27 * 27, 27 * 27)
     assert np.linalg.norm(out - vint @ states) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# split-precision (mixed) path: error <= 1e-10 absolute against FP64
+
+@pytest.mark.parametrize("Kd,P,Cc", [(32, 64, 64), (176, 300, 130), (149, 1000, 77)])
+def test_split_mode_product_vs_numpy(L, Kd, P, Cc):
+    """BF16 digit slices with exact FP32 sums: error bounded by ~2^-40 of the column scales."""
+    say("split mode product", Kd, P, Cc)
+    rng = np.random.default_rng(Kd * 7 + P)
+    X = rng.standard_normal((Kd, P)) * np.exp(rng.uniform(-8, 2, size=(1, P)))
+    B = rng.standard_normal((Kd, Cc))
+    dX, dB, dY = DevArray.from_numpy(X), DevArray.from_numpy(B), DevArray(8 * P * Cc)
+    _lib.check(L.gli_mode_product_split_device(dX.ptr, P, P, dB.ptr, Cc, Cc, Kd, dY.ptr, Cc), "split mode product")
+    Y = dY.to_numpy(np.float64, P * Cc).reshape(P, Cc)
+    ref = X.T @ B
+    bound = Kd * 2.0 ** -38 * np.abs(X).max(axis=0)[:, None] * np.abs(B).max(axis=0)[None, :]
+    assert np.all(np.abs(Y - ref) <= bound)
+
+
+@pytest.mark.parametrize("name,n", [("golden_sample_n9_K57s1.npz", 9), ("golden_sample_n11_K57s0.npz", 11),
+                                    ("golden_sample_n13_K57s0.npz", 13)])
+def test_eri_mixed_golden(L, name, n):
+    say("eri mixed golden", name)
+    g = golden(name)
+    got = eri_values_device(g["coeffs"], sym(n), g["eri_indices"], float(g["tol"]), 1)
+    assert np.max(np.abs(got - g["eri_values"])) <= 1e-10
+
+
+def test_eri_mixed_full_7(L):
+    """every unique 7^3 element: mixed vs the FP64 tensor path, max abs error <= 1e-10"""
+    say("eri mixed full 7")
+    c = golden("coeffs_K57_s1.npy")
+    d, n = eri_enumerate_device(7)
+    idx = d.to_numpy(np.int64, n)
+    v64 = eri_values_device(c, sym(7), idx, 1e-16, 0)
+    vmx = eri_values_device(c, sym(7), idx, 1e-16, 1)
+    assert np.max(np.abs(vmx - v64)) <= 1e-10
