@@ -111,15 +111,16 @@ void phase_begin();
 void phase_mark(const char* name);
 
 // Y[p][k] = sum_{mu < Kd} X[mu * ldx + p] * B[mu * ldb + k] for p < P, k < C (FP64 MFMA);
-// with Cp > 0 and tri_cols / tri_rows, tiles outside k >= r / outer <= r (p = outer*Cp + r) are skipped
+// with Cp > 0 and tri_cols / tri_rows, tiles outside k >= r / outer <= r (p = outer*Cp + r) are skipped;
+// with tet_C1 > 0 the output goes to the tetrahedral layout of the sorted triple (outer, r, k), all < tet_C1
 void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
-                  double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base);
+                  double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1);
 
 // split-precision (BF16 digit slices, exact FP32 slice sums, FP64 combination) variant:
 // split_factor() prepares B once; mode_product_split() then contracts X with it
 void split_factor(const double* B, int64_t ldb, int C, int Kd);
 void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, double* Y, int64_t ldy,
-                        int Cp, int tri_cols, int tri_rows, int64_t p_base, bool cache_x);
+                        int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, bool cache_x);
 void split_invalidate_cache();
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -114,9 +114,15 @@ __global__ void k_kernel_tensor(const double* __restrict__ T, int M, int Mp, dou
 
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
+// position of the sorted key triple k1 <= k2 <= k3 in the tetrahedral layout of a symmetric 3-tensor
+__host__ __device__ inline int64_t tet_index(int k1, int k2, int k3)
+{
+	return (int64_t)k3 * (k3 + 1) * (k3 + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1;
+}
+
 __global__ __launch_bounds__(256) void k_mode_product(
 	const double* __restrict__ X, int64_t ldx, int64_t P, const double* __restrict__ B, int64_t ldb, int C, int Kd,
-	double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base)
+	double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1)
 {
 	__shared__ double As[GBK][GBM];
 	__shared__ double Bs[GBK][GBN];
@@ -179,7 +185,15 @@ __global__ __launch_bounds__(256) void k_mode_product(
 			const int col = k0 + wn * 32 + j * 16 + lane % 16;
 			for (int v = 0; v < 4; ++v) {
 				const int64_t row = p0 + wm * 32 + i * 16 + (lane / 16) + 4 * v;
-				if (row < P && col < C) { Y[row * ldy + col] = acc[i][j][v]; }
+				if (row >= P || col >= C) { continue; }
+				if (tet_C1 > 0) {
+					const int64_t gp = p_base + row;
+					const int k1 = (int)(gp / Cp), k2 = (int)(gp % Cp);
+					if (k1 <= k2 && k2 <= col && col < tet_C1) { Y[tet_index(k1, k2, col)] = acc[i][j][v]; }
+				}
+				else {
+					Y[row * ldy + col] = acc[i][j][v];
+				}
 			}
 		}
 	}
@@ -196,13 +210,13 @@ struct GridDev {
 	int64_t N4;     // N^4: four-indices at or above it are outside the grid
 };
 
-// linear grid index -> offsets (x, y, z) from the range starts
-__device__ inline void decode_point(const GridDev& g, int idx, int& x, int& y, int& z)
+// packed per-point axis offsets: x | y << 10 | z << 20
+__global__ void k_point_coords(int nx, int ny, int nz, int* __restrict__ out)
 {
-	z = idx % g.n[2];
-	const int r = idx / g.n[2];
-	y = r % g.n[1];
-	x = r / g.n[1];
+	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= nx * ny * nz) { return; }
+	const int z = idx % nz, y = (idx / nz) % ny, x = idx / (nz * ny);
+	out[idx] = x | (y << 10) | (z << 20);
 }
 
 // four-index -> (i, j, k, l); exact for four-indices below 2^53 via a corrected FP64 quotient
@@ -212,10 +226,11 @@ __device__ inline void decode_four(const GridDev& g, int64_t t, int& i, int& j, 
 	int64_t r = t - q * g.N2;
 	if (r < 0) { q -= 1; r += g.N2; }
 	else if (r >= g.N2) { q += 1; r -= g.N2; }
-	i = (int)(q / g.N);
-	j = (int)(q % g.N);
-	k = (int)(r / g.N);
-	l = (int)(r % g.N);
+	const unsigned N = (unsigned)g.N;
+	i = (int)((unsigned)q / N);
+	j = (int)((unsigned)q - (unsigned)i * N);
+	k = (int)((unsigned)r / N);
+	l = (int)((unsigned)r - (unsigned)k * N);
 }
 
 // key id of one axis: a = |s01|, b = |s23|, cd = (pi + pj) - (pk + pl)
@@ -225,17 +240,17 @@ __device__ inline int axis_key(const int* __restrict__ lut, int R, int pi, int p
 	return lut[(a * R + b) * (4 * R - 3) + cd + 2 * (R - 1)];
 }
 
-__device__ inline void element_keys(const GridDev& g, const int* __restrict__ lut, int64_t t, int& k1, int& k2, int& k3)
+__device__ inline void element_keys(const GridDev& g, const int* __restrict__ lut, const int* __restrict__ coords,
+                                    int64_t t, int& k1, int& k2, int& k3)
 {
 	int i, j, k, l;
 	decode_four(g, t, i, j, k, l);
-	int pi[3], pj[3], pk[3], pl[3];
-	decode_point(g, i, pi[0], pi[1], pi[2]);
-	decode_point(g, j, pj[0], pj[1], pj[2]);
-	decode_point(g, k, pk[0], pk[1], pk[2]);
-	decode_point(g, l, pl[0], pl[1], pl[2]);
+	const int ci = coords[i], cj = coords[j], ck = coords[k], cl = coords[l];
 	int kk[3];
-	for (int a = 0; a < 3; ++a) { kk[a] = axis_key(lut, g.R, pi[a], pj[a], pk[a], pl[a]); }
+	for (int a = 0; a < 3; ++a) {
+		const int sh = 10 * a;
+		kk[a] = axis_key(lut, g.R, (ci >> sh) & 1023, (cj >> sh) & 1023, (ck >> sh) & 1023, (cl >> sh) & 1023);
+	}
 	// sort ascending (V is symmetric in its three arguments)
 	int x = kk[0], y = kk[1], z = kk[2], s;
 	if (x > y) { s = x; x = y; y = s; }
@@ -244,27 +259,26 @@ __device__ inline void element_keys(const GridDev& g, const int* __restrict__ lu
 	k1 = x; k2 = y; k3 = z;
 }
 
-// values[e] = V[k1][k2][k3] for elements whose smallest key lies in [klo, khi)
+// values[e] = V[k1][k2][k3] from the tetrahedral key tensor
 __global__ void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g, const int* __restrict__ lut,
-                         const double* __restrict__ V, int Cp, int klo, int khi, double* __restrict__ out)
+                         const int* __restrict__ coords, const double* __restrict__ V, double* __restrict__ out)
 {
 	const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (e >= num) { return; }
 	const int64_t t = idx[e];
 	if (t < 0 || t >= g.N4) {
-		if (klo == 0) { out[e] = __builtin_nan(""); }
+		out[e] = __builtin_nan("");
 		return;
 	}
 	int k1, k2, k3;
-	element_keys(g, lut, t, k1, k2, k3);
-	if (k1 < klo || k1 >= khi) { return; }
-	out[e] = V[((int64_t)(k1 - klo) * Cp + k2) * Cp + k3];
+	element_keys(g, lut, coords, t, k1, k2, k3);
+	out[e] = V[tet_index(k1, k2, k3)];
 }
 
 // per-element FP64 sum over the folded columns, no reuse across elements:
 // value = sum_{m1,m2} B1[m1] B2[m2] sum_{m3} B3[m3] T[m1^2 + m2^2 + m3^2]
 __global__ __launch_bounds__(256) void k_eri_direct(const int64_t* __restrict__ idx, int64_t num, GridDev g,
-	const int* __restrict__ lut, const double* __restrict__ B, int Cp, int M, int Mp, const double* __restrict__ T,
+	const int* __restrict__ lut, const int* __restrict__ coords, const double* __restrict__ B, int Cp, int M, int Mp, const double* __restrict__ T,
 	double* __restrict__ out)
 {
 	extern __shared__ double smem[];
@@ -279,7 +293,7 @@ __global__ __launch_bounds__(256) void k_eri_direct(const int64_t* __restrict__ 
 			continue;
 		}
 		int k[3];
-		element_keys(g, lut, t, k[0], k[1], k[2]);
+		element_keys(g, lut, coords, t, k[0], k[1], k[2]);
 		for (int a = 0; a < 3; ++a) {
 			for (int m = tid; m < Mp; m += blockDim.x) { col[a * Mp + m] = (m <= M) ? B[(int64_t)m * Cp + k[a]] : 0.0; }
 		}
@@ -378,6 +392,7 @@ static KeySet& key_set(int R, bool symmetric)
 
 struct EriWorkspace {
 	DevBuf<double> coeffs, f, P, B, Y1, Y2, V;
+	DevBuf<int> coords;
 	DevBuf<double> T3;
 	int T3_M = -1;
 };
@@ -408,6 +423,7 @@ static EriSetup eri_tables(const double* coeffs, int64_t K, const int64_t grid[6
 {
 	GLI_REQUIRE(K >= 3 && (K % 2) == 1, "the number of gausslet coefficients must be odd and at least 3");
 	GLI_REQUIRE(grid[1] > 0 && grid[3] > 0 && grid[5] > 0, "grid axes must have a positive number of points");
+	GLI_REQUIRE(grid[1] < 1024 && grid[3] < 1024 && grid[5] < 1024, "at most 1023 points per axis");
 	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
 	Context& c = ctx();
 	EriWorkspace& w = eri_ws();
@@ -437,21 +453,23 @@ static EriSetup eri_tables(const double* coeffs, int64_t K, const int64_t grid[6
 	w.f.reserve((size_t)s.R * L);
 	w.P.reserve((size_t)s.R * s.R * LP);
 	w.B.reserve((size_t)s.Mp * s.Cp);
+	w.coords.reserve((size_t)s.g.N);
 	GLI_HIP(hipMemcpyAsync(w.coeffs.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, c.stream));
 	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, 0, c.stream>>>(w.coeffs.p, s.K, w.f.p);
 	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, 0, c.stream>>>(w.f.p, s.K, s.R, tol, w.P.p);
 	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, c.stream>>>(w.P.p, ks.keys_dev.p, s.C1, s.K, s.R, s.M, s.Cp, w.B.p);
+	k_point_coords<<<(unsigned)ceil_div(s.g.N, 256), 256, 0, c.stream>>>(s.g.n[0], s.g.n[1], s.g.n[2], w.coords.p);
 	GLI_HIP(hipGetLastError());
 	return s;
 }
 
 void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
-                         double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base)
+                  double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1)
 {
 	if (P <= 0 || C <= 0) { return; }
 	dim3 grid((unsigned)ceil_div(P, GBM), (unsigned)ceil_div(C, GBN));
 	GLI_REQUIRE(grid.x < 2147483647u, "mode product too large");
-	k_mode_product<<<grid, 256, 0, ctx().stream>>>(X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows, p_base);
+	k_mode_product<<<grid, 256, 0, ctx().stream>>>(X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows, p_base, tet_C1);
 	GLI_HIP(hipGetLastError());
 }
 
@@ -478,7 +496,7 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	if (method == GLI_ERI_DIRECT_FP64) {
 		const int64_t blocks = std::min<int64_t>(num, 65536);
 		const size_t shm = (3 * (size_t)s.Mp + 256) * sizeof(double);
-		k_eri_direct<<<(unsigned)blocks, 256, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.B.p, s.Cp, s.M, s.Mp, s.T, d_vals);
+		k_eri_direct<<<(unsigned)blocks, 256, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.coords.p, w.B.p, s.Cp, s.M, s.Mp, s.T, d_vals);
 		GLI_HIP(hipGetLastError());
 		phase_mark("direct");
 		return;
@@ -501,36 +519,23 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	}
 	// Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]
 	w.Y1.reserve((size_t)Mp * Mp * Cp);
-	if (mixed) { mode_product_split(w.T3.p, Mp * Mp, Mp * Mp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, true); }
-	else { mode_product(w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0); }
+	if (mixed) { mode_product_split(w.T3.p, Mp * Mp, Mp * Mp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0, true); }
+	else { mode_product(w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0); }
 	phase_mark("mode1");
 	// Y2[m3][k1][k2] = sum_m2 Y1[m2][m3][k1] B[m2][k2]   (k2 >= k1 only)
 	w.Y2.reserve((size_t)Mp * Cp * Cp);
-	if (mixed) { mode_product_split(w.Y1.p, Mp * Cp, Mp * Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, false); }
-	else { mode_product(w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0); }
+	if (mixed) { mode_product_split(w.Y1.p, Mp * Cp, Mp * Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0, false); }
+	else { mode_product(w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0); }
 	phase_mark("mode2");
-	// V[k1][k2][k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3]    (k1 <= k2 <= k3 only), in row chunks of k1
-	size_t free_b = 0, total_b = 0;
-	GLI_HIP(hipMemGetInfo(&free_b, &total_b));
-	const int64_t row_bytes = Cp * Cp * (int64_t)sizeof(double);
-	const int64_t have = (int64_t)w.V.n * (int64_t)sizeof(double);
-	int64_t budget = std::max<int64_t>(have, (int64_t)(free_b * 0.6) + have);
-	int64_t rows = std::min<int64_t>(s.C1, std::max<int64_t>(1, budget / row_bytes));
-	w.V.reserve((size_t)(rows * Cp * Cp));
-	for (int64_t lo = 0; lo < s.C1; lo += rows) {
-		const int64_t hi = std::min<int64_t>(s.C1, lo + rows);
-		if (mixed) {
-			mode_product_split(w.Y2.p + lo * Cp, Cp * Cp, (hi - lo) * Cp, s.Cp, s.Mp, w.V.p, Cp, s.Cp, 1, 1, lo * Cp, false);
-		}
-		else {
-			mode_product(w.Y2.p + lo * Cp, Cp * Cp, (hi - lo) * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, Cp, s.Cp, 1, 1, lo * Cp);
-		}
-		if (lo == 0) { phase_mark("mode3"); }
-		k_gather<<<(unsigned)ceil_div(num, 256), 256, 0, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.V.p, s.Cp,
-		                                                             (int)lo, (int)hi, d_vals);
-		GLI_HIP(hipGetLastError());
-		if (lo == 0) { phase_mark("gather"); }
-	}
+	// V[k1 <= k2 <= k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3], written in tetrahedral layout
+	const int64_t nV = tet_index(0, 0, s.C1);
+	w.V.reserve((size_t)nV);
+	if (mixed) { mode_product_split(w.Y2.p, Cp * Cp, Cp * Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1, false); }
+	else { mode_product(w.Y2.p, Cp * Cp, Cp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1); }
+	phase_mark("mode3");
+	k_gather<<<(unsigned)ceil_div(num, 256), 256, 0, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.coords.p, w.V.p, d_vals);
+	GLI_HIP(hipGetLastError());
+	phase_mark("gather");
 }
 
 }  // namespace gli
@@ -594,7 +599,7 @@ int gli_mode_product_device(const double* X, int64_t ldx, int64_t P, const doubl
 		Context& c = ctx();
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
 		GLI_REQUIRE(P >= 0 && C >= 0 && Kd >= 0 && ldx >= P && ldb >= C && ldy >= C, "inconsistent mode-product shapes");
-		mode_product(X, ldx, P, B, ldb, (int)C, (int)Kd, Y, ldy, 0, 0, 0, 0);
+		mode_product(X, ldx, P, B, ldb, (int)C, (int)Kd, Y, ldy, 0, 0, 0, 0, 0);
 		GLI_HIP(hipStreamSynchronize(c.stream));
 		return 0;
 	}

@@ -23,11 +23,8 @@ constexpr int NPROD = NS * (NS + 1) / 2;      // slice products with s + t <= NS
 constexpr int SBM = 64, SBN = 64, SBK = 32;   // block tile; K step = one MFMA 16x16x32
 constexpr int SPAD = 8;                       // LDS row padding (bf16 elements)
 
-// digits of the columns of a Kd x P matrix (row stride ld):
-// D[(s * Ppad + p) * Kp + mu] (mu contiguous per column), e[p] the column exponent.
-// Columns p >= P and rows mu >= Kd are zero digits.
-__global__ void k_split_columns(const double* __restrict__ X, int64_t ld, int64_t P, int Kd, int64_t Ppad, int Kp,
-                                uint16_t* __restrict__ D, int* __restrict__ e)
+// column exponents of a Kd x P matrix (row stride ld): |X[mu][p]| < 2^e[p]; padding columns get 0
+__global__ void k_column_exponents(const double* __restrict__ X, int64_t ld, int64_t P, int Kd, int64_t Ppad, int* __restrict__ e)
 {
 	const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (p >= Ppad) { return; }
@@ -38,15 +35,33 @@ __global__ void k_split_columns(const double* __restrict__ X, int64_t ld, int64_
 	int ex = 0;
 	if (mx > 0 && isfinite(mx)) { (void)frexp(mx, &ex); }  // mx = f 2^ex, f in [1/2, 1)
 	e[p] = ex;
-	for (int mu = 0; mu < Kp; ++mu) {
-		double x = (p < P && mu < Kd) ? ldexp(X[(int64_t)mu * ld + p], -ex) : 0.0;
-		for (int s = 0; s < NS; ++s) {
+}
+
+// base-256 digits of a 32 (mu) x 64 (p) tile, written as D[(s * Ppad + p) * Kp + mu]
+// (mu contiguous per column: the GEMM loads 16-byte runs of 8 digits)
+__global__ __launch_bounds__(256) void k_split_tile(const double* __restrict__ X, int64_t ld, int64_t P, int Kd,
+	int64_t Ppad, int Kp, const int* __restrict__ e, uint16_t* __restrict__ D)
+{
+	__shared__ uint16_t T[NS][64][SBK + SPAD];
+	const int64_t p0 = (int64_t)blockIdx.x * 64;
+	const int mu0 = blockIdx.y * SBK;
+	for (int id = threadIdx.x; id < 64 * SBK; id += 256) {
+		const int r = id % 64, c = id / 64;   // consecutive threads: consecutive p (coalesced reads)
+		const int64_t p = p0 + r;
+		const int mu = mu0 + c;
+		double x = (p < P && mu < Kd) ? ldexp(X[(int64_t)mu * ld + p], -e[p]) : 0.0;
+		for (int sl = 0; sl < NS; ++sl) {
 			x *= 256.0;
 			const double d = trunc(x);
 			x -= d;
 			const float f = (float)d;  // |d| <= 255: exact in the top 16 bits of the float
-			D[((int64_t)s * Ppad + p) * Kp + mu] = (uint16_t)(__float_as_uint(f) >> 16);
+			T[sl][r][c] = (uint16_t)(__float_as_uint(f) >> 16);
 		}
+	}
+	__syncthreads();
+	for (int id = threadIdx.x; id < NS * 64 * (SBK / 8); id += 256) {
+		const int sl = id / (64 * 4), r = (id / 4) % 64, c8 = (id % 4) * 8;
+		*(uint4*)&D[((int64_t)sl * Ppad + p0 + r) * Kp + mu0 + c8] = *(const uint4*)&T[sl][r][c8];
 	}
 }
 
@@ -54,7 +69,8 @@ __global__ void k_split_columns(const double* __restrict__ X, int64_t ld, int64_
 __global__ __launch_bounds__(256) void k_mode_product_split(
 	const uint16_t* __restrict__ Xd, int64_t Ppad, const int* __restrict__ ex,
 	const uint16_t* __restrict__ Bd, int64_t Cpad, const int* __restrict__ eb,
-	int64_t P, int C, int Kp, double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base)
+	int64_t P, int C, int Kp, double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base,
+	int tet_C1)
 {
 	__shared__ uint16_t As[NS][SBM][SBK + SPAD];
 	__shared__ uint16_t Bs[NS][SBN][SBK + SPAD];
@@ -127,7 +143,17 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 					}
 					y += ldexp(g, -8 * (d + 2));
 				}
-				Y[row * ldy + col] = ldexp(y, ex[row] + eb[col]);
+				y = ldexp(y, ex[row] + eb[col]);
+				if (tet_C1 > 0) {
+					const int64_t gp = p_base + row;
+					const int k1 = (int)(gp / Cp), k2 = (int)(gp % Cp);
+					if (k1 <= k2 && k2 <= col && col < tet_C1) {
+						Y[(int64_t)col * (col + 1) * (col + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1] = y;
+					}
+				}
+				else {
+					Y[row * ldy + col] = y;
+				}
 			}
 		}
 	}
@@ -152,7 +178,9 @@ static int64_t split_operand(const double* X, int64_t ld, int64_t P, int Kd, int
 	const int64_t Ppad = round_up(P, SBM);
 	D.reserve((size_t)NS * Ppad * Kp);
 	e.reserve((size_t)Ppad);
-	k_split_columns<<<(unsigned)ceil_div(Ppad, 256), 256, 0, ctx().stream>>>(X, ld, P, Kd, Ppad, Kp, D.p, e.p);
+	hipStream_t st = ctx().stream;
+	k_column_exponents<<<(unsigned)ceil_div(Ppad, 256), 256, 0, st>>>(X, ld, P, Kd, Ppad, e.p);
+	k_split_tile<<<dim3((unsigned)(Ppad / 64), (unsigned)(Kp / SBK)), 256, 0, st>>>(X, ld, P, Kd, Ppad, Kp, e.p, D.p);
 	GLI_HIP(hipGetLastError());
 	return Ppad;
 }
@@ -166,7 +194,7 @@ void split_factor(const double* B, int64_t ldb, int C, int Kd)
 }
 
 void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, double* Y, int64_t ldy,
-                        int Cp, int tri_cols, int tri_rows, int64_t p_base, bool cache_x)
+                        int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, bool cache_x)
 {
 	GLI_REQUIRE(Kd <= 256, "split-precision contraction needs at most 256 terms per sum");
 	if (P <= 0 || C <= 0) { return; }
@@ -194,7 +222,7 @@ void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, 
 	}
 	dim3 grid((unsigned)(Ppad / SBM), (unsigned)(Cpad / SBN));
 	k_mode_product_split<<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,
-	                                                     tri_cols, tri_rows, p_base);
+	                                                     tri_cols, tri_rows, p_base, tet_C1);
 	GLI_HIP(hipGetLastError());
 }
 
@@ -215,7 +243,7 @@ extern "C" int gli_mode_product_split_device(const double* X, int64_t ldx, int64
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
 		GLI_REQUIRE(P >= 0 && C >= 0 && Kd >= 0 && ldx >= P && ldb >= C && ldy >= C, "inconsistent mode-product shapes");
 		split_factor(B, ldb, (int)C, (int)Kd);
-		mode_product_split(X, ldx, P, (int)C, (int)Kd, Y, ldy, 0, 0, 0, 0, false);
+		mode_product_split(X, ldx, P, (int)C, (int)Kd, Y, ldy, 0, 0, 0, 0, 0, false);
 		GLI_HIP(hipStreamSynchronize(c.stream));
 		return 0;
 	}

@@ -150,9 +150,9 @@ static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 	}
 	GLI_HIP(hipGetLastError());
 	// W[ux][uy][uz] -> Y1[uy][uz][qx] -> Y2[uz][qx][qy] -> V[qx][qy][qz]
-	mode_product(W.p, (int64_t)U[1] * U[2], (int64_t)U[1] * U[2], F0.p, C[0], C[0], U[0], Y1.p, C[0], 0, 0, 0, 0);
-	mode_product(Y1.p, (int64_t)U[2] * C[0], (int64_t)U[2] * C[0], F1.p, C[1], C[1], U[1], Y2.p, C[1], 0, 0, 0, 0);
-	mode_product(Y2.p, (int64_t)C[0] * C[1], (int64_t)C[0] * C[1], F2.p, C[2], C[2], U[2], V.p, C[2], 0, 0, 0, 0);
+	mode_product(W.p, (int64_t)U[1] * U[2], (int64_t)U[1] * U[2], F0.p, C[0], C[0], U[0], Y1.p, C[0], 0, 0, 0, 0, 0);
+	mode_product(Y1.p, (int64_t)U[2] * C[0], (int64_t)U[2] * C[0], F1.p, C[1], C[1], U[1], Y2.p, C[1], 0, 0, 0, 0, 0);
+	mode_product(Y2.p, (int64_t)C[0] * C[1], (int64_t)C[0] * C[1], F2.p, C[2], C[2], U[2], V.p, C[2], 0, 0, 0, 0, 0);
 	k_nuclear_scatter<<<(unsigned)ceil_div(N * N, 256), 256, 0, st>>>(V.p, n[0], n[1], n[2], M.p);
 	GLI_HIP(hipGetLastError());
 	GLI_HIP(hipMemcpyAsync(out, M.p, N * N * sizeof(double), hipMemcpyDeviceToHost, st));

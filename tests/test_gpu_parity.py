@@ -275,11 +275,15 @@ def test_nuclear(L):
 
 
 def test_nuclear_golden_7(L):
-    """7^3, two unit charges at (0, 0, +-0.7): sampled entries, row sums and trace of the reference build."""
+    """7^3, two unit charges at (0, 0, +-0.7): sampled entries, row sums and trace of the reference build.
+
+    The reference sums each entry sequentially over about 2.9 million signed
+    terms (113^3 primitive centres x 2 nuclei, random-sign coefficients), which
+    carries a rounding error of order 1e-13 by itself; the bound is 3e-13."""
     say("nuclear 7")
     g = golden("golden_n7_K57s0.npz")
     m = gli.compute_nuclear_gausslet_integrals(g["coeffs"], sym(7), g["nuc_pos"], g["nuc_charges"], float(g["tol"]))
-    assert np.max(np.abs(m[g["nuc_rows"], g["nuc_cols"]] - g["nuc_sample"])) <= 1e-13
+    assert np.max(np.abs(m[g["nuc_rows"], g["nuc_cols"]] - g["nuc_sample"])) <= 3e-13
     assert np.max(np.abs(m.sum(axis=1) - g["nuc_rowsum"])) <= 1e-11
     assert abs(np.trace(m) - g["nuc_trace"]) <= 1e-11
     assert np.array_equal(m, m.T)
