@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--direct-sample", type=int, default=16384)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--asymmetric", action="store_true",
+                    help="coefficients without mirror symmetry (general axis-key set, GEMM-dominated)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would take a contiguous, "
@@ -132,7 +134,7 @@ def main():
     L = _lib.require_gpu()
     stream = torch.cuda.ExternalStream(L.gli_stream())
     method = METHODS[args.method]
-    c = synthetic_coefficients(args.K, args.seed)
+    c = synthetic_coefficients(args.K, args.seed, symmetrise=not args.asymmetric)
     g = sym_grid6(args.n)
 
     # enumeration (grid only), resident in HBM
@@ -247,7 +249,8 @@ def main():
         "vs_baseline": None,
         "dtype": {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64"}[args.method],
         "data": "synthetic (seeded gausslet coefficients)",
-        "config": {"workload": "eri_%dx%dx%d_K%d_seed%d" % (args.n, args.n, args.n, args.K, args.seed),
+        "config": {"workload": "eri_%dx%dx%d_K%d_seed%d%s" % (args.n, args.n, args.n, args.K, args.seed,
+                                                             "_asym" if args.asymmetric else ""),
                    "grid": args.n, "K": args.K, "seed": args.seed, "tol": args.tol, "method": args.method,
                    "unique_elements": num, "axis_keys": C1},
         "roofline": roof,

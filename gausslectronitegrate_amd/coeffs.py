@@ -17,12 +17,14 @@ def overlap_matrix(K):
     return (np.sqrt(np.pi) / 3.0) * np.exp(-0.25 * d * d)
 
 
-def synthetic_coefficients(K, seed):
+def synthetic_coefficients(K, seed, symmetrise=True):
+    """symmetrise=False keeps the raw draws (no mirror symmetry), for the general key set."""
     if K % 2 != 1 or K < 3:
         raise ValueError("K must be odd and >= 3")
     rng = np.random.default_rng(seed)
     j = np.arange(K) - (K - 1) // 2
     c = rng.standard_normal(K) * np.exp(-(4.0 * j / 57.0) ** 2)
-    c = 0.5 * (c + c[::-1])
+    if symmetrise:
+        c = 0.5 * (c + c[::-1])
     c = c / np.sqrt(c @ overlap_matrix(K) @ c)
     return np.ascontiguousarray(c, dtype=np.float64)

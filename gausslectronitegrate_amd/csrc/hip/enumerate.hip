@@ -40,9 +40,17 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 	return ((x + g.h) * g.n + (y + g.h)) * g.n + (z + g.h);
 }
 
-// is the centred 4-tuple P (coordinates relative to the grid centre) the
-// minimum of its orbit?  IJ = i*N + j, KL = k*N + l.
-__device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], int IJ, int KL)
+// Is the centred 4-tuple P (coordinates relative to the grid centre) the minimum of its orbit?
+// i = t's first index (an orbit-minimal point), IJ = i*N + j, KL = k*N + l.
+//
+// An image's four-index starts with the smallest of its four image points.  For a
+// placement whose points q all have orbit minimum >= i, every image point is >= i, so an
+// image can only fall below t when some g maps a point q onto i itself; then
+// minimg[q] == i and g belongs to canon[q], the elements taking q to its orbit minimum.
+// A placement containing a point with orbit minimum < i always has an image below t.
+// This is the same decision as testing all 48 elements (eri_integrals.c:300-352, 463-487).
+__device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], int i, int IJ, int KL,
+                                 const int* __restrict__ minimg, const unsigned long long* __restrict__ canon)
 {
 	int c[3];
 	for (int a = 0; a < 3; ++a) {
@@ -53,22 +61,28 @@ __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], int IJ, i
 	for (int bx = 0; bx <= abs(c[0]); ++bx) {
 		for (int by = 0; by <= abs(c[1]); ++by) {
 			for (int bz = 0; bz <= abs(c[2]); ++bz) {
-				int Q[4][3];
+				int Q[4][3], lq[4];
 				for (int p = 0; p < 4; ++p) {
 					Q[p][0] = P[p][0] - bx * c[0];
 					Q[p][1] = P[p][1] - by * c[1];
 					Q[p][2] = P[p][2] - bz * c[2];
+					lq[p] = lin_c(g, Q[p][0], Q[p][1], Q[p][2]);
+					if (minimg[lq[p]] < i) { return false; }
 				}
-				for (int e = 0; e < 48; ++e) {
-					const int c0 = c_oh_col[e][0], c1 = c_oh_col[e][1], c2 = c_oh_col[e][2];
-					const int s0 = c_oh_sgn[e][0], s1 = c_oh_sgn[e][1], s2 = c_oh_sgn[e][2];
-					int L[4];
-					for (int p = 0; p < 4; ++p) { L[p] = lin_c(g, s0 * Q[p][c0], s1 * Q[p][c1], s2 * Q[p][c2]); }
-					const int a = min(L[0], L[1]), b = max(L[0], L[1]);
-					const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
-					int ab = a * g.N + b, cd = cc * g.N + d;
-					if (ab > cd) { const int t = ab; ab = cd; cd = t; }
-					if (ab < IJ || (ab == IJ && cd < KL)) { return false; }
+				for (int p = 0; p < 4; ++p) {
+					if (minimg[lq[p]] != i) { continue; }
+					for (unsigned long long m = canon[lq[p]]; m; m &= m - 1) {
+						const int e = __builtin_ctzll(m);
+						const int c0 = c_oh_col[e][0], c1 = c_oh_col[e][1], c2 = c_oh_col[e][2];
+						const int s0 = c_oh_sgn[e][0], s1 = c_oh_sgn[e][1], s2 = c_oh_sgn[e][2];
+						int L[4];
+						for (int r = 0; r < 4; ++r) { L[r] = lin_c(g, s0 * Q[r][c0], s1 * Q[r][c1], s2 * Q[r][c2]); }
+						const int a = min(L[0], L[1]), b = max(L[0], L[1]);
+						const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
+						int ab = a * g.N + b, cd = cc * g.N + d;
+						if (ab > cd) { const int t = ab; ab = cd; cd = t; }
+						if (ab < IJ || (ab == IJ && cd < KL)) { return false; }
+					}
 				}
 			}
 		}
@@ -92,7 +106,7 @@ __device__ inline unsigned centre_mask(const EnumGrid& g, int pi, int pj, int pk
 template <int kWrite>
 __global__ __launch_bounds__(256) void k_eri_rows(EnumGrid g, const int2* __restrict__ units, const int64_t* __restrict__ row_base,
 	const int* __restrict__ klist, const int* __restrict__ kstart, const int* __restrict__ klen,
-	const int* __restrict__ unit_icand, const int* __restrict__ minimg,
+	const int* __restrict__ unit_icand, const int* __restrict__ minimg, const unsigned long long* __restrict__ canon,
 	uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int64_t* __restrict__ out)
 {
 	const int u = blockIdx.x;
@@ -128,7 +142,7 @@ __global__ __launch_bounds__(256) void k_eri_rows(EnumGrid g, const int2* __rest
 						const int l = lin_c(g, lx, ly, lz);
 						if (l < lmin || minimg[l] < i) { continue; }
 						P[3][0] = lx; P[3][1] = ly; P[3][2] = lz;
-						if (eri_is_canonical(g, P, IJ, k * g.N + l)) {
+						if (eri_is_canonical(g, P, i, IJ, k * g.N + l, minimg, canon)) {
 							if (kWrite) {
 								out[pos++] = (((int64_t)i * g.N + j) * g.N + k) * g.N + l;
 							}
@@ -213,6 +227,7 @@ template int64_t exclusive_scan<uint32_t>(const uint32_t*, int64_t, int64_t*);
 struct EnumPlan {
 	EnumGrid g;
 	std::vector<int> minimg;       // per point: smallest linear index over its O_h orbit
+	std::vector<unsigned long long> canon;  // per point: the elements g with lin(g p) == minimg
 	std::vector<int2> units;       // (i, j)
 	std::vector<int> unit_icand;   // index of i in the orbit-minimal list
 	std::vector<int64_t> row_base; // first row of each unit
@@ -229,14 +244,20 @@ static void plan_enumeration(int n, EnumPlan& p)
 	p.g.N = n * n * n;
 	const int N = p.g.N, h = p.g.h;
 	p.minimg.assign(N, 0);
+	p.canon.assign(N, 0);
 	for (int idx = 0; idx < N; ++idx) {
 		const int q[3] = { idx / (n * n) - h, (idx / n) % n - h, idx % n - h };
+		int img[48];
 		int best = N;
 		for (int e = 0; e < 48; ++e) {
 			const int x = sgn[e][0] * q[col[e][0]], y = sgn[e][1] * q[col[e][1]], z = sgn[e][2] * q[col[e][2]];
-			best = std::min(best, ((x + h) * n + (y + h)) * n + (z + h));
+			img[e] = ((x + h) * n + (y + h)) * n + (z + h);
+			best = std::min(best, img[e]);
 		}
 		p.minimg[idx] = best;
+		for (int e = 0; e < 48; ++e) {
+			if (img[e] == best) { p.canon[idx] |= 1ull << e; }
+		}
 	}
 	std::vector<int> icand;
 	for (int idx = 0; idx < N; ++idx) {
@@ -282,16 +303,18 @@ static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 	GLI_HIP(hipMemcpyAsync(kstart.p, p.kstart.data(), p.kstart.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(klen.p, p.klen.data(), p.klen.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(minimg.p, p.minimg.data(), p.minimg.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	DevBuf<unsigned long long> canon(p.canon.size());
+	GLI_HIP(hipMemcpyAsync(canon.p, p.canon.data(), p.canon.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(row_base.p, p.row_base.data(), p.row_base.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
 	DevBuf<uint32_t> counts(p.rows);
 	DevBuf<int64_t> offsets(p.rows);
 	const unsigned nu = (unsigned)p.units.size();
-	k_eri_rows<0><<<nu, 256, 0, st>>>(p.g, units.p, row_base.p, klist.p, kstart.p, klen.p, unit_ic.p, minimg.p, counts.p, nullptr, nullptr);
+	k_eri_rows<0><<<nu, 256, 0, st>>>(p.g, units.p, row_base.p, klist.p, kstart.p, klen.p, unit_ic.p, minimg.p, canon.p, counts.p, nullptr, nullptr);
 	GLI_HIP(hipGetLastError());
 	const int64_t total = exclusive_scan<uint32_t>(counts.p, p.rows, offsets.p);
 	int64_t* out = nullptr;
 	GLI_HIP(hipMalloc(&out, (size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));
-	k_eri_rows<1><<<nu, 256, 0, st>>>(p.g, units.p, row_base.p, klist.p, kstart.p, klen.p, unit_ic.p, minimg.p, nullptr, offsets.p, out);
+	k_eri_rows<1><<<nu, 256, 0, st>>>(p.g, units.p, row_base.p, klist.p, kstart.p, klen.p, unit_ic.p, minimg.p, canon.p, nullptr, offsets.p, out);
 	GLI_HIP(hipGetLastError());
 	GLI_HIP(hipStreamSynchronize(st));
 	*d_out = out;

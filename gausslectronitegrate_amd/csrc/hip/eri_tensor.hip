@@ -259,20 +259,63 @@ __device__ inline void element_keys(const GridDev& g, const int* __restrict__ lu
 	k1 = x; k2 = y; k3 = z;
 }
 
-// values[e] = V[k1][k2][k3] from the tetrahedral key tensor
-__global__ void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g, const int* __restrict__ lut,
-                         const int* __restrict__ coords, const double* __restrict__ V, double* __restrict__ out)
+// values[e] = V[k1][k2][k3] from the tetrahedral key tensor.
+// Persistent grid: each block stages the key table (int16) and the point coordinates in LDS once,
+// then every thread keeps GATHER_EPT independent elements in flight (index load, key decode,
+// V load, value store), so the dependent loads of one element overlap those of the others.
+constexpr int GATHER_EPT = 4;
+
+__global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g,
+	const short* __restrict__ lut, int nlut, const int* __restrict__ coords, const double* __restrict__ V,
+	double* __restrict__ out)
 {
-	const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (e >= num) { return; }
-	const int64_t t = idx[e];
-	if (t < 0 || t >= g.N4) {
-		out[e] = __builtin_nan("");
-		return;
+	extern __shared__ int gsh[];
+	int* scoord = gsh;
+	short* slut = (short*)(gsh + g.N);
+	for (int q = threadIdx.x; q < g.N; q += blockDim.x) { scoord[q] = coords[q]; }
+	for (int q = threadIdx.x; q < nlut; q += blockDim.x) { slut[q] = lut[q]; }
+	__syncthreads();
+	const int W = 4 * g.R - 3, off = 2 * (g.R - 1);
+	const int64_t stride = (int64_t)gridDim.x * blockDim.x * GATHER_EPT;
+	for (int64_t base = (int64_t)blockIdx.x * blockDim.x * GATHER_EPT + threadIdx.x; base < num; base += stride) {
+		int64_t t[GATHER_EPT];
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) {
+			const int64_t e = base + (int64_t)u * blockDim.x;
+			t[u] = (e < num) ? idx[e] : -1;
+		}
+		int64_t vi[GATHER_EPT];
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) {
+			vi[u] = -1;
+			if (t[u] >= 0 && t[u] < g.N4) {
+				int i, j, k, l;
+				decode_four(g, t[u], i, j, k, l);
+				const int ci = scoord[i], cj = scoord[j], ck = scoord[k], cl = scoord[l];
+				int kk[3];
+#pragma unroll
+				for (int a = 0; a < 3; ++a) {
+					const int sh = 10 * a;
+					const int pi = (ci >> sh) & 1023, pj = (cj >> sh) & 1023, pk = (ck >> sh) & 1023, pl = (cl >> sh) & 1023;
+					const int aa = abs(pi - pj), bb = abs(pk - pl), cd = (pi + pj) - (pk + pl);
+					kk[a] = slut[(aa * g.R + bb) * W + cd + off];
+				}
+				int x = kk[0], y = kk[1], z = kk[2], sw;
+				if (x > y) { sw = x; x = y; y = sw; }
+				if (y > z) { sw = y; y = z; z = sw; }
+				if (x > y) { sw = x; x = y; y = sw; }
+				vi[u] = tet_index(x, y, z);
+			}
+		}
+		double v[GATHER_EPT];
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) {
+			const int64_t e = base + (int64_t)u * blockDim.x;
+			if (e < num) { out[e] = v[u]; }
+		}
 	}
-	int k1, k2, k3;
-	element_keys(g, lut, coords, t, k1, k2, k3);
-	out[e] = V[tet_index(k1, k2, k3)];
 }
 
 // per-element FP64 sum over the folded columns, no reuse across elements:
@@ -345,6 +388,7 @@ struct KeySet {
 	std::vector<int> lut;    // (a * R + b) * (4R - 3) + cd + 2(R - 1) -> key id
 	DevBuf<int4> keys_dev;
 	DevBuf<int> lut_dev;
+	DevBuf<short> lut16_dev;  // the same table as int16 (key ids stay below 32768)
 };
 
 static KeySet& key_set(int R, bool symmetric)
@@ -386,6 +430,10 @@ static KeySet& key_set(int R, bool symmetric)
 	ks->lut_dev.alloc(ks->lut.size());
 	GLI_HIP(hipMemcpy(ks->keys_dev.p, ks->keys.data(), ks->keys.size() * sizeof(int4), hipMemcpyHostToDevice));
 	GLI_HIP(hipMemcpy(ks->lut_dev.p, ks->lut.data(), ks->lut.size() * sizeof(int), hipMemcpyHostToDevice));
+	GLI_REQUIRE(ks->C1 < 32768, "too many axis keys for the 16-bit key table");
+	std::vector<short> l16(ks->lut.begin(), ks->lut.end());
+	ks->lut16_dev.alloc(l16.size());
+	GLI_HIP(hipMemcpy(ks->lut16_dev.p, l16.data(), l16.size() * sizeof(short), hipMemcpyHostToDevice));
 	cache[{R, symmetric}] = ks;
 	return *ks;
 }
@@ -533,7 +581,15 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	if (mixed) { mode_product_split(w.Y2.p, Cp * Cp, Cp * Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1, false); }
 	else { mode_product(w.Y2.p, Cp * Cp, Cp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1); }
 	phase_mark("mode3");
-	k_gather<<<(unsigned)ceil_div(num, 256), 256, 0, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.coords.p, w.V.p, d_vals);
+	{
+		const int nlut = (int)s.ks->lut.size();
+		const size_t shm = (size_t)s.g.N * sizeof(int) + (size_t)nlut * sizeof(short);
+		GLI_REQUIRE(shm <= 64 * 1024, "grid too large for the LDS-staged gather tables");
+		int cus = 256;
+		GLI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+		const int64_t blocks = std::min<int64_t>(ceil_div(num, 256 * GATHER_EPT), (int64_t)cus * 8);
+		k_gather<<<(unsigned)blocks, 256, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut16_dev.p, nlut, w.coords.p, w.V.p, d_vals);
+	}
 	GLI_HIP(hipGetLastError());
 	phase_mark("gather");
 }
