@@ -146,24 +146,30 @@ __global__ __launch_bounds__(256) void k_mode_product(
 		for (int j = 0; j < 2; ++j) { acc[i][j] = d4{0, 0, 0, 0}; }
 	}
 
-	for (int mu0 = 0; mu0 < Kd; mu0 += GBK) {
+	// software pipeline: the next mu-tile is loaded into registers while the current one
+	// (already in LDS) feeds the MFMAs
+	constexpr int PER = (GBK * GBM) / 256;
+	double ra[PER], rb[PER];
+	auto gload = [&](int mu0) {
 #pragma unroll
-		for (int r = 0; r < (GBK * GBM) / 256; ++r) {
+		for (int r = 0; r < PER; ++r) {
 			const int id = tid + 256 * r;
 			const int mu = id / GBM, p = id % GBM;
-			double v = 0;
-			if (mu0 + mu < Kd && p0 + p < P) { v = X[(int64_t)(mu0 + mu) * ldx + p0 + p]; }
-			As[mu][p] = v;
+			ra[r] = (mu0 + mu < Kd && p0 + p < P) ? X[(int64_t)(mu0 + mu) * ldx + p0 + p] : 0.0;
+			const int k = id % GBN;
+			rb[r] = (mu0 + mu < Kd && k0 + k < C) ? B[(int64_t)(mu0 + mu) * ldb + k0 + k] : 0.0;
 		}
+	};
+	gload(0);
+	for (int mu0 = 0; mu0 < Kd; mu0 += GBK) {
 #pragma unroll
-		for (int r = 0; r < (GBK * GBN) / 256; ++r) {
+		for (int r = 0; r < PER; ++r) {
 			const int id = tid + 256 * r;
-			const int mu = id / GBN, k = id % GBN;
-			double v = 0;
-			if (mu0 + mu < Kd && k0 + k < C) { v = B[(int64_t)(mu0 + mu) * ldb + k0 + k]; }
-			Bs[mu][k] = v;
+			As[id / GBM][id % GBM] = ra[r];
+			Bs[id / GBN][id % GBN] = rb[r];
 		}
 		__syncthreads();
+		if (mu0 + GBK < Kd) { gload(mu0 + GBK); }
 #pragma unroll
 		for (int kk = 0; kk < GBK; kk += 4) {
 			double a[2], b[2];
@@ -259,62 +265,101 @@ __device__ inline void element_keys(const GridDev& g, const int* __restrict__ lu
 	k1 = x; k2 = y; k3 = z;
 }
 
-// values[e] = V[k1][k2][k3] from the tetrahedral key tensor.
-// Persistent grid: each block stages the key table (int16) and the point coordinates in LDS once,
-// then every thread keeps GATHER_EPT independent elements in flight (index load, key decode,
-// V load, value store), so the dependent loads of one element overlap those of the others.
+// quotient a / d for 0 <= a < 2^24 from a float reciprocal and one correction step
+__device__ inline int qdiv(int a, int d, float inv)
+{
+	int q = (int)((float)a * inv);
+	const int r = a - q * d;
+	if (r < 0) { q -= 1; }
+	else if (r >= d) { q += 1; }
+	return q;
+}
+
+struct DecodeConsts {
+	double inv_N2;   // 1 / N^2
+	float inv_N;     // 1 / N
+	float inv_nz;    // 1 / n_z
+	float inv_nyz;   // 1 / (n_y n_z)
+};
+
+// tetrahedral V index of a four-index; exact arithmetic decode (no table loads but the key table)
+__device__ inline int64_t gather_index(const GridDev& g, const DecodeConsts& dc, const short* __restrict__ lut, int64_t t)
+{
+	const int W = 4 * g.R - 3, off = 2 * (g.R - 1);
+	if (g.N > 4096) {
+		// large grids: integer division (the float quotients are exact only below 2^24)
+		int pt[4];
+		decode_four(g, t, pt[0], pt[1], pt[2], pt[3]);
+		int c[4][3];
+		for (int a = 0; a < 4; ++a) {
+			c[a][2] = pt[a] % g.n[2];
+			c[a][1] = (pt[a] / g.n[2]) % g.n[1];
+			c[a][0] = pt[a] / (g.n[2] * g.n[1]);
+		}
+		int kk[3];
+		for (int a = 0; a < 3; ++a) {
+			kk[a] = lut[(abs(c[0][a] - c[1][a]) * g.R + abs(c[2][a] - c[3][a])) * W + (c[0][a] + c[1][a]) - (c[2][a] + c[3][a]) + off];
+		}
+		int x = kk[0], y = kk[1], z = kk[2], sw;
+		if (x > y) { sw = x; x = y; y = sw; }
+		if (y > z) { sw = y; y = z; z = sw; }
+		if (x > y) { sw = x; x = y; y = sw; }
+		return tet_index(x, y, z);
+	}
+	int64_t q = (int64_t)((double)t * dc.inv_N2);
+	int64_t r = t - q * g.N2;
+	if (r < 0) { q -= 1; r += g.N2; }
+	else if (r >= g.N2) { q += 1; r -= g.N2; }
+	const int qi = (int)q, ri = (int)r;
+	const int i = qdiv(qi, g.N, dc.inv_N), k = qdiv(ri, g.N, dc.inv_N);
+	const int p[4] = { i, qi - i * g.N, k, ri - k * g.N };
+	const int nyz = g.n[1] * g.n[2];
+	int c[4][3];
+#pragma unroll
+	for (int a = 0; a < 4; ++a) {
+		const int x = qdiv(p[a], nyz, dc.inv_nyz);
+		const int rem = p[a] - x * nyz;
+		const int y = qdiv(rem, g.n[2], dc.inv_nz);
+		c[a][0] = x;
+		c[a][1] = y;
+		c[a][2] = rem - y * g.n[2];
+	}
+	int kk[3];
+#pragma unroll
+	for (int a = 0; a < 3; ++a) {
+		kk[a] = lut[(abs(c[0][a] - c[1][a]) * g.R + abs(c[2][a] - c[3][a])) * W + (c[0][a] + c[1][a]) - (c[2][a] + c[3][a]) + off];
+	}
+	int x = kk[0], y = kk[1], z = kk[2], sw;
+	if (x > y) { sw = x; x = y; y = sw; }
+	if (y > z) { sw = y; y = z; z = sw; }
+	if (x > y) { sw = x; x = y; y = sw; }
+	return tet_index(x, y, z);
+}
+
+// values[e] = V[k1][k2][k3] from the tetrahedral key tensor; GATHER_EPT independent elements per
+// thread keep several dependent load chains (index -> key table -> V) in flight
 constexpr int GATHER_EPT = 4;
 
-__global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g,
-	const short* __restrict__ lut, int nlut, const int* __restrict__ coords, const double* __restrict__ V,
-	double* __restrict__ out)
+__global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g, DecodeConsts dc,
+	const short* __restrict__ lut, const double* __restrict__ V, double* __restrict__ out)
 {
-	extern __shared__ int gsh[];
-	int* scoord = gsh;
-	short* slut = (short*)(gsh + g.N);
-	for (int q = threadIdx.x; q < g.N; q += blockDim.x) { scoord[q] = coords[q]; }
-	for (int q = threadIdx.x; q < nlut; q += blockDim.x) { slut[q] = lut[q]; }
-	__syncthreads();
-	const int W = 4 * g.R - 3, off = 2 * (g.R - 1);
-	const int64_t stride = (int64_t)gridDim.x * blockDim.x * GATHER_EPT;
-	for (int64_t base = (int64_t)blockIdx.x * blockDim.x * GATHER_EPT + threadIdx.x; base < num; base += stride) {
-		int64_t t[GATHER_EPT];
+	const int64_t base = (int64_t)blockIdx.x * blockDim.x * GATHER_EPT + threadIdx.x;
+	int64_t t[GATHER_EPT];
 #pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) {
-			const int64_t e = base + (int64_t)u * blockDim.x;
-			t[u] = (e < num) ? idx[e] : -1;
-		}
-		int64_t vi[GATHER_EPT];
+	for (int u = 0; u < GATHER_EPT; ++u) {
+		const int64_t e = base + (int64_t)u * blockDim.x;
+		t[u] = (e < num) ? idx[e] : -1;
+	}
+	int64_t vi[GATHER_EPT];
 #pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) {
-			vi[u] = -1;
-			if (t[u] >= 0 && t[u] < g.N4) {
-				int i, j, k, l;
-				decode_four(g, t[u], i, j, k, l);
-				const int ci = scoord[i], cj = scoord[j], ck = scoord[k], cl = scoord[l];
-				int kk[3];
+	for (int u = 0; u < GATHER_EPT; ++u) { vi[u] = (t[u] >= 0 && t[u] < g.N4) ? gather_index(g, dc, lut, t[u]) : -1; }
+	double v[GATHER_EPT];
 #pragma unroll
-				for (int a = 0; a < 3; ++a) {
-					const int sh = 10 * a;
-					const int pi = (ci >> sh) & 1023, pj = (cj >> sh) & 1023, pk = (ck >> sh) & 1023, pl = (cl >> sh) & 1023;
-					const int aa = abs(pi - pj), bb = abs(pk - pl), cd = (pi + pj) - (pk + pl);
-					kk[a] = slut[(aa * g.R + bb) * W + cd + off];
-				}
-				int x = kk[0], y = kk[1], z = kk[2], sw;
-				if (x > y) { sw = x; x = y; y = sw; }
-				if (y > z) { sw = y; y = z; z = sw; }
-				if (x > y) { sw = x; x = y; y = sw; }
-				vi[u] = tet_index(x, y, z);
-			}
-		}
-		double v[GATHER_EPT];
+	for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
 #pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
-#pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) {
-			const int64_t e = base + (int64_t)u * blockDim.x;
-			if (e < num) { out[e] = v[u]; }
-		}
+	for (int u = 0; u < GATHER_EPT; ++u) {
+		const int64_t e = base + (int64_t)u * blockDim.x;
+		if (e < num) { out[e] = v[u]; }
 	}
 }
 
@@ -582,13 +627,12 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	else { mode_product(w.Y2.p, Cp * Cp, Cp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1); }
 	phase_mark("mode3");
 	{
-		const int nlut = (int)s.ks->lut.size();
-		const size_t shm = (size_t)s.g.N * sizeof(int) + (size_t)nlut * sizeof(short);
-		GLI_REQUIRE(shm <= 64 * 1024, "grid too large for the LDS-staged gather tables");
-		int cus = 256;
-		GLI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
-		const int64_t blocks = std::min<int64_t>(ceil_div(num, 256 * GATHER_EPT), (int64_t)cus * 8);
-		k_gather<<<(unsigned)blocks, 256, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut16_dev.p, nlut, w.coords.p, w.V.p, d_vals);
+		DecodeConsts dc;
+		dc.inv_N2 = 1.0 / (double)s.g.N2;
+		dc.inv_N = 1.0f / (float)s.g.N;
+		dc.inv_nz = 1.0f / (float)s.g.n[2];
+		dc.inv_nyz = 1.0f / (float)(s.g.n[1] * s.g.n[2]);
+		k_gather<<<(unsigned)ceil_div(num, 256 * GATHER_EPT), 256, 0, c.stream>>>(d_idx, num, s.g, dc, s.ks->lut16_dev.p, w.V.p, d_vals);
 	}
 	GLI_HIP(hipGetLastError());
 	phase_mark("gather");

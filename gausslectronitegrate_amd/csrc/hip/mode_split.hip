@@ -2,14 +2,16 @@
 //
 // Y[p][k] = sum_{mu < Kd} X[mu][p] B[mu][k] is evaluated from base-256 digit
 // slices (the error-free splitting of Ozaki, Ogita, Oishi and Rump, 2012):
-// every column p of X and k of B is scaled by a power of two 2^e to |x| < 1
-// and written as x = sum_s d_s 256^-(s+1) + r with integer digits
-// |d_s| <= 255, which BF16 holds exactly.  A slice product sum_mu d_s d'_t
-// is an integer below 255^2 * Kd < 2^24 for Kd <= 256, so the FP32 MFMA
-// accumulation is exact; the products with s + t <= NS - 1 are combined in
-// FP64 with weights 256^-(s+t+2) 2^(e_p + e_k).  With NS = 5 digits the
-// dropped terms are below 2^-40 of the column scales (about 1e-11 absolute on
-// the ERI key tensor of an 11^3 grid with K = 57).
+// every column p of X and k of B is scaled by a power of two to |x| < 1/2 and
+// written with round-to-nearest digits as x = sum_s d_s 256^-(s+1) + r,
+// |d_s| <= 128 (exact in BF16), |r| <= 256^-NS / 2.  A slice product
+// d_s d'_t is at most 2^14; the products with the same digit sum s + t = d
+// (at most NS of them) share one FP32 MFMA accumulator, whose partial sums
+// stay integers below 2^24 over a chunk of KC = 192 terms, hence exact.  Each
+// chunk is flushed into one FP64 accumulator with weights 256^-(d+2), and the
+// column scales are applied at the end.  With NS = 5 the dropped terms are
+// below about 2^-40 of the column scales (near 1e-11 absolute on the ERI key
+// tensor of an 11^3 grid with K = 57).
 #include "common.hpp"
 #include "../gli_device.h"
 
@@ -19,7 +21,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int NS = 5;                         // digit slices per operand
-constexpr int NPROD = NS * (NS + 1) / 2;      // slice products with s + t <= NS - 1
 constexpr int SBM = 64, SBN = 64, SBK = 32;   // block tile; K step = one MFMA 16x16x32
 constexpr int SPAD = 8;                       // LDS row padding (bf16 elements)
 
@@ -34,7 +35,7 @@ __global__ void k_column_exponents(const double* __restrict__ X, int64_t ld, int
 	}
 	int ex = 0;
 	if (mx > 0 && isfinite(mx)) { (void)frexp(mx, &ex); }  // mx = f 2^ex, f in [1/2, 1)
-	e[p] = ex;
+	e[p] = ex + 1;  // |X / 2^e| < 1/2
 }
 
 // base-256 digits of a 32 (mu) x 64 (p) tile, written as D[(s * Ppad + p) * Kp + mu]
@@ -52,9 +53,9 @@ __global__ __launch_bounds__(256) void k_split_tile(const double* __restrict__ X
 		double x = (p < P && mu < Kd) ? ldexp(X[(int64_t)mu * ld + p], -e[p]) : 0.0;
 		for (int sl = 0; sl < NS; ++sl) {
 			x *= 256.0;
-			const double d = trunc(x);
+			const double d = rint(x);  // |x| < 128 before the first digit, <= 128 after
 			x -= d;
-			const float f = (float)d;  // |d| <= 255: exact in the top 16 bits of the float
+			const float f = (float)d;  // |d| <= 128: exact in the top 16 bits of the float
 			T[sl][r][c] = (uint16_t)(__float_as_uint(f) >> 16);
 		}
 	}
@@ -66,6 +67,9 @@ __global__ __launch_bounds__(256) void k_split_tile(const double* __restrict__ X
 }
 
 // Y[p][k] from the digits of X (Ppad columns) and B (Cpad columns)
+constexpr int KC = 192;  // exactness bound of an FP32 group accumulator: NS * 2^14 * KC < 2^24
+
+template <bool kOneChunk>  // true when the whole sum fits one exact chunk (Kp <= KC)
 __global__ __launch_bounds__(256) void k_mode_product_split(
 	const uint16_t* __restrict__ Xd, int64_t Ppad, const int* __restrict__ ex,
 	const uint16_t* __restrict__ Bd, int64_t Cpad, const int* __restrict__ eb,
@@ -88,71 +92,94 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 	}
 
 	const int wm = wave >> 1, wn = wave & 1;
-	f4 acc[NPROD][2][2];
-	for (int q = 0; q < NPROD; ++q) {
-		for (int i = 0; i < 2; ++i) {
-			for (int j = 0; j < 2; ++j) { acc[q][i][j] = f4{0, 0, 0, 0}; }
+	f4 grp[NS][2][2];                   // FP32 group accumulators, one per digit sum
+	double y[2][2][kOneChunk ? 1 : 4];  // FP64 accumulators across chunks
+	for (int i = 0; i < 2; ++i) {
+		for (int j = 0; j < 2; ++j) {
+			for (int v = 0; v < (kOneChunk ? 1 : 4); ++v) { y[i][j][v] = 0; }
+			for (int d = 0; d < NS; ++d) { grp[d][i][j] = f4{0, 0, 0, 0}; }
 		}
 	}
 
+	// register prefetch of the next chunk: NS * 64 rows * 4 runs of 8 digits per operand
+	constexpr int PER = NS * SBM * (SBK / 8) / 256;
+	uint4 ra[PER], rb[PER];
+	auto gload = [&](int mu0) {
+#pragma unroll
+		for (int r = 0; r < PER; ++r) {
+			const int id = tid + 256 * r;
+			const int sl = id / (SBM * 4), row = (id / 4) % SBM, c8 = (id % 4) * 8;
+			ra[r] = *(const uint4*)&Xd[((int64_t)sl * Ppad + p0 + row) * Kp + mu0 + c8];
+			rb[r] = *(const uint4*)&Bd[((int64_t)sl * Cpad + k0 + row) * Kp + mu0 + c8];
+		}
+	};
+	gload(0);
 	for (int mu0 = 0; mu0 < Kp; mu0 += SBK) {
-		// 16-byte loads: NS * 64 rows * 32 digits = NS * 256 chunks of 8 digits per operand
-		for (int id = tid; id < NS * SBM * (SBK / 8); id += 256) {
-			const int s = id / (SBM * 4), r = (id / 4) % SBM, c8 = (id % 4) * 8;
-			const uint4 va = *(const uint4*)&Xd[((int64_t)s * Ppad + p0 + r) * Kp + mu0 + c8];
-			*(uint4*)&As[s][r][c8] = va;
-			const uint4 vb = *(const uint4*)&Bd[((int64_t)s * Cpad + k0 + r) * Kp + mu0 + c8];
-			*(uint4*)&Bs[s][r][c8] = vb;
+#pragma unroll
+		for (int r = 0; r < PER; ++r) {
+			const int id = tid + 256 * r;
+			const int sl = id / (SBM * 4), row = (id / 4) % SBM, c8 = (id % 4) * 8;
+			*(uint4*)&As[sl][row][c8] = ra[r];
+			*(uint4*)&Bs[sl][row][c8] = rb[r];
 		}
 		__syncthreads();
+		if (mu0 + SBK < Kp) { gload(mu0 + SBK); }
 		bf16x8 b[NS][2];
 		for (int t = 0; t < NS; ++t) {
 			for (int j = 0; j < 2; ++j) { b[t][j] = *(const bf16x8*)&Bs[t][wn * 32 + j * 16 + lane % 16][8 * (lane / 16)]; }
 		}
-		int q = 0;
-		for (int s = 0; s < NS; ++s) {
+		for (int sl = 0; sl < NS; ++sl) {
 			bf16x8 a[2];
-			for (int i = 0; i < 2; ++i) { a[i] = *(const bf16x8*)&As[s][wm * 32 + i * 16 + lane % 16][8 * (lane / 16)]; }
-			for (int t = 0; t < NS - s; ++t, ++q) {
+			for (int i = 0; i < 2; ++i) { a[i] = *(const bf16x8*)&As[sl][wm * 32 + i * 16 + lane % 16][8 * (lane / 16)]; }
+			for (int t = 0; t < NS - sl; ++t) {
 				for (int i = 0; i < 2; ++i) {
 					for (int j = 0; j < 2; ++j) {
-						acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[t][j], acc[q][i][j], 0, 0, 0);
+						grp[sl + t][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[t][j], grp[sl + t][i][j], 0, 0, 0);
 					}
 				}
 			}
 		}
 		__syncthreads();
+		// flush the exact group sums into FP64 at the end of every KC chunk and at the end
+		if (!kOneChunk && (((mu0 + SBK) % KC) == 0 || mu0 + SBK >= Kp)) {
+			for (int i = 0; i < 2; ++i) {
+				for (int j = 0; j < 2; ++j) {
+					for (int v = 0; v < 4; ++v) {
+						double acc = 0;
+						for (int d = NS - 1; d >= 0; --d) { acc += ldexp((double)grp[d][i][j][v], -8 * (d + 2)); }
+						y[i][j][kOneChunk ? 0 : v] += acc;
+					}
+					for (int d = 0; d < NS; ++d) { grp[d][i][j] = f4{0, 0, 0, 0}; }
+				}
+			}
+		}
 	}
 
-	// FP64 combination, smallest weights first
+	// accumulator layout of the 16x16 FP32 block: lane holds column lane%16, rows 4*(lane/16)+v
 	for (int i = 0; i < 2; ++i) {
 		for (int j = 0; j < 2; ++j) {
 			const int col = k0 + wn * 32 + j * 16 + lane % 16;
 			for (int v = 0; v < 4; ++v) {
 				const int64_t row = p0 + wm * 32 + i * 16 + 4 * (lane / 16) + v;
 				if (row >= P || col >= C) { continue; }
-				double y = 0;
-				for (int d = NS - 1; d >= 0; --d) {
-					// products with s + t = d
-					double g = 0;
-					int qq = 0;
-					for (int s = 0; s < NS; ++s) {
-						for (int t = 0; t < NS - s; ++t, ++qq) {
-							if (s + t == d) { g += (double)acc[qq][i][j][v]; }
-						}
-					}
-					y += ldexp(g, -8 * (d + 2));
+				double acc;
+				if (kOneChunk) {
+					acc = 0;
+					for (int d = NS - 1; d >= 0; --d) { acc += ldexp((double)grp[d][i][j][v], -8 * (d + 2)); }
 				}
-				y = ldexp(y, ex[row] + eb[col]);
+				else {
+					acc = y[i][j][kOneChunk ? 0 : v];
+				}
+				const double val = ldexp(acc, ex[row] + eb[col]);
 				if (tet_C1 > 0) {
 					const int64_t gp = p_base + row;
 					const int k1 = (int)(gp / Cp), k2 = (int)(gp % Cp);
 					if (k1 <= k2 && k2 <= col && col < tet_C1) {
-						Y[(int64_t)col * (col + 1) * (col + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1] = y;
+						Y[(int64_t)col * (col + 1) * (col + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1] = val;
 					}
 				}
 				else {
-					Y[row * ldy + col] = y;
+					Y[row * ldy + col] = val;
 				}
 			}
 		}
@@ -196,7 +223,6 @@ void split_factor(const double* B, int64_t ldb, int C, int Kd)
 void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, double* Y, int64_t ldy,
                         int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, bool cache_x)
 {
-	GLI_REQUIRE(Kd <= 256, "split-precision contraction needs at most 256 terms per sum");
 	if (P <= 0 || C <= 0) { return; }
 	SplitWorkspace& w = split_ws();
 	const int Kp = (int)round_up(Kd, SBK);
@@ -221,8 +247,14 @@ void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, 
 		xe = w.xe.p;
 	}
 	dim3 grid((unsigned)(Ppad / SBM), (unsigned)(Cpad / SBN));
-	k_mode_product_split<<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,
-	                                                     tri_cols, tri_rows, p_base, tet_C1);
+	if (Kp <= KC) {
+		k_mode_product_split<true><<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,
+		                                                           tri_cols, tri_rows, p_base, tet_C1);
+	}
+	else {
+		k_mode_product_split<false><<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,
+		                                                            tri_cols, tri_rows, p_base, tet_C1);
+	}
 	GLI_HIP(hipGetLastError());
 }
 

@@ -310,7 +310,7 @@ def test_project_and_apply(L):
 # ---------------------------------------------------------------------------
 # split-precision (mixed) path: error <= 1e-10 absolute against FP64
 
-@pytest.mark.parametrize("Kd,P,Cc", [(32, 64, 64), (176, 300, 130), (149, 1000, 77)])
+@pytest.mark.parametrize("Kd,P,Cc", [(32, 64, 64), (176, 300, 130), (149, 1000, 77), (300, 200, 100), (520, 70, 90)])
 def test_split_mode_product_vs_numpy(L, Kd, P, Cc):
     """BF16 digit slices with exact FP32 sums: error bounded by ~2^-40 of the column scales."""
     say("split mode product", Kd, P, Cc)
