@@ -57,6 +57,36 @@ def eri_step(L, c, g, d_idx, num, tol, method, d_val):
                                        d_idx, num, tol, method, d_val), "gli_eri_values_device")
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 counter summary, else None.
+
+    The summary line is `traffic_bytes_per_launch: <bytes>`: FETCH_SIZE + WRITE_SIZE of one dispatch,
+    each scaled by the ratio the same counter shows for an elementwise kernel of known size."""
+    path = os.path.join(REPO, "profiles", "%s_%s_traffic.md" % (kernel, workload))
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].strip().split()[0])
+    except OSError:
+        return None
+    return None
+
+
+def calibrate_copy(nbytes):
+    """One elementwise kernel of known traffic (read nbytes, write nbytes), for counter calibration."""
+    import torch
+    n = int(nbytes) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "torch.neg float64", "bytes_read": 8 * n, "bytes_written": 8 * n, "launches": 3}),
+          flush=True)
+
+
 def phase_times(L):
     ms = (C.c_double * 16)()
     names = (C.c_char_p * 16)()
@@ -124,7 +154,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--asymmetric", action="store_true",
                     help="coefficients without mirror symmetry (general axis-key set, GEMM-dominated)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip timing the split-precision method beside the FP64 one")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
+                    help="only run an elementwise kernel of known traffic (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would take a contiguous, "
               "cost-balanced slice of the sorted index list); run with --gpus 1", file=sys.stderr)
@@ -154,8 +191,11 @@ def main():
         eri_step(L, c, g, d_idx, num, args.tol, method, d_val.ptr)
     L.gli_synchronize()
     torch.cuda.synchronize()
+    # the library records HIP events at its phase boundaries on its own stream during the timed
+    # steps (no synchronisation between steps); phases are averaged over the K steps
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
+    L.gli_enable_phase_timing(1)
     t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
@@ -164,18 +204,10 @@ def main():
     L.gli_synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    phases = phase_times(L)
+    L.gli_enable_phase_timing(0)
     step_ms = ev0.elapsed_time(ev1) / args.steps
     log("step: %.3f ms (events), wall %.3f ms/step" % (step_ms, wall * 1e3 / args.steps))
-
-    # per-phase HIP events over a few extra steps: dominant kernel duration
-    L.gli_enable_phase_timing(1)
-    phases = {}
-    reps = 5
-    for _ in range(reps):
-        eri_step(L, c, g, d_idx, num, args.tol, method, d_val.ptr)
-        for k, v in phase_times(L).items():
-            phases[k] = phases.get(k, 0.0) + v / reps
-    L.gli_enable_phase_timing(0)
     log("phases (ms):", {k: round(v, 4) for k, v in phases.items()})
 
     # per-element FP64 path on a seeded sample, and errors against it
@@ -207,7 +239,37 @@ def main():
     direct_rate = len(s_idx) / (direct_ms / 1e3)
     log("direct FP64 sample: %d elements in %.2f ms; max abs err %.3g" % (len(s_idx), direct_ms, err_abs))
 
+    # the split-precision method on the same workload: time and error over every element
+    compare = None
+    if not args.no_compare and method != METHODS["mixed"]:
+        mixed = METHODS["mixed"]
+        d_mix = Dev(L, 8 * num)
+        eri_step(L, c, g, d_idx, num, args.tol, mixed, d_mix.ptr)
+        L.gli_synchronize()
+        torch.cuda.synchronize()
+        m0 = torch.cuda.Event(enable_timing=True)
+        m1 = torch.cuda.Event(enable_timing=True)
+        m0.record(stream)
+        for _ in range(args.steps):
+            eri_step(L, c, g, d_idx, num, args.tol, mixed, d_mix.ptr)
+        m1.record(stream)
+        L.gli_synchronize()
+        torch.cuda.synchronize()
+        mix_ms = m0.elapsed_time(m1) / args.steps
+        mix_vals = np.empty(num)
+        _lib.check(L.gli_memcpy(mix_vals.ctypes.data, d_mix.ptr, 8 * num, 1), "copy mixed values")
+        del d_mix
+        diff = np.abs(mix_vals - vals)
+        big = np.abs(vals) >= 1e-10
+        compare = {"method": "mixed", "ms_per_step": mix_ms, "value": num / (mix_ms / 1e3),
+                   "max_abs_err_vs_fp64_all_elements": float(diff.max()),
+                   "max_rel_err_vs_fp64_where_abs_ge_1e-10": float((diff[big] / np.abs(vals[big])).max()) if big.any() else None,
+                   "max_abs_err_vs_fp64_direct_sample": float(np.max(np.abs(mix_vals[samp] - direct_vals)))}
+        del mix_vals, diff, big
+        log("mixed: %.3f ms/step, max abs err vs FP64 over all elements %.3g" % (mix_ms, compare["max_abs_err_vs_fp64_all_elements"]))
+
     # roofline of the dominant kernel
+    workload = "eri_%dx%dx%d_K%d_seed%d%s" % (args.n, args.n, args.n, args.K, args.seed, "_asym" if args.asymmetric else "")
     dom = max(phases.items(), key=lambda kv: kv[1]) if phases else ("step", step_ms)
     name, dom_ms = dom
     M = 6 * (args.n - 1) + 2 * (args.K - 1)
@@ -217,7 +279,8 @@ def main():
     if name == "gather":
         achieved = 16.0 * num / (dom_ms / 1e3) / 1e9   # index in + value out, bytes per element
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_gather"}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_gather", workload),
+                "kernel": "k_gather", "algorithmic_bytes_per_launch": 16 * num}
     else:
         flops = {"mode1": 2.0 * Mp * Mp * Cp * Mp, "mode2": 2.0 * Mp * Cp * Cp * Mp / 2,
                  "mode3": 2.0 * Cp * Cp * Cp * Mp / 6}.get(name, 0.0)
@@ -249,8 +312,7 @@ def main():
         "vs_baseline": None,
         "dtype": {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64"}[args.method],
         "data": "synthetic (seeded gausslet coefficients)",
-        "config": {"workload": "eri_%dx%dx%d_K%d_seed%d%s" % (args.n, args.n, args.n, args.K, args.seed,
-                                                             "_asym" if args.asymmetric else ""),
+        "config": {"workload": workload,
                    "grid": args.n, "K": args.K, "seed": args.seed, "tol": args.tol, "method": args.method,
                    "unique_elements": num, "axis_keys": C1},
         "roofline": roof,
@@ -263,6 +325,7 @@ def main():
         "max_abs_err_vs_cpu_reference": cpu_err,
         "speedup_vs_fp64_direct": value / direct_rate,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "split_precision": compare,
     }
     L.gli_device_free(d_idx)
     print(json.dumps(line), flush=True)

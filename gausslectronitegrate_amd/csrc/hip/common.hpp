@@ -92,12 +92,33 @@ struct Context {
 	// ERIDA kernel table, w = sqrt(2)/3, rho = 9 r^2
 	std::vector<double> erida_table_host;
 	DevBuf<double> erida_table_dev;
-	// events of the last ERI value evaluation, one per phase boundary
-	static constexpr int kMaxPhases = 16;
-	hipEvent_t phase_ev[kMaxPhases + 1] = {};
-	const char* phase_name[kMaxPhases] = {};
-	int num_phases = 0;
+	// phase timing: a ring of event sets, one per evaluation, so that back-to-back evaluations
+	// are timed without synchronising between them; durations accumulate per phase name
+	static constexpr int kMaxPhases = 16, kPhaseSets = 64, kMaxPhaseNames = 32;
+	struct PhaseSet {
+		hipEvent_t ev[kMaxPhases + 1] = {};
+		const char* name[kMaxPhases] = {};
+		int n = 0;
+		bool pending = false;
+	};
+	PhaseSet* phase_sets = nullptr;   // created on first enable
+	int phase_cur = -1, phase_next = 0;
 	bool phases_on = false;
+	const char* acc_name[kMaxPhaseNames] = {};
+	double acc_ms[kMaxPhaseNames] = {};
+	int acc_calls[kMaxPhaseNames] = {};
+	int acc_n = 0;
+	// pinned staging ring for small host-to-device uploads (coefficients), so that an upload
+	// never waits for earlier work on the stream
+	static constexpr int kStageSlots = 32;
+	static constexpr size_t kStageBytes = 8192;
+	char* stage = nullptr;
+	hipEvent_t stage_ev[kStageSlots] = {};
+	bool stage_used[kStageSlots] = {};
+	int stage_next = 0;
+	// kernel tuning knobs (gli_set_tuning)
+	int tile_grouped = 1;   // mode-product tile order
+	int gather_ept = 4;     // gather elements per lane
 };
 
 Context& ctx();
@@ -105,6 +126,37 @@ Context& ctx();
 // ERI / ERIDA Coulomb tables of at least n entries, device resident
 const double* eri_table(int64_t n);
 const double* erida_table(int64_t n);
+
+// Output-tile order of the mode products.  Workgroups are dispatched round-robin over the 8 XCDs
+// (workgroup b runs on XCD b % 8), each with its own L2.  With the grouped order the column tiles
+// of one row stripe run back to back on the same XCD, so the stripe of the large operand X is
+// read from HBM once and then hits that XCD's L2; with the plain order (row tiles fastest) every
+// column tile re-reads X from HBM.
+struct TileOrder {
+	int nr, nc;   // row and column tiles
+	int grouped;  // 1: XCD-grouped order
+	__host__ __device__ int blocks() const { return grouped ? ((nr + 7) / 8) * 8 * nc : nr * nc; }
+	// (row tile, column tile) of workgroup b; false for padding workgroups
+	__device__ bool tile(int b, int& rt, int& ct) const
+	{
+		if (grouped) {
+			const int xcd = b & 7, local = b >> 3;
+			rt = (local / nc) * 8 + xcd;
+			ct = local % nc;
+			return rt < nr;
+		}
+		rt = b % nr;
+		ct = b / nr;
+		return true;
+	}
+};
+
+// the tile order used by the mode products (1 = XCD-grouped, the default; gli_set_tuning changes it)
+int tile_order_grouped();
+
+// asynchronous host-to-device copy of a small host buffer on the library stream (the source may
+// be reused as soon as this returns)
+void upload_async(void* dst, const void* src, size_t bytes);
 
 // phase timing (records an event on the library stream when enabled)
 void phase_begin();

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <string>
 
 namespace gli {
 
@@ -32,7 +33,6 @@ Context& ctx()
 		GLI_HIP(hipGetDevice(&c->device));
 		// default priority: non-default priorities are not used anywhere in this library
 		GLI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-		for (int i = 0; i <= Context::kMaxPhases; ++i) { GLI_HIP(hipEventCreate(&c->phase_ev[i])); }
 	});
 	return *c;
 }
@@ -85,20 +85,84 @@ const double* erida_table(int64_t n)
 	return table(c.erida_table_host, c.erida_table_dev, n, sqrt2_3, 3.L);
 }
 
+void upload_async(void* dst, const void* src, size_t bytes)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	if (bytes == 0) { return; }
+	if (bytes > Context::kStageBytes) {
+		GLI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream));
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		return;
+	}
+	if (c.stage == nullptr) {
+		GLI_HIP(hipHostMalloc((void**)&c.stage, Context::kStageSlots * Context::kStageBytes, hipHostMallocDefault));
+		for (int i = 0; i < Context::kStageSlots; ++i) { GLI_HIP(hipEventCreateWithFlags(&c.stage_ev[i], hipEventDisableTiming)); }
+	}
+	const int slot = c.stage_next;
+	c.stage_next = (slot + 1) % Context::kStageSlots;
+	if (c.stage_used[slot]) { GLI_HIP(hipEventSynchronize(c.stage_ev[slot])); }
+	char* buf = c.stage + (size_t)slot * Context::kStageBytes;
+	std::memcpy(buf, src, bytes);
+	GLI_HIP(hipMemcpyAsync(dst, buf, bytes, hipMemcpyHostToDevice, c.stream));
+	GLI_HIP(hipEventRecord(c.stage_ev[slot], c.stream));
+	c.stage_used[slot] = true;
+}
+
+int tile_order_grouped()
+{
+	return ctx().tile_grouped;
+}
+
+// fold one completed event set into the per-name sums
+static void phase_harvest(Context& c, Context::PhaseSet& set)
+{
+	if (!set.pending) { return; }
+	set.pending = false;
+	if (set.n == 0) { return; }
+	GLI_HIP(hipEventSynchronize(set.ev[set.n]));
+	for (int i = 0; i < set.n; ++i) {
+		float t = 0;
+		GLI_HIP(hipEventElapsedTime(&t, set.ev[i], set.ev[i + 1]));
+		int slot = 0;
+		while (slot < c.acc_n && std::strcmp(c.acc_name[slot], set.name[i]) != 0) { ++slot; }
+		if (slot == c.acc_n) {
+			if (c.acc_n == Context::kMaxPhaseNames) { continue; }
+			c.acc_name[c.acc_n] = set.name[i];
+			c.acc_ms[c.acc_n] = 0;
+			c.acc_calls[c.acc_n] = 0;
+			c.acc_n++;
+		}
+		c.acc_ms[slot] += t;
+		c.acc_calls[slot] += 1;
+	}
+}
+
 void phase_begin()
 {
 	Context& c = ctx();
-	c.num_phases = 0;
-	if (c.phases_on) { GLI_HIP(hipEventRecord(c.phase_ev[0], c.stream)); }
+	if (!c.phases_on) {
+		c.phase_cur = -1;
+		return;
+	}
+	Context::PhaseSet& set = c.phase_sets[c.phase_next];
+	phase_harvest(c, set);
+	set.n = 0;
+	set.pending = true;
+	GLI_HIP(hipEventRecord(set.ev[0], c.stream));
+	c.phase_cur = c.phase_next;
+	c.phase_next = (c.phase_next + 1) % Context::kPhaseSets;
 }
 
 void phase_mark(const char* name)
 {
 	Context& c = ctx();
-	if (!c.phases_on || c.num_phases >= Context::kMaxPhases) { return; }
-	c.phase_name[c.num_phases] = name;
-	c.num_phases++;
-	GLI_HIP(hipEventRecord(c.phase_ev[c.num_phases], c.stream));
+	if (!c.phases_on || c.phase_cur < 0) { return; }
+	Context::PhaseSet& set = c.phase_sets[c.phase_cur];
+	if (set.n >= Context::kMaxPhases) { return; }
+	set.name[set.n] = name;
+	set.n++;
+	GLI_HIP(hipEventRecord(set.ev[set.n], c.stream));
 }
 
 }  // namespace gli
@@ -144,10 +208,62 @@ int gli_synchronize(void)
 	}
 }
 
+int gli_set_tuning(const char* knob, int value)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const std::string k = knob ? knob : "";
+		if (k == "tile_order") {
+			GLI_REQUIRE(value == 0 || value == 1, "tile_order must be 0 or 1");
+			c.tile_grouped = value;
+		}
+		else if (k == "gather_ept") {
+			GLI_REQUIRE(value == 2 || value == 4 || value == 8, "gather_ept must be 2, 4 or 8");
+			c.gather_ept = value;
+		}
+		else {
+			GLI_REQUIRE(false, "unknown tuning knob");
+		}
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_get_tuning(const char* knob)
+{
+	try {
+		Context& c = ctx();
+		const std::string k = knob ? knob : "";
+		if (k == "tile_order") { return c.tile_grouped; }
+		if (k == "gather_ept") { return c.gather_ept; }
+		GLI_REQUIRE(false, "unknown tuning knob");
+	}
+	catch (const Failure&) {
+	}
+	return -1;
+}
+
 int gli_enable_phase_timing(int on)
 {
 	try {
-		ctx().phases_on = (on != 0);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		if (on && c.phase_sets == nullptr) {
+			c.phase_sets = new Context::PhaseSet[Context::kPhaseSets];
+			for (int s = 0; s < Context::kPhaseSets; ++s) {
+				for (int i = 0; i <= Context::kMaxPhases; ++i) { GLI_HIP(hipEventCreate(&c.phase_sets[s].ev[i])); }
+			}
+		}
+		if (on && !c.phases_on) {
+			// a fresh measurement: drop what earlier evaluations left
+			for (int s = 0; s < Context::kPhaseSets; ++s) { c.phase_sets[s].pending = false; }
+			c.acc_n = 0;
+		}
+		c.phases_on = (on != 0);
+		c.phase_cur = -1;
 		return 0;
 	}
 	catch (const Failure&) {
@@ -159,14 +275,19 @@ int gli_phase_times(double* ms, const char** names, int max)
 {
 	try {
 		Context& c = ctx();
-		GLI_HIP(hipStreamSynchronize(c.stream));
-		const int n = std::min(max, c.num_phases);
-		for (int i = 0; i < n; ++i) {
-			float t = 0;
-			GLI_HIP(hipEventElapsedTime(&t, c.phase_ev[i], c.phase_ev[i + 1]));
-			ms[i] = t;
-			if (names != nullptr) { names[i] = c.phase_name[i]; }
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		if (c.phase_sets != nullptr) {
+			// harvest oldest first
+			for (int k = 0; k < Context::kPhaseSets; ++k) {
+				phase_harvest(c, c.phase_sets[(c.phase_next + k) % Context::kPhaseSets]);
+			}
 		}
+		const int n = std::min(max, c.acc_n);
+		for (int i = 0; i < n; ++i) {
+			ms[i] = c.acc_ms[i] / std::max(c.acc_calls[i], 1);
+			if (names != nullptr) { names[i] = c.acc_name[i]; }
+		}
+		c.acc_n = 0;
 		return n;
 	}
 	catch (const Failure&) {

@@ -122,14 +122,16 @@ __host__ __device__ inline int64_t tet_index(int k1, int k2, int k3)
 
 __global__ __launch_bounds__(256) void k_mode_product(
 	const double* __restrict__ X, int64_t ldx, int64_t P, const double* __restrict__ B, int64_t ldb, int C, int Kd,
-	double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1)
+	double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, TileOrder order)
 {
 	__shared__ double As[GBK][GBM];
 	__shared__ double Bs[GBK][GBN];
 	const int tid = threadIdx.x;
 	const int lane = tid & 63, wave = tid >> 6;
-	const int64_t p0 = (int64_t)blockIdx.x * GBM;
-	const int k0 = blockIdx.y * GBN;
+	int rt, ct;
+	if (!order.tile(blockIdx.x, rt, ct)) { return; }
+	const int64_t p0 = (int64_t)rt * GBM;
+	const int k0 = ct * GBN;
 
 	if (tri_cols || tri_rows) {
 		// global row index (with p_base) decomposes as outer * Cp + r; Cp % GBM == 0 keeps outer fixed per tile
@@ -280,6 +282,12 @@ struct DecodeConsts {
 	float inv_N;     // 1 / N
 	float inv_nz;    // 1 / n_z
 	float inv_nyz;   // 1 / (n_y n_z)
+	// reciprocals rounded up by one ulp: for a < 2^12 (float) and a < 2^24 (double, divisor
+	// N) resp. a < 2^48 (divisor N^2), trunc(a * inv_up) is the exact quotient, no correction
+	// step needed (exhaustively checked for every divisor up to 4096)
+	double inv_N2_up, inv_N_up;
+	float inv_nyz_up, inv_nz_up;
+	int uniform_pairs;   // 1: grids with N <= 4096 and tet indices below 2^32 take the fast path
 };
 
 // tetrahedral V index of a four-index; exact arithmetic decode (no table loads but the key table)
@@ -336,29 +344,109 @@ __device__ inline int64_t gather_index(const GridDev& g, const DecodeConsts& dc,
 	return tet_index(x, y, z);
 }
 
-// values[e] = V[k1][k2][k3] from the tetrahedral key tensor; GATHER_EPT independent elements per
-// thread keep several dependent load chains (index -> key table -> V) in flight
-constexpr int GATHER_EPT = 4;
-
-__global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g, DecodeConsts dc,
-	const short* __restrict__ lut, const double* __restrict__ V, double* __restrict__ out)
+// values[e] = V[k1][k2][k3] from the tetrahedral key tensor.  Each wave takes GATHER_EPT groups
+// of 64 consecutive elements (coalesced index loads and value stores, several dependent load
+// chains index -> key table -> V in flight per lane).  In a sorted list the leading pair (i, j)
+// changes only every few thousand elements, so it is decoded once per wave; a group whose lanes
+// all share it decodes only (k, l) per lane with exact reciprocal multiplies.  Any other group
+// (a pair boundary, the tail, unsorted or out-of-range input) takes the general per-lane decode.
+__device__ inline int64_t uniform_i64(int64_t v)
 {
-	const int64_t base = (int64_t)blockIdx.x * blockDim.x * GATHER_EPT + threadIdx.x;
+	const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+	const int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+	return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// packed coordinates of a point p < 2^12: exact float quotients (see DecodeConsts)
+__device__ inline void point_xyz(unsigned p, const GridDev& g, const DecodeConsts& dc, unsigned& x, unsigned& y, unsigned& z)
+{
+	const unsigned nyz = (unsigned)(g.n[1] * g.n[2]);
+	x = (unsigned)((float)p * dc.inv_nyz_up);
+	const unsigned rem = p - __umul24(x, nyz);
+	y = (unsigned)((float)rem * dc.inv_nz_up);
+	z = rem - __umul24(y, (unsigned)g.n[2]);
+}
+
+template <int GATHER_EPT>
+__global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g, DecodeConsts dc,
+	const short* __restrict__ lut, const uint32_t* __restrict__ tet3, const double* __restrict__ V, double* __restrict__ out)
+{
+	const int lane = threadIdx.x & 63;
+	const int64_t wave_base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * GATHER_EPT);
 	int64_t t[GATHER_EPT];
 #pragma unroll
 	for (int u = 0; u < GATHER_EPT; ++u) {
-		const int64_t e = base + (int64_t)u * blockDim.x;
+		const int64_t e = wave_base + u * 64 + lane;
 		t[u] = (e < num) ? idx[e] : -1;
 	}
-	int64_t vi[GATHER_EPT];
+	// the wave's pair (i, j) from its first element
+	const int W = 4 * g.R - 3, off = 2 * (g.R - 1);
+	const int64_t t0 = uniform_i64(t[0]);
+	int64_t pair_base = -1;   // (i N + j) N^2, or -1 when the fast path is off for this wave
+	int base_key[3] = { 0, 0, 0 };
+	if (dc.uniform_pairs && t0 >= 0 && t0 < g.N4) {
+		const int64_t ij = (int64_t)((double)t0 * dc.inv_N2_up);
+		const unsigned i = (unsigned)((double)(unsigned)ij * dc.inv_N_up);
+		const unsigned j = (unsigned)ij - __umul24(i, (unsigned)g.N);
+		unsigned ci[3], cj[3];
+		point_xyz(i, g, dc, ci[0], ci[1], ci[2]);
+		point_xyz(j, g, dc, cj[0], cj[1], cj[2]);
 #pragma unroll
-	for (int u = 0; u < GATHER_EPT; ++u) { vi[u] = (t[u] >= 0 && t[u] < g.N4) ? gather_index(g, dc, lut, t[u]) : -1; }
+		for (int a = 0; a < 3; ++a) {
+			const int d = abs((int)ci[a] - (int)cj[a]);
+			base_key[a] = __builtin_amdgcn_readfirstlane(d * g.R * W + (int)(ci[a] + cj[a]) + off);
+		}
+		pair_base = uniform_i64(ij * g.N2);
+	}
+	bool fast = pair_base >= 0;
+#pragma unroll
+	for (int u = 0; u < GATHER_EPT; ++u) { fast = fast && (uint64_t)(t[u] - pair_base) < (uint64_t)g.N2; }
 	double v[GATHER_EPT];
+	if (__all(fast)) {
+		// all groups of the wave share the pair: phase the loads across groups so that the
+		// in-order load counter never makes one group's V read wait for another's key reads
+		const unsigned short* __restrict__ ulut = (const unsigned short*)lut;
+		unsigned kk[GATHER_EPT][3];
 #pragma unroll
-	for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
+		for (int u = 0; u < GATHER_EPT; ++u) {
+			const unsigned kl = (unsigned)(t[u] - pair_base);
+			const unsigned k = (unsigned)((double)kl * dc.inv_N_up);
+			const unsigned l = kl - __umul24(k, (unsigned)g.N);
+			unsigned ck[3], cl[3];
+			point_xyz(k, g, dc, ck[0], ck[1], ck[2]);
+			point_xyz(l, g, dc, cl[0], cl[1], cl[2]);
+#pragma unroll
+			for (int a = 0; a < 3; ++a) {
+				// (|ci - cj| R + |ck - cl|) W + (ci + cj) - (ck + cl) + 2(R - 1), with the pair part uniform
+				const unsigned li = __umul24(__usad(ck[a], cl[a], 0u), (unsigned)W) + ((unsigned)base_key[a] - ck[a] - cl[a]);
+				kk[u][a] = ulut[li];
+			}
+		}
+		uint32_t vi[GATHER_EPT];
+		unsigned k3[GATHER_EPT];
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) {
+			const unsigned lo = min(kk[u][0], kk[u][1]), hi = max(kk[u][0], kk[u][1]);
+			const unsigned mid = min(hi, kk[u][2]);
+			k3[u] = max(hi, kk[u][2]);
+			const unsigned k1 = min(lo, mid), k2 = max(lo, mid);
+			vi[u] = (__umul24(k2, k2 + 1u) >> 1) + k1;
+		}
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] += tet3[k3[u]]; }
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = V[vi[u]]; }
+	}
+	else {
+		int64_t vi[GATHER_EPT];
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] = (t[u] >= 0 && t[u] < g.N4) ? gather_index(g, dc, lut, t[u]) : -1; }
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
+	}
 #pragma unroll
 	for (int u = 0; u < GATHER_EPT; ++u) {
-		const int64_t e = base + (int64_t)u * blockDim.x;
+		const int64_t e = wave_base + u * 64 + lane;
 		if (e < num) { out[e] = v[u]; }
 	}
 }
@@ -434,6 +522,8 @@ struct KeySet {
 	DevBuf<int4> keys_dev;
 	DevBuf<int> lut_dev;
 	DevBuf<short> lut16_dev;  // the same table as int16 (key ids stay below 32768)
+	DevBuf<uint32_t> tet3_dev;  // tet_index(0, 0, k) per key
+	bool tet32 = false;         // every tetrahedral index fits 32 bits
 };
 
 static KeySet& key_set(int R, bool symmetric)
@@ -479,6 +569,12 @@ static KeySet& key_set(int R, bool symmetric)
 	std::vector<short> l16(ks->lut.begin(), ks->lut.end());
 	ks->lut16_dev.alloc(l16.size());
 	GLI_HIP(hipMemcpy(ks->lut16_dev.p, l16.data(), l16.size() * sizeof(short), hipMemcpyHostToDevice));
+	// C(k3 + 2, 3): tetrahedral offset of the largest key, as long as every index fits 32 bits
+	ks->tet32 = tet_index(0, 0, ks->C1) <= (int64_t)UINT32_MAX;
+	std::vector<uint32_t> t3((size_t)ks->C1);
+	for (int k = 0; k < ks->C1; ++k) { t3[k] = ks->tet32 ? (uint32_t)tet_index(0, 0, k) : 0u; }
+	ks->tet3_dev.alloc(t3.size());
+	GLI_HIP(hipMemcpy(ks->tet3_dev.p, t3.data(), t3.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 	cache[{R, symmetric}] = ks;
 	return *ks;
 }
@@ -547,7 +643,7 @@ static EriSetup eri_tables(const double* coeffs, int64_t K, const int64_t grid[6
 	w.P.reserve((size_t)s.R * s.R * LP);
 	w.B.reserve((size_t)s.Mp * s.Cp);
 	w.coords.reserve((size_t)s.g.N);
-	GLI_HIP(hipMemcpyAsync(w.coeffs.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, c.stream));
+	upload_async(w.coeffs.p, coeffs, K * sizeof(double));
 	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, 0, c.stream>>>(w.coeffs.p, s.K, w.f.p);
 	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, 0, c.stream>>>(w.f.p, s.K, s.R, tol, w.P.p);
 	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, c.stream>>>(w.P.p, ks.keys_dev.p, s.C1, s.K, s.R, s.M, s.Cp, w.B.p);
@@ -560,9 +656,10 @@ void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int6
                   double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1)
 {
 	if (P <= 0 || C <= 0) { return; }
-	dim3 grid((unsigned)ceil_div(P, GBM), (unsigned)ceil_div(C, GBN));
-	GLI_REQUIRE(grid.x < 2147483647u, "mode product too large");
-	k_mode_product<<<grid, 256, 0, ctx().stream>>>(X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows, p_base, tet_C1);
+	GLI_REQUIRE(ceil_div(P, GBM) * ceil_div(C, GBN) < 2147483647 - 8 * ceil_div(C, GBN), "mode product too large");
+	TileOrder order{(int)ceil_div(P, GBM), (int)ceil_div(C, GBN), tile_order_grouped()};
+	k_mode_product<<<(unsigned)order.blocks(), 256, 0, ctx().stream>>>(X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows,
+	                                                                   p_base, tet_C1, order);
 	GLI_HIP(hipGetLastError());
 }
 
@@ -632,7 +729,16 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 		dc.inv_N = 1.0f / (float)s.g.N;
 		dc.inv_nz = 1.0f / (float)s.g.n[2];
 		dc.inv_nyz = 1.0f / (float)(s.g.n[1] * s.g.n[2]);
-		k_gather<<<(unsigned)ceil_div(num, 256 * GATHER_EPT), 256, 0, c.stream>>>(d_idx, num, s.g, dc, s.ks->lut16_dev.p, w.V.p, d_vals);
+		dc.inv_N2_up = std::nextafter(1.0 / (double)s.g.N2, HUGE_VAL);
+		dc.inv_N_up = std::nextafter(1.0 / (double)s.g.N, HUGE_VAL);
+		dc.inv_nyz_up = std::nextafterf(1.0f / (float)(s.g.n[1] * s.g.n[2]), HUGE_VALF);
+		dc.inv_nz_up = std::nextafterf(1.0f / (float)s.g.n[2], HUGE_VALF);
+		dc.uniform_pairs = (s.g.N <= 4096 && s.ks->tet32) ? 1 : 0;
+		const int ept = c.gather_ept;
+		const unsigned blocks = (unsigned)ceil_div(num, 256 * ept);
+		if (ept == 2) { k_gather<2><<<blocks, 256, 0, c.stream>>>(d_idx, num, s.g, dc, s.ks->lut16_dev.p, s.ks->tet3_dev.p, w.V.p, d_vals); }
+		else if (ept == 8) { k_gather<8><<<blocks, 256, 0, c.stream>>>(d_idx, num, s.g, dc, s.ks->lut16_dev.p, s.ks->tet3_dev.p, w.V.p, d_vals); }
+		else { k_gather<4><<<blocks, 256, 0, c.stream>>>(d_idx, num, s.g, dc, s.ks->lut16_dev.p, s.ks->tet3_dev.p, w.V.p, d_vals); }
 	}
 	GLI_HIP(hipGetLastError());
 	phase_mark("gather");

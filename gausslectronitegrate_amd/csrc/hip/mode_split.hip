@@ -74,14 +74,16 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 	const uint16_t* __restrict__ Xd, int64_t Ppad, const int* __restrict__ ex,
 	const uint16_t* __restrict__ Bd, int64_t Cpad, const int* __restrict__ eb,
 	int64_t P, int C, int Kp, double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base,
-	int tet_C1)
+	int tet_C1, TileOrder order)
 {
 	__shared__ uint16_t As[NS][SBM][SBK + SPAD];
 	__shared__ uint16_t Bs[NS][SBN][SBK + SPAD];
 	const int tid = threadIdx.x;
 	const int lane = tid & 63, wave = tid >> 6;
-	const int64_t p0 = (int64_t)blockIdx.x * SBM;
-	const int k0 = blockIdx.y * SBN;
+	int rt, ct;
+	if (!order.tile(blockIdx.x, rt, ct)) { return; }
+	const int64_t p0 = (int64_t)rt * SBM;
+	const int k0 = ct * SBN;
 
 	if (tri_cols || tri_rows) {
 		const int64_t gp = p_base + p0;
@@ -246,14 +248,15 @@ void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, 
 		xd = w.xd.p;
 		xe = w.xe.p;
 	}
-	dim3 grid((unsigned)(Ppad / SBM), (unsigned)(Cpad / SBN));
+	const TileOrder order{(int)(Ppad / SBM), (int)(Cpad / SBN), tile_order_grouped()};
+	const unsigned grid = (unsigned)order.blocks();
 	if (Kp <= KC) {
 		k_mode_product_split<true><<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,
-		                                                           tri_cols, tri_rows, p_base, tet_C1);
+		                                                           tri_cols, tri_rows, p_base, tet_C1, order);
 	}
 	else {
 		k_mode_product_split<false><<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,
-		                                                            tri_cols, tri_rows, p_base, tet_C1);
+		                                                            tri_cols, tri_rows, p_base, tet_C1, order);
 	}
 	GLI_HIP(hipGetLastError());
 }

@@ -303,7 +303,7 @@ static void erida_values_dev(const double* coeffs, int64_t K, const int64_t g[6]
 	const int Mp = (int)round_up(M + 1, 16);
 	const double* T = erida_table(3 * (int64_t)M * M + 1);
 	DevBuf<double> dc(K), F(2 * K - 1);
-	GLI_HIP(hipMemcpyAsync(dc.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, st));
+	upload_async(dc.p, coeffs, K * sizeof(double));
 	k_erida_factors<<<(unsigned)ceil_div(2 * K - 1, 128), 128, 0, st>>>(dc.p, (int)K, tol, F.p);
 	if (num > 0) {
 		const size_t shm = (3 * (size_t)Mp + 256) * sizeof(double);

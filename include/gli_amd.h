@@ -36,7 +36,17 @@ void gli_clear_error(void);
 void* gli_stream(void);
 int gli_synchronize(void);
 
-/* phase timing of the last ERI evaluation (HIP events on the library stream) */
+/* Kernel tuning knobs (set per workload by an autotuner; the defaults are tuned for the 11^3 grid):
+ *   "tile_order"  output-tile order of the mode products: 0 row tiles fastest, 1 XCD-grouped (default)
+ *   "gather_ept"  elements per lane in the per-element gather: 2, 4 (default) or 8
+ * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
+int gli_set_tuning(const char* knob, int value);
+int gli_get_tuning(const char* knob);
+
+/* Phase timing of ERI evaluations with HIP events on the library stream.  While enabled, every
+ * evaluation records its phase boundaries (no synchronisation between evaluations);
+ * gli_phase_times returns, per phase name, the average duration in ms over the evaluations since
+ * timing was enabled or last queried, and starts a new average. */
 int gli_enable_phase_timing(int on);
 int gli_phase_times(double* ms, const char** names, int max);
 
