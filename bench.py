@@ -94,7 +94,7 @@ def phase_times(L):
     return {names[i].decode(): ms[i] for i in range(max(n, 0))}
 
 
-def cpu_baseline(c, n, sample_idx, tol, target_s):
+def cpu_baseline(c, n, all_idx, tol, target_s, seed, label):
     """Reference C build (oracle/_ref) if present, else the oracle restatement, on all host threads."""
     from oracle import oracle as orc
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
@@ -124,19 +124,123 @@ def cpu_baseline(c, n, sample_idx, tol, target_s):
 
         def run(idx):
             return orc.eri_sparse(c, ((-h, n), (-h, n), (-h, n)), idx, tol)
-    # calibrate on a small slice, then size the timed sample to about target_s
-    probe = sample_idx[: max(threads * 2, 32)]
+    # calibrate on a small seeded sample, then time a seeded sample sized to about target_s
+    rng = np.random.default_rng(2024)
+    probe_pos = np.sort(rng.choice(len(all_idx), size=min(len(all_idx), max(threads * 64, 1024)), replace=False))
     t0 = time.perf_counter()
-    run(probe)
-    dt = time.perf_counter() - t0
-    per = dt / len(probe)
-    count = int(min(len(sample_idx), max(len(probe), target_s / max(per, 1e-9))))
-    sub = sample_idx[:count]
+    run(all_idx[probe_pos])
+    per = (time.perf_counter() - t0) / len(probe_pos)
+    count = int(min(len(all_idx), max(len(probe_pos), target_s / max(per, 1e-9))))
+    pos = np.sort(rng.choice(len(all_idx), size=count, replace=False))
+    sub = np.ascontiguousarray(all_idx[pos])
     t0 = time.perf_counter()
     vals = run(sub)
     dt = time.perf_counter() - t0
     return {"value": count / dt, "unit": "elements/s", "cores": threads, "kind": kind,
-            "sample": "%d seeded symmetry-unique 11^3 four-indices (K=57, seed 0, tol 1e-16), %.1f s" % (count, dt)}, sub, vals
+            "sample": "%d seeded uniformly drawn symmetry-unique %s four-indices (K=%d, seed %d, tol %g), %.1f s"
+                      % (count, label, len(c), seed, tol, dt)}, pos, vals
+
+
+# grids of the mixed-size batch: (n, K, coefficient seed, spacing a in bohr); outputs are scaled by 1/a
+BATCH = [(5, 29, 0, 0.5), (7, 43, 0, 0.75), (9, 57, 1, 1.0), (11, 57, 0, 1.5)]
+
+
+def batch_bench(args):
+    """One step = every unique four-index of every grid of the batch, from the coefficients to the
+    scaled values in HBM, as one replay of the plan's captured graph.  The error is taken over every
+    element against the single-grid FP64 path (times 1/a)."""
+    import torch
+    L = _lib.require_gpu()
+    stream = torch.cuda.ExternalStream(L.gli_stream())
+    method = METHODS[args.method]
+    n_g = len(BATCH)
+    coeffs, grids, idxs, nums, vals = [], [], [], [], []
+    enum_ms = 0.0
+    for n, K, seed, a in BATCH:
+        coeffs.append(synthetic_coefficients(K, seed))
+        grids.append(sym_grid6(n))
+        p = C.c_void_p()
+        num = C.c_int64()
+        t0 = time.perf_counter()
+        _lib.check(L.gli_eri_enumerate_device(n, C.byref(p), C.byref(num)), "enumeration")
+        enum_ms += (time.perf_counter() - t0) * 1e3
+        idxs.append(p)
+        nums.append(num.value)
+        vals.append(Dev(L, 8 * num.value))
+    total = int(sum(nums))
+    log("batch: %s, %d unique four-indices" % ([b[:2] for b in BATCH], total))
+    cptr = (_abi.P_double * n_g)(*[c.ctypes.data_as(_abi.P_double) for c in coeffs])
+    Ks = np.array([len(c) for c in coeffs], dtype=np.int64)
+    g6 = np.concatenate(grids)
+    dptr = (C.c_void_p * n_g)(*[p.value for p in idxs])
+    nn = np.array(nums, dtype=np.int64)
+    vptr = (C.c_void_p * n_g)(*[v.ptr.value for v in vals])
+    scales = np.array([1.0 / b[3] for b in BATCH])
+    plan = C.c_void_p()
+    t0 = time.perf_counter()
+    _lib.check(L.gli_eri_batch_create(n_g, cptr, Ks.ctypes.data_as(_abi.P_glong), g6.ctypes.data_as(_abi.P_glong), dptr,
+                                      nn.ctypes.data_as(_abi.P_glong), args.tol, method, vptr, 1, C.byref(plan)),
+               "gli_eri_batch_create")
+    create_ms = (time.perf_counter() - t0) * 1e3
+    buf = C.create_string_buffer(1 << 16)
+    L.gli_eri_batch_describe(plan, buf, len(buf))
+    tuned = json.loads(buf.value.decode())
+    sc = scales.ctypes.data_as(C.POINTER(C.c_double))
+    for _ in range(args.warmup):
+        _lib.check(L.gli_eri_batch_run(plan, cptr, sc), "batch run")
+    L.gli_synchronize()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        _lib.check(L.gli_eri_batch_run(plan, cptr, sc), "batch run")
+    ev1.record(stream)
+    L.gli_synchronize()
+    torch.cuda.synchronize()
+    step_ms = ev0.elapsed_time(ev1) / args.steps
+    log("batch step: %.3f ms" % step_ms)
+    # errors over every element against the single-grid FP64 path
+    errs = []
+    for g, (n, K, seed, a) in enumerate(BATCH):
+        got = np.empty(nums[g])
+        _lib.check(L.gli_memcpy(got.ctypes.data, vals[g].ptr, 8 * nums[g], 1), "copy batch values")
+        ref_d = Dev(L, 8 * nums[g])
+        eri_step(L, coeffs[g], grids[g], idxs[g], nums[g], args.tol, 0, ref_d.ptr)
+        L.gli_synchronize()
+        ref = np.empty(nums[g])
+        _lib.check(L.gli_memcpy(ref.ctypes.data, ref_d.ptr, 8 * nums[g], 1), "copy reference values")
+        ref *= scales[g]
+        d = np.abs(got - ref)
+        big = np.abs(ref) >= 1e-10
+        errs.append({"grid": n, "K": K, "seed": seed, "spacing": a, "elements": nums[g],
+                     "max_abs_err_vs_fp64": float(d.max()),
+                     "max_rel_err_vs_fp64_where_abs_ge_1e-10": float((d[big] / np.abs(ref[big])).max()) if big.any() else None})
+        del got, ref, d, big, ref_d
+    L.gli_eri_batch_destroy(plan)
+    for p in idxs:
+        L.gli_device_free(p)
+    line = {
+        "metric": "unique ERI elements/s, mixed-size batch (5^3 K29, 7^3 K43, 9^3 K57, 11^3 K57), one graph replay per step",
+        "value": total / (step_ms / 1e3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64"}[args.method],
+        "data": "synthetic (seeded gausslet coefficients)",
+        "config": {"workload": "eri_batch_5-7-9-11", "grids": BATCH, "tol": args.tol, "method": args.method,
+                   "unique_elements": total},
+        "plan_create_ms": create_ms,
+        "enumeration_ms": enum_ms,
+        "autotune": tuned,
+        "errors": errs,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -156,11 +260,19 @@ def main():
                     help="coefficients without mirror symmetry (general axis-key set, GEMM-dominated)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip timing the split-precision method beside the FP64 one")
+    ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
+                    help="set a kernel tuning knob (gli_set_tuning) before timing; repeatable")
+    ap.add_argument("--batch", action="store_true",
+                    help="mixed-size batch: 5^3 (K=29), 7^3 (K=43), 9^3 and 11^3 (K=57) in one autotuned plan, "
+                         "replayed as one hipGraph per step")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
                     help="only run an elementwise kernel of known traffic (counter calibration) and exit")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.gpus == 1 and args.batch:
+        batch_bench(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would take a contiguous, "
@@ -170,6 +282,10 @@ def main():
     import torch  # device memory and events are plumbing here; the work runs in libgli_amd.so
     L = _lib.require_gpu()
     stream = torch.cuda.ExternalStream(L.gli_stream())
+    for kv in args.tune:
+        knob, value = kv.split("=", 1)
+        _lib.check(L.gli_set_tuning(knob.encode(), int(value)), "gli_set_tuning " + kv)
+    tuning = {k: L.gli_get_tuning(k.encode()) for k in ("tile_order", "gemm_shape", "gather_ept", "gather_order")}
     method = METHODS[args.method]
     c = synthetic_coefficients(args.K, args.seed, symmetrise=not args.asymmetric)
     g = sym_grid6(args.n)
@@ -293,9 +409,9 @@ def main():
     cpu_err = None
     if not args.no_cpu:
         log("CPU baseline ...")
-        cpu, sub, cpu_vals = cpu_baseline(c, args.n, s_idx, args.tol, args.cpu_seconds)
-        pos = np.searchsorted(s_idx, sub)
-        cpu_err = float(np.max(np.abs(vals[samp][pos] - cpu_vals)))
+        cpu, pos, cpu_vals = cpu_baseline(c, args.n, host_idx, args.tol, args.cpu_seconds, args.seed,
+                                          "%d^3" % args.n)
+        cpu_err = float(np.max(np.abs(vals[pos] - cpu_vals)))
         log("cpu:", cpu, "max abs err vs cpu", cpu_err)
 
     value = num / (step_ms / 1e3)
@@ -314,7 +430,7 @@ def main():
         "data": "synthetic (seeded gausslet coefficients)",
         "config": {"workload": workload,
                    "grid": args.n, "K": args.K, "seed": args.seed, "tol": args.tol, "method": args.method,
-                   "unique_elements": num, "axis_keys": C1},
+                   "unique_elements": num, "axis_keys": C1, "tuning": tuning},
         "roofline": roof,
         "cpu_baseline": cpu,
         "phases_ms": phases,

@@ -39,6 +39,12 @@ EXTRA = {
     "gli_eri_values_batch_device": (C.c_int, [C.c_int, C.POINTER(_abi.P_double), _abi.P_glong, _abi.P_glong,
                                               C.POINTER(C.c_void_p), _abi.P_glong, C.c_double, C.c_int,
                                               C.POINTER(C.c_void_p)]),
+    "gli_eri_batch_create": (C.c_int, [C.c_int, C.POINTER(_abi.P_double), _abi.P_glong, _abi.P_glong,
+                                        C.POINTER(C.c_void_p), _abi.P_glong, C.c_double, C.c_int,
+                                        C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "gli_eri_batch_run": (C.c_int, [C.c_void_p, C.POINTER(_abi.P_double), C.POINTER(C.c_double)]),
+    "gli_eri_batch_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "gli_eri_batch_destroy": (None, [C.c_void_p]),
     "gli_eri_key_count": (C.c_int, [C.c_int64, C.c_int]),
     "gli_eri_sparse_key": (C.c_int64, [C.POINTER(_abi.Grid), _abi.P_glong * 48, C.POINTER(_abi.GridPoint)]),
     "gli_erida_sparse_key": (C.c_int64, [C.POINTER(_abi.Grid), _abi.P_glong * 48, C.POINTER(_abi.GridPoint)]),

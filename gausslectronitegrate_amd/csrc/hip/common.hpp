@@ -117,8 +117,10 @@ struct Context {
 	bool stage_used[kStageSlots] = {};
 	int stage_next = 0;
 	// kernel tuning knobs (gli_set_tuning)
-	int tile_grouped = 1;   // mode-product tile order
+	int tile_grouped = 0;   // mode-product tile order (the XCD-grouped order measured slower on the ERI shapes)
 	int gather_ept = 4;     // gather elements per lane
+	int gather_xcd = 1;     // gather walks one contiguous chunk of the list per XCD
+	int gemm_shape = 0;     // FP64 mode-product tile shape (0: 64x64, 1: 128x64, 2: 64x128, 3: 128x128)
 };
 
 Context& ctx();
@@ -145,13 +147,15 @@ struct TileOrder {
 			ct = local % nc;
 			return rt < nr;
 		}
+		if (b >= nr * nc) { return false; }
 		rt = b % nr;
 		ct = b / nr;
 		return true;
 	}
 };
 
-// the tile order used by the mode products (1 = XCD-grouped, the default; gli_set_tuning changes it)
+// the tile order used by the mode products (0 = row tiles fastest, the default; 1 = XCD-grouped;
+// gli_set_tuning changes it)
 int tile_order_grouped();
 
 // asynchronous host-to-device copy of a small host buffer on the library stream (the source may
@@ -165,14 +169,38 @@ void phase_mark(const char* name);
 // Y[p][k] = sum_{mu < Kd} X[mu * ldx + p] * B[mu * ldb + k] for p < P, k < C (FP64 MFMA);
 // with Cp > 0 and tri_cols / tri_rows, tiles outside k >= r / outer <= r (p = outer*Cp + r) are skipped;
 // with tet_C1 > 0 the output goes to the tetrahedral layout of the sorted triple (outer, r, k), all < tet_C1
+// one FP64 mode product Y[p][k] = sum_mu X[mu][p] B[mu][k] (see eri_tensor.hip for the fields)
+struct ModeArgs {
+	const double* X;
+	int64_t ldx, P;
+	const double* B;
+	int64_t ldb;
+	int C, Kd;
+	double* Y;
+	int64_t ldy;
+	int Cp, tri_cols, tri_rows;
+	int64_t p_base;
+	int tet_C1;
+};
+constexpr int kGemmShapes = 4;
+bool gemm_shape_fits(int shape, int Cp, int tri, int64_t p_base);
+void mode_product_shaped(const ModeArgs& a, int shape, int grouped);
+void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped);
+
 void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
                   double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1);
 
 // split-precision (BF16 digit slices, exact FP32 slice sums, FP64 combination) variant:
-// split_factor() prepares B once; mode_product_split() then contracts X with it
-void split_factor(const double* B, int64_t ldb, int C, int Kd);
+// split_factor() prepares B once; mode_product_split() then contracts X with it.  Both use the
+// library's shared split workspace unless a caller-owned one is given (a captured graph needs
+// buffers that no later call can reallocate); tile_order < 0 takes the context knob.
+struct SplitWorkspace;
+SplitWorkspace* split_workspace_new();
+void split_workspace_delete(SplitWorkspace* w);
+void split_factor(const double* B, int64_t ldb, int C, int Kd, SplitWorkspace* ws = nullptr);
 void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, double* Y, int64_t ldy,
-                        int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, bool cache_x);
+                        int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, bool cache_x,
+                        SplitWorkspace* ws = nullptr, int tile_order = -1);
 void split_invalidate_cache();
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -222,6 +222,14 @@ int gli_set_tuning(const char* knob, int value)
 			GLI_REQUIRE(value == 2 || value == 4 || value == 8, "gather_ept must be 2, 4 or 8");
 			c.gather_ept = value;
 		}
+		else if (k == "gemm_shape") {
+			GLI_REQUIRE(value >= 0 && value < kGemmShapes, "gemm_shape must be 0..3");
+			c.gemm_shape = value;
+		}
+		else if (k == "gather_order") {
+			GLI_REQUIRE(value == 0 || value == 1, "gather_order must be 0 or 1");
+			c.gather_xcd = value;
+		}
 		else {
 			GLI_REQUIRE(false, "unknown tuning knob");
 		}
@@ -239,6 +247,8 @@ int gli_get_tuning(const char* knob)
 		const std::string k = knob ? knob : "";
 		if (k == "tile_order") { return c.tile_grouped; }
 		if (k == "gather_ept") { return c.gather_ept; }
+		if (k == "gather_order") { return c.gather_xcd; }
+		if (k == "gemm_shape") { return c.gemm_shape; }
 		GLI_REQUIRE(false, "unknown tuning knob");
 	}
 	catch (const Failure&) {

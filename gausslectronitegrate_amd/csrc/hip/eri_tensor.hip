@@ -20,11 +20,17 @@
 // re-association of the reference sum: same terms, different order.
 #include "common.hpp"
 #include "../gli_device.h"
+#include "gli_amd.h"
 
 #include <algorithm>
 #include <cmath>
 #include <map>
 #include <tuple>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include <string>
+#include <memory>
 
 namespace gli {
 
@@ -106,13 +112,20 @@ __global__ void k_kernel_tensor(const double* __restrict__ T, int M, int Mp, dou
 // ---------------------------------------------------------------------------
 // mode product  Y[p][k] = sum_{mu < Kd} X[mu][p] B[mu][k]   (FP64 MFMA 16x16x4)
 //
-// 64x64 output tile per workgroup of 4 waves (each 32x32 = 2x2 MFMA blocks),
-// mu staged through LDS 16 at a time.  Optional triangular skipping for the
-// symmetric tensor: with p = outer * Cp + r,
+// BM x BN output tile per workgroup of 4 waves in a 2 x 2 layout, each wave (BM/2) x (BN/2)
+// = (BM/32) x (BN/32) MFMA blocks; mu staged through LDS GBK at a time with the next slice
+// prefetched into registers.  Optional triangular skipping for the symmetric tensor: with
+// p = outer * Cp + r,
 //   tri_cols: only k >= r is needed;  tri_rows: only outer <= r is needed.
-// Tiles entirely outside the needed region are not computed (never read later).
+// Tiles entirely outside the needed region are not computed (never read later).  The skip test
+// needs (p_base + p0) / Cp fixed per tile: Cp and p_base multiples of BM.
 
-constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GBK = 16;
+constexpr int KEY_ALIGN = 64;   // key columns are padded to a multiple of this
+
+// tile shapes (kGemmShapes of them): 0: 64x64, 1: 128x64, 2: 64x128, 3: 128x128
+__host__ __device__ constexpr int shape_bm(int s) { return (s == 1 || s == 3) ? 128 : 64; }
+__host__ __device__ constexpr int shape_bn(int s) { return (s == 2 || s == 3) ? 128 : 64; }
 
 // position of the sorted key triple k1 <= k2 <= k3 in the tetrahedral layout of a symmetric 3-tensor
 __host__ __device__ inline int64_t tet_index(int k1, int k2, int k3)
@@ -120,66 +133,75 @@ __host__ __device__ inline int64_t tet_index(int k1, int k2, int k3)
 	return (int64_t)k3 * (k3 + 1) * (k3 + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1;
 }
 
-__global__ __launch_bounds__(256) void k_mode_product(
-	const double* __restrict__ X, int64_t ldx, int64_t P, const double* __restrict__ B, int64_t ldb, int C, int Kd,
-	double* __restrict__ Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, TileOrder order)
+template <int BM, int BN>
+__device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, double (*As)[BM], double (*Bs)[BN])
 {
-	__shared__ double As[GBK][GBM];
-	__shared__ double Bs[GBK][GBN];
+	constexpr int TM = BM / 32, TN = BN / 32;
 	const int tid = threadIdx.x;
 	const int lane = tid & 63, wave = tid >> 6;
-	int rt, ct;
-	if (!order.tile(blockIdx.x, rt, ct)) { return; }
-	const int64_t p0 = (int64_t)rt * GBM;
-	const int k0 = ct * GBN;
+	const int64_t p0 = (int64_t)rt * BM;
+	const int k0 = ct * BN;
 
-	if (tri_cols || tri_rows) {
-		// global row index (with p_base) decomposes as outer * Cp + r; Cp % GBM == 0 keeps outer fixed per tile
-		const int64_t gp = p_base + p0;
-		const int64_t outer = gp / Cp;
-		const int64_t rlo = gp % Cp, rhi = rlo + GBM - 1;
-		if (tri_cols && k0 + GBN - 1 < rlo) { return; }
-		if (tri_rows && rhi < outer) { return; }
+	if (a.tri_cols || a.tri_rows) {
+		const int64_t gp = a.p_base + p0;
+		const int64_t outer = gp / a.Cp;
+		const int64_t rlo = gp % a.Cp, rhi = rlo + BM - 1;
+		if (a.tri_cols && k0 + BN - 1 < rlo) { return; }
+		if (a.tri_rows && rhi < outer) { return; }
 	}
 
 	const int wm = wave >> 1, wn = wave & 1;
-	d4 acc[2][2];
-	for (int i = 0; i < 2; ++i) {
-		for (int j = 0; j < 2; ++j) { acc[i][j] = d4{0, 0, 0, 0}; }
+	d4 acc[TM][TN];
+#pragma unroll
+	for (int i = 0; i < TM; ++i) {
+#pragma unroll
+		for (int j = 0; j < TN; ++j) { acc[i][j] = d4{0, 0, 0, 0}; }
 	}
 
-	// software pipeline: the next mu-tile is loaded into registers while the current one
+	// software pipeline: the next mu-slice is loaded into registers while the current one
 	// (already in LDS) feeds the MFMAs
-	constexpr int PER = (GBK * GBM) / 256;
-	double ra[PER], rb[PER];
+	constexpr int PA = (GBK * BM) / 256, PB = (GBK * BN) / 256;
+	double ra[PA], rb[PB];
 	auto gload = [&](int mu0) {
 #pragma unroll
-		for (int r = 0; r < PER; ++r) {
+		for (int r = 0; r < PA; ++r) {
 			const int id = tid + 256 * r;
-			const int mu = id / GBM, p = id % GBM;
-			ra[r] = (mu0 + mu < Kd && p0 + p < P) ? X[(int64_t)(mu0 + mu) * ldx + p0 + p] : 0.0;
-			const int k = id % GBN;
-			rb[r] = (mu0 + mu < Kd && k0 + k < C) ? B[(int64_t)(mu0 + mu) * ldb + k0 + k] : 0.0;
+			const int mu = id / BM, p = id % BM;
+			ra[r] = (mu0 + mu < a.Kd && p0 + p < a.P) ? a.X[(int64_t)(mu0 + mu) * a.ldx + p0 + p] : 0.0;
+		}
+#pragma unroll
+		for (int r = 0; r < PB; ++r) {
+			const int id = tid + 256 * r;
+			const int mu = id / BN, k = id % BN;
+			rb[r] = (mu0 + mu < a.Kd && k0 + k < a.C) ? a.B[(int64_t)(mu0 + mu) * a.ldb + k0 + k] : 0.0;
 		}
 	};
 	gload(0);
-	for (int mu0 = 0; mu0 < Kd; mu0 += GBK) {
+	for (int mu0 = 0; mu0 < a.Kd; mu0 += GBK) {
 #pragma unroll
-		for (int r = 0; r < PER; ++r) {
+		for (int r = 0; r < PA; ++r) {
 			const int id = tid + 256 * r;
-			As[id / GBM][id % GBM] = ra[r];
-			Bs[id / GBN][id % GBN] = rb[r];
+			As[id / BM][id % BM] = ra[r];
+		}
+#pragma unroll
+		for (int r = 0; r < PB; ++r) {
+			const int id = tid + 256 * r;
+			Bs[id / BN][id % BN] = rb[r];
 		}
 		__syncthreads();
-		if (mu0 + GBK < Kd) { gload(mu0 + GBK); }
+		if (mu0 + GBK < a.Kd) { gload(mu0 + GBK); }
 #pragma unroll
 		for (int kk = 0; kk < GBK; kk += 4) {
-			double a[2], b[2];
-			for (int i = 0; i < 2; ++i) { a[i] = As[kk + lane / 16][wm * 32 + i * 16 + lane % 16]; }
-			for (int j = 0; j < 2; ++j) { b[j] = Bs[kk + lane / 16][wn * 32 + j * 16 + lane % 16]; }
-			for (int i = 0; i < 2; ++i) {
-				for (int j = 0; j < 2; ++j) {
-					acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+			double av[TM], bv[TN];
+#pragma unroll
+			for (int i = 0; i < TM; ++i) { av[i] = As[kk + lane / 16][wm * (BM / 2) + i * 16 + lane % 16]; }
+#pragma unroll
+			for (int j = 0; j < TN; ++j) { bv[j] = Bs[kk + lane / 16][wn * (BN / 2) + j * 16 + lane % 16]; }
+#pragma unroll
+			for (int i = 0; i < TM; ++i) {
+#pragma unroll
+				for (int j = 0; j < TN; ++j) {
+					acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
 				}
 			}
 		}
@@ -188,23 +210,58 @@ __global__ __launch_bounds__(256) void k_mode_product(
 
 	// accumulator layout of the 16x16 FP64 block (v_mfma_f64_16x16x4f64): lane holds
 	// column lane%16 and rows lane/16 + 4 v, v = 0..3
-	for (int i = 0; i < 2; ++i) {
-		for (int j = 0; j < 2; ++j) {
-			const int col = k0 + wn * 32 + j * 16 + lane % 16;
+#pragma unroll
+	for (int i = 0; i < TM; ++i) {
+#pragma unroll
+		for (int j = 0; j < TN; ++j) {
+			const int col = k0 + wn * (BN / 2) + j * 16 + lane % 16;
+#pragma unroll
 			for (int v = 0; v < 4; ++v) {
-				const int64_t row = p0 + wm * 32 + i * 16 + (lane / 16) + 4 * v;
-				if (row >= P || col >= C) { continue; }
-				if (tet_C1 > 0) {
-					const int64_t gp = p_base + row;
-					const int k1 = (int)(gp / Cp), k2 = (int)(gp % Cp);
-					if (k1 <= k2 && k2 <= col && col < tet_C1) { Y[tet_index(k1, k2, col)] = acc[i][j][v]; }
+				const int64_t row = p0 + wm * (BM / 2) + i * 16 + (lane / 16) + 4 * v;
+				if (row >= a.P || col >= a.C) { continue; }
+				if (a.tet_C1 > 0) {
+					const int64_t gp = a.p_base + row;
+					const int k1 = (int)(gp / a.Cp), k2 = (int)(gp % a.Cp);
+					if (k1 <= k2 && k2 <= col && col < a.tet_C1) { a.Y[tet_index(k1, k2, col)] = acc[i][j][v]; }
 				}
 				else {
-					Y[row * ldy + col] = acc[i][j][v];
+					a.Y[row * a.ldy + col] = acc[i][j][v];
 				}
 			}
 		}
 	}
+}
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_mode_product(ModeArgs a, TileOrder order)
+{
+	__shared__ double As[GBK][BM];
+	__shared__ double Bs[GBK][BN];
+	int rt, ct;
+	if (!order.tile(blockIdx.x, rt, ct)) { return; }
+	mode_product_tile<BM, BN>(a, rt, ct, As, Bs);
+}
+
+// several independent mode products (one per grid of a batch) in one launch; each problem's
+// workgroups start at a multiple of 8 so that the XCD-grouped order keeps its meaning
+constexpr int kMaxGroup = 16;
+struct ModeGroup {
+	ModeArgs a[kMaxGroup];
+	TileOrder order[kMaxGroup];
+	int block_begin[kMaxGroup + 1];
+	int n;
+};
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void k_mode_product_grouped(ModeGroup grp)
+{
+	__shared__ double As[GBK][BM];
+	__shared__ double Bs[GBK][BN];
+	int g = 0;
+	while (g + 1 < grp.n && (int)blockIdx.x >= grp.block_begin[g + 1]) { ++g; }
+	int rt, ct;
+	if (!grp.order[g].tile((int)blockIdx.x - grp.block_begin[g], rt, ct)) { return; }
+	mode_product_tile<BM, BN>(grp.a[g], rt, ct, As, Bs);
 }
 
 // ---------------------------------------------------------------------------
@@ -367,17 +424,35 @@ __device__ inline void point_xyz(unsigned p, const GridDev& g, const DecodeConst
 	z = rem - __umul24(y, (unsigned)g.n[2]);
 }
 
+// One gather problem: the unique four-indices of one grid and where their values go.  The
+// optional scale (device pointer, one double) multiplies every value: the 1/a of a physical
+// grid spacing a, applied on the device instead of by the caller.
+struct GatherDesc {
+	const int64_t* idx;
+	int64_t num;
+	GridDev g;
+	DecodeConsts dc;
+	const short* lut;
+	const uint32_t* tet3;
+	const double* V;
+	double* out;
+	const double* scale;
+	int64_t piece_begin;   // first workgroup-sized piece of this problem in a grouped launch
+};
+
 template <int GATHER_EPT>
-__global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx, int64_t num, GridDev g, DecodeConsts dc,
-	const short* __restrict__ lut, const uint32_t* __restrict__ tet3, const double* __restrict__ V, double* __restrict__ out)
+__device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base)
 {
 	const int lane = threadIdx.x & 63;
-	const int64_t wave_base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * GATHER_EPT);
+	const GridDev& g = d.g;
+	const DecodeConsts& dc = d.dc;
+	const short* __restrict__ lut = d.lut;
+	const double* __restrict__ V = d.V;
 	int64_t t[GATHER_EPT];
 #pragma unroll
 	for (int u = 0; u < GATHER_EPT; ++u) {
 		const int64_t e = wave_base + u * 64 + lane;
-		t[u] = (e < num) ? idx[e] : -1;
+		t[u] = (e < d.num) ? d.idx[e] : -1;
 	}
 	// the wave's pair (i, j) from its first element
 	const int W = 4 * g.R - 3, off = 2 * (g.R - 1);
@@ -393,8 +468,8 @@ __global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx,
 		point_xyz(j, g, dc, cj[0], cj[1], cj[2]);
 #pragma unroll
 		for (int a = 0; a < 3; ++a) {
-			const int d = abs((int)ci[a] - (int)cj[a]);
-			base_key[a] = __builtin_amdgcn_readfirstlane(d * g.R * W + (int)(ci[a] + cj[a]) + off);
+			const int dd = abs((int)ci[a] - (int)cj[a]);
+			base_key[a] = __builtin_amdgcn_readfirstlane(dd * g.R * W + (int)(ci[a] + cj[a]) + off);
 		}
 		pair_base = uniform_i64(ij * g.N2);
 	}
@@ -433,7 +508,7 @@ __global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx,
 			vi[u] = (__umul24(k2, k2 + 1u) >> 1) + k1;
 		}
 #pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] += tet3[k3[u]]; }
+		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] += d.tet3[k3[u]]; }
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = V[vi[u]]; }
 	}
@@ -444,11 +519,47 @@ __global__ __launch_bounds__(256) void k_gather(const int64_t* __restrict__ idx,
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
 	}
+	if (d.scale != nullptr) {
+		const double sc = *d.scale;
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) { v[u] *= sc; }
+	}
 #pragma unroll
 	for (int u = 0; u < GATHER_EPT; ++u) {
 		const int64_t e = wave_base + u * 64 + lane;
-		if (e < num) { out[e] = v[u]; }
+		if (e < d.num) { d.out[e] = v[u]; }
 	}
+}
+
+// Workgroup b runs on XCD b % 8.  With xcd_chunks > 0 each XCD walks its own contiguous chunk of
+// xcd_chunks workgroup-sized pieces (of the concatenated lists in a grouped launch), so the V
+// entries its resident waves touch (neighbouring elements share the pair (i, j) and with it most
+// of their axis keys) stay in that XCD's L2 instead of being spread over all eight.
+__device__ inline int64_t gather_piece(int64_t xcd_chunks)
+{
+	return (xcd_chunks > 0) ? (int64_t)(blockIdx.x & 7) * xcd_chunks + (blockIdx.x >> 3) : (int64_t)blockIdx.x;
+}
+
+template <int GATHER_EPT>
+__global__ __launch_bounds__(256) void k_gather(GatherDesc d, int64_t xcd_chunks)
+{
+	const int64_t piece = gather_piece(xcd_chunks);
+	const int64_t wave_base = (piece * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * GATHER_EPT);
+	if (wave_base >= d.num) { return; }
+	gather_wave<GATHER_EPT>(d, wave_base);
+}
+
+// all grids of a batch in one launch: descriptors in device memory, pieces concatenated
+template <int GATHER_EPT>
+__global__ __launch_bounds__(256) void k_gather_grouped(const GatherDesc* __restrict__ descs, int n, int64_t xcd_chunks)
+{
+	const int64_t piece = gather_piece(xcd_chunks);
+	int gi = 0;
+	while (gi + 1 < n && piece >= descs[gi + 1].piece_begin) { ++gi; }
+	const GatherDesc d = descs[gi];
+	const int64_t wave_base = ((piece - d.piece_begin) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * GATHER_EPT);
+	if (piece < d.piece_begin || wave_base >= d.num) { return; }
+	gather_wave<GATHER_EPT>(d, wave_base);
 }
 
 // per-element FP64 sum over the folded columns, no reuse across elements:
@@ -607,26 +718,26 @@ struct EriSetup {
 	const double* T;
 };
 
-// tables f, P, B (and T3) for a coefficient set on a grid; all on the library stream
-static EriSetup eri_tables(const double* coeffs, int64_t K, const int64_t grid[6], double tol)
+// shapes, key set and Coulomb table for K coefficients on a grid (no device work but the table)
+static EriSetup eri_shape(int64_t K, const int64_t grid[6], bool sym)
 {
 	GLI_REQUIRE(K >= 3 && (K % 2) == 1, "the number of gausslet coefficients must be odd and at least 3");
 	GLI_REQUIRE(grid[1] > 0 && grid[3] > 0 && grid[5] > 0, "grid axes must have a positive number of points");
 	GLI_REQUIRE(grid[1] < 1024 && grid[3] < 1024 && grid[5] < 1024, "at most 1023 points per axis");
-	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
-	Context& c = ctx();
-	EriWorkspace& w = eri_ws();
+	{
+		const double N = (double)grid[1] * grid[3] * grid[5];
+		GLI_REQUIRE(N * N * N * N < 9007199254740992.0, "grid too large: four-indices must stay below 2^53");
+	}
 	EriSetup s;
 	s.K = (int)K;
 	s.R = (int)std::max(grid[1], std::max(grid[3], grid[5]));
 	s.M = 6 * (s.R - 1) + 2 * (s.K - 1);
 	s.Mp = (int)round_up(s.M + 1, 16);
 	GLI_REQUIRE(s.Mp <= 2048, "grid too long: the folded kernel range 6(n-1) + 2(K-1) must stay below 2048");
-	const bool sym = mirror_symmetric(coeffs, K);
 	const KeySet& ks = key_set(s.R, sym);
 	s.ks = &ks;
 	s.C1 = ks.C1;
-	s.Cp = (int)round_up(s.C1, GBM);
+	s.Cp = (int)round_up(s.C1, KEY_ALIGN);
 	s.g.n[0] = (int)grid[1];
 	s.g.n[1] = (int)grid[3];
 	s.g.n[2] = (int)grid[5];
@@ -634,32 +745,161 @@ static EriSetup eri_tables(const double* coeffs, int64_t K, const int64_t grid[6
 	s.g.R = s.R;
 	s.g.N2 = (int64_t)s.g.N * s.g.N;
 	s.g.N4 = s.g.N2 * s.g.N2;
-	const int64_t nT = 3 * (int64_t)s.M * s.M + 1;
-	s.T = eri_table(nT);
-
-	const int L = 2 * s.K - 1, LP = 4 * s.K - 3;
-	w.coeffs.reserve(K);
-	w.f.reserve((size_t)s.R * L);
-	w.P.reserve((size_t)s.R * s.R * LP);
-	w.B.reserve((size_t)s.Mp * s.Cp);
-	w.coords.reserve((size_t)s.g.N);
-	upload_async(w.coeffs.p, coeffs, K * sizeof(double));
-	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, 0, c.stream>>>(w.coeffs.p, s.K, w.f.p);
-	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, 0, c.stream>>>(w.f.p, s.K, s.R, tol, w.P.p);
-	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, c.stream>>>(w.P.p, ks.keys_dev.p, s.C1, s.K, s.R, s.M, s.Cp, w.B.p);
-	k_point_coords<<<(unsigned)ceil_div(s.g.N, 256), 256, 0, c.stream>>>(s.g.n[0], s.g.n[1], s.g.n[2], w.coords.p);
-	GLI_HIP(hipGetLastError());
+	s.T = eri_table(3 * (int64_t)s.M * s.M + 1);
 	return s;
+}
+
+// f, P (zeroed where |P| <= tol) and the folded key columns B from device-resident coefficients
+static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol, double* f, double* P, double* B)
+{
+	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
+	hipStream_t st = ctx().stream;
+	const int L = 2 * s.K - 1, LP = 4 * s.K - 3;
+	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, 0, st>>>(d_coeffs, s.K, f);
+	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, 0, st>>>(f, s.K, s.R, tol, P);
+	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.ks->keys_dev.p, s.C1, s.K, s.R, s.M, s.Cp, B);
+	GLI_HIP(hipGetLastError());
+}
+
+static void launch_kernel_tensor(const EriSetup& s, double* T3)
+{
+	k_kernel_tensor<<<dim3((unsigned)ceil_div(s.Mp, 64), (unsigned)s.Mp, (unsigned)s.Mp), 64, 0, ctx().stream>>>(s.T, s.M, s.Mp, T3);
+	GLI_HIP(hipGetLastError());
+}
+
+static DecodeConsts decode_consts(const EriSetup& s)
+{
+	DecodeConsts dc;
+	dc.inv_N2 = 1.0 / (double)s.g.N2;
+	dc.inv_N = 1.0f / (float)s.g.N;
+	dc.inv_nz = 1.0f / (float)s.g.n[2];
+	dc.inv_nyz = 1.0f / (float)(s.g.n[1] * s.g.n[2]);
+	dc.inv_N2_up = std::nextafter(1.0 / (double)s.g.N2, HUGE_VAL);
+	dc.inv_N_up = std::nextafter(1.0 / (double)s.g.N, HUGE_VAL);
+	dc.inv_nyz_up = std::nextafterf(1.0f / (float)(s.g.n[1] * s.g.n[2]), HUGE_VALF);
+	dc.inv_nz_up = std::nextafterf(1.0f / (float)s.g.n[2], HUGE_VALF);
+	dc.uniform_pairs = (s.g.N <= 4096 && s.ks->tet32) ? 1 : 0;
+	return dc;
+}
+
+static GatherDesc gather_desc(const EriSetup& s, const int64_t* d_idx, int64_t num, const double* V, double* out,
+                              const double* scale)
+{
+	GatherDesc d;
+	d.idx = d_idx;
+	d.num = num;
+	d.g = s.g;
+	d.dc = decode_consts(s);
+	d.lut = s.ks->lut16_dev.p;
+	d.tet3 = s.ks->tet3_dev.p;
+	d.V = V;
+	d.out = out;
+	d.scale = scale;
+	d.piece_begin = 0;
+	return d;
+}
+
+static int64_t gather_pieces(int64_t num, int ept)
+{
+	return ceil_div(num, 256 * (int64_t)ept);
+}
+
+static void launch_gather(const GatherDesc& d, int ept, int xcd)
+{
+	const int64_t pieces = gather_pieces(d.num, ept);
+	if (pieces == 0) { return; }
+	const int64_t chunks = xcd ? ceil_div(pieces, 8) : 0;
+	const int64_t blocks = xcd ? 8 * chunks : pieces;
+	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
+	hipStream_t st = ctx().stream;
+	if (ept == 2) { k_gather<2><<<(unsigned)blocks, 256, 0, st>>>(d, chunks); }
+	else if (ept == 8) { k_gather<8><<<(unsigned)blocks, 256, 0, st>>>(d, chunks); }
+	else { k_gather<4><<<(unsigned)blocks, 256, 0, st>>>(d, chunks); }
+	GLI_HIP(hipGetLastError());
+}
+
+static void launch_gather_grouped(const GatherDesc* d_descs, int n, int64_t total_pieces, int ept, int xcd)
+{
+	if (total_pieces == 0) { return; }
+	const int64_t chunks = xcd ? ceil_div(total_pieces, 8) : 0;
+	const int64_t blocks = xcd ? 8 * chunks : total_pieces;
+	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
+	hipStream_t st = ctx().stream;
+	if (ept == 2) { k_gather_grouped<2><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); }
+	else if (ept == 8) { k_gather_grouped<8><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); }
+	else { k_gather_grouped<4><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); }
+	GLI_HIP(hipGetLastError());
+}
+
+bool gemm_shape_fits(int shape, int Cp, int tri, int64_t p_base)
+{
+	if (shape < 0 || shape >= kGemmShapes) { return false; }
+	return !tri || (Cp % shape_bm(shape) == 0 && p_base % shape_bm(shape) == 0);
+}
+
+static TileOrder tile_order_for(int shape, int64_t P, int C, int grouped)
+{
+	return TileOrder{(int)ceil_div(P, shape_bm(shape)), (int)ceil_div(C, shape_bn(shape)), grouped};
+}
+
+template <int BM, int BN>
+static void launch_mode(const ModeArgs& a, const TileOrder& o, hipStream_t st)
+{
+	k_mode_product<BM, BN><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o);
+}
+
+void mode_product_shaped(const ModeArgs& a, int shape, int grouped)
+{
+	if (a.P <= 0 || a.C <= 0) { return; }
+	GLI_REQUIRE(gemm_shape_fits(shape, a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0, a.p_base),
+	            "mode-product tile shape does not fit the key padding");
+	GLI_REQUIRE(ceil_div(a.P, 64) * ceil_div(a.C, 64) < 2147483647 - 8 * ceil_div(a.C, 64), "mode product too large");
+	const TileOrder o = tile_order_for(shape, a.P, a.C, grouped);
+	hipStream_t st = ctx().stream;
+	switch (shape) {
+		case 1: launch_mode<128, 64>(a, o, st); break;
+		case 2: launch_mode<64, 128>(a, o, st); break;
+		case 3: launch_mode<128, 128>(a, o, st); break;
+		default: launch_mode<64, 64>(a, o, st); break;
+	}
+	GLI_HIP(hipGetLastError());
 }
 
 void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
                   double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1)
 {
-	if (P <= 0 || C <= 0) { return; }
-	GLI_REQUIRE(ceil_div(P, GBM) * ceil_div(C, GBN) < 2147483647 - 8 * ceil_div(C, GBN), "mode product too large");
-	TileOrder order{(int)ceil_div(P, GBM), (int)ceil_div(C, GBN), tile_order_grouped()};
-	k_mode_product<<<(unsigned)order.blocks(), 256, 0, ctx().stream>>>(X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows,
-	                                                                   p_base, tet_C1, order);
+	const ModeArgs a{X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows, p_base, tet_C1};
+	int shape = ctx().gemm_shape;
+	if (!gemm_shape_fits(shape, Cp, tri_cols || tri_rows || tet_C1 > 0, p_base)) { shape = 0; }
+	mode_product_shaped(a, shape, tile_order_grouped());
+}
+
+void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped)
+{
+	GLI_REQUIRE(n >= 1 && n <= kMaxGroup, "too many problems in one grouped mode product");
+	ModeGroup grp;
+	grp.n = n;
+	int blocks = 0;
+	for (int g = 0; g < n; ++g) {
+		GLI_REQUIRE(gemm_shape_fits(shape, a[g].Cp, a[g].tri_cols || a[g].tri_rows || a[g].tet_C1 > 0, a[g].p_base),
+		            "mode-product tile shape does not fit the key padding");
+		grp.a[g] = a[g];
+		grp.order[g] = tile_order_for(shape, std::max<int64_t>(a[g].P, 1), std::max(a[g].C, 1), grouped);
+		grp.block_begin[g] = blocks;
+		const int64_t nb = (a[g].P > 0 && a[g].C > 0) ? round_up(grp.order[g].blocks(), 8) : 0;
+		GLI_REQUIRE(blocks + nb < 2147483647, "grouped mode product too large");
+		if (nb == 0) { grp.order[g].nr = 0; }
+		blocks += (int)nb;
+	}
+	grp.block_begin[n] = blocks;
+	if (blocks == 0) { return; }
+	hipStream_t st = ctx().stream;
+	switch (shape) {
+		case 1: k_mode_product_grouped<128, 64><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		case 2: k_mode_product_grouped<64, 128><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		case 3: k_mode_product_grouped<128, 128><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		default: k_mode_product_grouped<64, 64><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+	}
 	GLI_HIP(hipGetLastError());
 }
 
@@ -672,18 +912,23 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	GLI_REQUIRE(num >= 0, "negative number of indices");
 	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64 || method == GLI_ERI_DIRECT_FP64 || method == GLI_ERI_TENSOR_MIXED,
 	            "unknown ERI method");
-	{
-		const double N = (double)grid[1] * grid[3] * grid[5];
-		GLI_REQUIRE(N * N * N * N < 9007199254740992.0, "grid too large: four-indices must stay below 2^53");
-	}
+	GLI_REQUIRE(coeffs != nullptr && K >= 3, "the number of gausslet coefficients must be odd and at least 3");
 	phase_begin();
-	EriSetup s = eri_tables(coeffs, K, grid, tol);
+	const EriSetup s = eri_shape(K, grid, mirror_symmetric(coeffs, K));
 	EriWorkspace& w = eri_ws();
+	w.coeffs.reserve(K);
+	w.f.reserve((size_t)s.R * (2 * s.K - 1));
+	w.P.reserve((size_t)s.R * s.R * (4 * s.K - 3));
+	w.B.reserve((size_t)s.Mp * s.Cp);
+	upload_async(w.coeffs.p, coeffs, K * sizeof(double));
+	launch_tables(s, w.coeffs.p, tol, w.f.p, w.P.p, w.B.p);
 	phase_mark("tables");
 	if (num == 0) { return; }
 	const int64_t Mp = s.Mp, Cp = s.Cp;
 
 	if (method == GLI_ERI_DIRECT_FP64) {
+		w.coords.reserve((size_t)s.g.N);
+		k_point_coords<<<(unsigned)ceil_div(s.g.N, 256), 256, 0, c.stream>>>(s.g.n[0], s.g.n[1], s.g.n[2], w.coords.p);
 		const int64_t blocks = std::min<int64_t>(num, 65536);
 		const size_t shm = (3 * (size_t)s.Mp + 256) * sizeof(double);
 		k_eri_direct<<<(unsigned)blocks, 256, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.coords.p, w.B.p, s.Cp, s.M, s.Mp, s.T, d_vals);
@@ -697,8 +942,7 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	// kernel tensor, cached per folded range
 	if (w.T3_M != s.M) {
 		w.T3.reserve((size_t)Mp * Mp * Mp);
-		k_kernel_tensor<<<dim3((unsigned)ceil_div(Mp, 64), (unsigned)Mp, (unsigned)Mp), 64, 0, c.stream>>>(s.T, s.M, s.Mp, w.T3.p);
-		GLI_HIP(hipGetLastError());
+		launch_kernel_tensor(s, w.T3.p);
 		w.T3_M = s.M;
 		split_invalidate_cache();
 	}
@@ -723,25 +967,373 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	if (mixed) { mode_product_split(w.Y2.p, Cp * Cp, Cp * Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1, false); }
 	else { mode_product(w.Y2.p, Cp * Cp, Cp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1); }
 	phase_mark("mode3");
-	{
-		DecodeConsts dc;
-		dc.inv_N2 = 1.0 / (double)s.g.N2;
-		dc.inv_N = 1.0f / (float)s.g.N;
-		dc.inv_nz = 1.0f / (float)s.g.n[2];
-		dc.inv_nyz = 1.0f / (float)(s.g.n[1] * s.g.n[2]);
-		dc.inv_N2_up = std::nextafter(1.0 / (double)s.g.N2, HUGE_VAL);
-		dc.inv_N_up = std::nextafter(1.0 / (double)s.g.N, HUGE_VAL);
-		dc.inv_nyz_up = std::nextafterf(1.0f / (float)(s.g.n[1] * s.g.n[2]), HUGE_VALF);
-		dc.inv_nz_up = std::nextafterf(1.0f / (float)s.g.n[2], HUGE_VALF);
-		dc.uniform_pairs = (s.g.N <= 4096 && s.ks->tet32) ? 1 : 0;
-		const int ept = c.gather_ept;
-		const unsigned blocks = (unsigned)ceil_div(num, 256 * ept);
-		if (ept == 2) { k_gather<2><<<blocks, 256, 0, c.stream>>>(d_idx, num, s.g, dc, s.ks->lut16_dev.p, s.ks->tet3_dev.p, w.V.p, d_vals); }
-		else if (ept == 8) { k_gather<8><<<blocks, 256, 0, c.stream>>>(d_idx, num, s.g, dc, s.ks->lut16_dev.p, s.ks->tet3_dev.p, w.V.p, d_vals); }
-		else { k_gather<4><<<blocks, 256, 0, c.stream>>>(d_idx, num, s.g, dc, s.ks->lut16_dev.p, s.ks->tet3_dev.p, w.V.p, d_vals); }
-	}
-	GLI_HIP(hipGetLastError());
+	launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept, c.gather_xcd);
 	phase_mark("gather");
+}
+
+// ---------------------------------------------------------------------------
+// batched evaluation of several grids: a plan owns every buffer, picks its kernel variants by
+// timing them on its own shapes, and replays one captured hipGraph per evaluation
+//
+// graph: parameter upload (coefficients and output scales, pinned -> device), per-grid factor /
+// product / fold tables, one grouped launch per mode product over all grids (or per-grid
+// split-precision products when those are faster), one grouped gather over the concatenated
+// index lists of all grids.
+
+struct BatchGrid {
+	EriSetup s;
+	int64_t K = 0;
+	bool sym = false;
+	const int64_t* d_idx = nullptr;
+	int64_t num = 0;
+	double* d_vals = nullptr;
+	DevBuf<double> f, P, B, Y1, Y2, V;
+	const double* T3 = nullptr;
+	size_t coeff_off = 0, scale_off = 0;
+};
+
+struct BatchChoice {
+	int gemm_shape[3] = { 0, 0, 0 };
+	int gemm_order[3] = { 1, 1, 1 };
+	bool split[3] = { false, false, false };
+	int gather_ept = 4;
+	int gather_order = 1;
+};
+
+struct BatchPlan {
+	int method = GLI_ERI_TENSOR_FP64;
+	double tol = 0;
+	std::vector<std::unique_ptr<BatchGrid>> grids;
+	std::vector<std::pair<int, std::unique_ptr<DevBuf<double>>>> t3;   // kernel tensor per folded range M
+	DevBuf<double> d_params;
+	double* h_params = nullptr;
+	size_t n_params = 0;
+	DevBuf<GatherDesc> d_gdesc;
+	int gdesc_ept = -1;
+	int64_t total_pieces = 0;
+	SplitWorkspace* split_ws = nullptr;
+	BatchChoice choice;
+	std::string report;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	hipEvent_t done = nullptr;
+	bool launched = false;
+
+	~BatchPlan()
+	{
+		if (launched && done) { (void)hipEventSynchronize(done); }
+		if (exec) { (void)hipGraphExecDestroy(exec); }
+		if (graph) { (void)hipGraphDestroy(graph); }
+		if (done) { (void)hipEventDestroy(done); }
+		if (h_params) { (void)hipHostFree(h_params); }
+		if (split_ws) { split_workspace_delete(split_ws); }
+	}
+};
+
+static ModeArgs batch_mode_args(const BatchGrid& bg, int mode)
+{
+	const EriSetup& s = bg.s;
+	const int64_t Mp = s.Mp, Cp = s.Cp;
+	if (mode == 0) { return ModeArgs{bg.T3, Mp * Mp, Mp * Mp, bg.B.p, Cp, s.Cp, s.Mp, bg.Y1.p, Cp, s.Cp, 0, 0, 0, 0}; }
+	if (mode == 1) { return ModeArgs{bg.Y1.p, Mp * Cp, Mp * Cp, bg.B.p, Cp, s.Cp, s.Mp, bg.Y2.p, Cp, s.Cp, 1, 0, 0, 0}; }
+	return ModeArgs{bg.Y2.p, Cp * Cp, Cp * Cp, bg.B.p, Cp, s.Cp, s.Mp, bg.V.p, 0, s.Cp, 1, 1, 0, s.C1};
+}
+
+// gather descriptors for one elements-per-lane choice (piece offsets depend on it)
+static void batch_gather_descs(BatchPlan& plan, int ept)
+{
+	if (plan.gdesc_ept == ept) { return; }
+	std::vector<GatherDesc> h;
+	int64_t pieces = 0;
+	for (auto& g : plan.grids) {
+		GatherDesc d = gather_desc(g->s, g->d_idx, g->num, g->V.p, g->d_vals, plan.d_params.p + g->scale_off);
+		d.piece_begin = pieces;
+		pieces += gather_pieces(g->num, ept);
+		h.push_back(d);
+	}
+	plan.d_gdesc.reserve(h.size());
+	GLI_HIP(hipMemcpyAsync(plan.d_gdesc.p, h.data(), h.size() * sizeof(GatherDesc), hipMemcpyHostToDevice, ctx().stream));
+	GLI_HIP(hipStreamSynchronize(ctx().stream));
+	plan.gdesc_ept = ept;
+	plan.total_pieces = pieces;
+}
+
+static void batch_mode(BatchPlan& plan, int mode, const BatchChoice& ch)
+{
+	ModeArgs a[kMaxGroup];
+	const int n = (int)plan.grids.size();
+	for (int g = 0; g < n; ++g) { a[g] = batch_mode_args(*plan.grids[g], mode); }
+	mode_product_grouped(a, n, ch.gemm_shape[mode], ch.gemm_order[mode]);
+}
+
+static void batch_mode_split(BatchPlan& plan, int g, int mode, const BatchChoice& ch)
+{
+	const ModeArgs a = batch_mode_args(*plan.grids[g], mode);
+	mode_product_split(a.X, a.ldx, a.P, a.C, a.Kd, a.Y, a.ldy, a.Cp, a.tri_cols, a.tri_rows, a.p_base, a.tet_C1, false,
+	                   plan.split_ws, ch.gemm_order[mode]);
+}
+
+// everything after the parameter upload
+static void batch_compute(BatchPlan& plan, const BatchChoice& ch)
+{
+	for (auto& g : plan.grids) { launch_tables(g->s, plan.d_params.p + g->coeff_off, plan.tol, g->f.p, g->P.p, g->B.p); }
+	const bool any_split = ch.split[0] || ch.split[1] || ch.split[2];
+	if (!any_split) {
+		for (int m = 0; m < 3; ++m) { batch_mode(plan, m, ch); }
+	}
+	else {
+		// the split workspace holds one factor at a time: grid by grid
+		for (size_t g = 0; g < plan.grids.size(); ++g) {
+			const BatchGrid& bg = *plan.grids[g];
+			split_factor(bg.B.p, bg.s.Cp, bg.s.Cp, bg.s.Mp, plan.split_ws);
+			for (int m = 0; m < 3; ++m) {
+				if (ch.split[m]) { batch_mode_split(plan, (int)g, m, ch); }
+				else { mode_product_shaped(batch_mode_args(bg, m), ch.gemm_shape[m], ch.gemm_order[m]); }
+			}
+		}
+	}
+	batch_gather_descs(plan, ch.gather_ept);
+	launch_gather_grouped(plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ch.gather_ept, ch.gather_order);
+}
+
+// median time (ms) of a few repetitions of f on the library stream
+template <typename F>
+static double time_ms(F f, int reps = 5)
+{
+	hipStream_t st = ctx().stream;
+	hipEvent_t e0, e1;
+	GLI_HIP(hipEventCreate(&e0));
+	GLI_HIP(hipEventCreate(&e1));
+	f();
+	std::vector<double> ts;
+	for (int r = 0; r < reps; ++r) {
+		GLI_HIP(hipEventRecord(e0, st));
+		f();
+		GLI_HIP(hipEventRecord(e1, st));
+		GLI_HIP(hipEventSynchronize(e1));
+		float ms = 0;
+		GLI_HIP(hipEventElapsedTime(&ms, e0, e1));
+		ts.push_back(ms);
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	std::sort(ts.begin(), ts.end());
+	return ts[ts.size() / 2];
+}
+
+static std::string fmt(const char* f, double v)
+{
+	char b[64];
+	std::snprintf(b, sizeof(b), f, v);
+	return b;
+}
+
+// the autotuner: every kernel choice is timed on the plan's own shapes and data
+static BatchChoice batch_autotune(BatchPlan& plan)
+{
+	BatchChoice ch;
+	std::string rep = "{\"modes\": [";
+	for (auto& g : plan.grids) { launch_tables(g->s, plan.d_params.p + g->coeff_off, plan.tol, g->f.p, g->P.p, g->B.p); }
+	const bool mixed = plan.method == GLI_ERI_TENSOR_MIXED;
+	for (int m = 0; m < 3; ++m) {
+		double best = 1e30;
+		rep += std::string(m ? ", " : "") + "{\"mode\": " + std::to_string(m + 1) + ", \"candidates\": [";
+		bool first = true;
+		for (int shape = 0; shape < kGemmShapes; ++shape) {
+			bool fits = true;
+			for (auto& g : plan.grids) {
+				const ModeArgs a = batch_mode_args(*g, m);
+				fits = fits && gemm_shape_fits(shape, a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0, a.p_base);
+			}
+			if (!fits) { continue; }
+			for (int order = 0; order < 2; ++order) {
+				BatchChoice c = ch;
+				c.gemm_shape[m] = shape;
+				c.gemm_order[m] = order;
+				c.split[m] = false;
+				const double t = time_ms([&] { batch_mode(plan, m, c); });
+				rep += std::string(first ? "" : ", ") + "{\"fp64_shape\": " + std::to_string(shape) + ", \"order\": " +
+				       std::to_string(order) + ", \"ms\": " + fmt("%.4f", t) + "}";
+				first = false;
+				if (t < best) {
+					best = t;
+					ch.gemm_shape[m] = shape;
+					ch.gemm_order[m] = order;
+					ch.split[m] = false;
+				}
+			}
+		}
+		if (mixed) {
+			for (int order = 0; order < 2; ++order) {
+				BatchChoice c = ch;
+				c.gemm_order[m] = order;
+				const double t = time_ms([&] {
+					for (size_t g = 0; g < plan.grids.size(); ++g) {
+						const BatchGrid& bg = *plan.grids[g];
+						split_factor(bg.B.p, bg.s.Cp, bg.s.Cp, bg.s.Mp, plan.split_ws);
+						batch_mode_split(plan, (int)g, m, c);
+					}
+				});
+				rep += std::string(first ? "" : ", ") + "{\"split_bf16\": 1, \"order\": " + std::to_string(order) +
+				       ", \"ms\": " + fmt("%.4f", t) + "}";
+				first = false;
+				if (t < best) {
+					best = t;
+					ch.gemm_order[m] = order;
+					ch.split[m] = true;
+				}
+			}
+		}
+		// leave this mode's output in place for the next mode's timing
+		if (ch.split[m]) {
+			for (size_t g = 0; g < plan.grids.size(); ++g) {
+				const BatchGrid& bg = *plan.grids[g];
+				split_factor(bg.B.p, bg.s.Cp, bg.s.Cp, bg.s.Mp, plan.split_ws);
+				batch_mode_split(plan, (int)g, m, ch);
+			}
+		}
+		else {
+			batch_mode(plan, m, ch);
+		}
+		rep += "], \"chosen\": " + std::string(ch.split[m] ? "\"split_bf16\"" : "\"fp64\"") + ", \"shape\": " +
+		       std::to_string(ch.gemm_shape[m]) + ", \"order\": " + std::to_string(ch.gemm_order[m]) + "}";
+	}
+	rep += "], \"gather\": [";
+	double best = 1e30;
+	bool first = true;
+	for (int ept : { 2, 4, 8 }) {
+		for (int order = 0; order < 2; ++order) {
+			batch_gather_descs(plan, ept);
+			const double t = time_ms([&] { launch_gather_grouped(plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ept, order); });
+			rep += std::string(first ? "" : ", ") + "{\"ept\": " + std::to_string(ept) + ", \"order\": " + std::to_string(order) +
+			       ", \"ms\": " + fmt("%.4f", t) + "}";
+			first = false;
+			if (t < best) {
+				best = t;
+				ch.gather_ept = ept;
+				ch.gather_order = order;
+			}
+		}
+	}
+	rep += "], \"gather_chosen\": {\"ept\": " + std::to_string(ch.gather_ept) + ", \"order\": " + std::to_string(ch.gather_order) + "}}";
+	plan.report = rep;
+	return ch;
+}
+
+static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t* K, const int64_t* grids,
+                               const int64_t* const* d_idx, const int64_t* num, double tol, int method, double* const* d_values,
+                               int autotune)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	GLI_REQUIRE(n >= 1 && n <= kMaxGroup, "a batch holds 1 to 16 grids");
+	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64 || method == GLI_ERI_TENSOR_MIXED, "a batch runs the key-tensor methods (FP64 or mixed)");
+	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
+	std::unique_ptr<BatchPlan> plan(new BatchPlan());
+	plan->method = method;
+	plan->tol = tol;
+	size_t params = 0;
+	for (int g = 0; g < n; ++g) {
+		GLI_REQUIRE(coeffs[g] != nullptr && num[g] >= 0, "bad batch entry");
+		std::unique_ptr<BatchGrid> bg(new BatchGrid());
+		bg->K = K[g];
+		bg->sym = mirror_symmetric(coeffs[g], K[g]);
+		bg->s = eri_shape(K[g], grids + 6 * g, bg->sym);
+		GLI_REQUIRE(method != GLI_ERI_TENSOR_MIXED || bg->s.Mp <= 256,
+		            "the split-precision method needs 6(n-1) + 2(K-1) < 256; use the FP64 method");
+		bg->d_idx = d_idx[g];
+		bg->num = num[g];
+		bg->d_vals = d_values[g];
+		const EriSetup& s = bg->s;
+		bg->f.alloc((size_t)s.R * (2 * s.K - 1));
+		bg->P.alloc((size_t)s.R * s.R * (4 * s.K - 3));
+		bg->B.alloc((size_t)s.Mp * s.Cp);
+		bg->Y1.alloc((size_t)s.Mp * s.Mp * s.Cp);
+		bg->Y2.alloc((size_t)s.Mp * s.Cp * s.Cp);
+		bg->V.alloc((size_t)tet_index(0, 0, s.C1));
+		bool found = false;
+		for (auto& t : plan->t3) {
+			if (t.first == s.M) {
+				bg->T3 = t.second->p;
+				found = true;
+			}
+		}
+		if (!found) {
+			std::unique_ptr<DevBuf<double>> t3(new DevBuf<double>((size_t)s.Mp * s.Mp * s.Mp));
+			launch_kernel_tensor(s, t3->p);
+			bg->T3 = t3->p;
+			plan->t3.emplace_back(s.M, std::move(t3));
+		}
+		bg->coeff_off = params;
+		params += (size_t)K[g];
+		plan->grids.push_back(std::move(bg));
+	}
+	for (auto& g : plan->grids) { g->scale_off = params++; }
+	plan->n_params = params;
+	plan->d_params.alloc(params);
+	GLI_HIP(hipHostMalloc((void**)&plan->h_params, params * sizeof(double), hipHostMallocDefault));
+	for (int g = 0; g < n; ++g) {
+		std::memcpy(plan->h_params + plan->grids[g]->coeff_off, coeffs[g], K[g] * sizeof(double));
+		plan->h_params[plan->grids[g]->scale_off] = 1.0;
+	}
+	if (method == GLI_ERI_TENSOR_MIXED) { plan->split_ws = split_workspace_new(); }
+	GLI_HIP(hipMemcpyAsync(plan->d_params.p, plan->h_params, params * sizeof(double), hipMemcpyHostToDevice, c.stream));
+
+	BatchChoice ch;
+	if (autotune) { ch = batch_autotune(*plan); }
+	else {
+		for (int m = 0; m < 3; ++m) {
+			ch.split[m] = (method == GLI_ERI_TENSOR_MIXED);
+			ch.gemm_order[m] = c.tile_grouped;
+			ch.gemm_shape[m] = c.gemm_shape;
+			for (auto& g : plan->grids) {
+				const ModeArgs a = batch_mode_args(*g, m);
+				if (!gemm_shape_fits(ch.gemm_shape[m], a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0, a.p_base)) { ch.gemm_shape[m] = 0; }
+			}
+		}
+		ch.gather_ept = c.gather_ept;
+		ch.gather_order = c.gather_xcd;
+		plan->report = "{\"autotuned\": false}";
+	}
+	plan->choice = ch;
+	// one eager pass sizes every workspace, then the same sequence is captured
+	batch_compute(*plan, ch);
+	GLI_HIP(hipStreamSynchronize(c.stream));
+	GLI_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+	try {
+		GLI_HIP(hipMemcpyAsync(plan->d_params.p, plan->h_params, params * sizeof(double), hipMemcpyHostToDevice, c.stream));
+		batch_compute(*plan, ch);
+	}
+	catch (...) {
+		hipGraph_t broken = nullptr;
+		(void)hipStreamEndCapture(c.stream, &broken);
+		if (broken) { (void)hipGraphDestroy(broken); }
+		throw;
+	}
+	GLI_HIP(hipStreamEndCapture(c.stream, &plan->graph));
+	GLI_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+	GLI_HIP(hipEventCreateWithFlags(&plan->done, hipEventDisableTiming));
+	return plan.release();
+}
+
+static void batch_run(BatchPlan& plan, const double* const* coeffs, const double* scale)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	if (plan.launched) { GLI_HIP(hipEventSynchronize(plan.done)); }
+	for (size_t g = 0; g < plan.grids.size(); ++g) {
+		const BatchGrid& bg = *plan.grids[g];
+		if (coeffs != nullptr) {
+			GLI_REQUIRE(coeffs[g] != nullptr, "missing coefficients in a batch run");
+			GLI_REQUIRE(mirror_symmetric(coeffs[g], bg.K) == bg.sym,
+			            "a batch plan is built for one coefficient symmetry per grid: mirror symmetry changed");
+			std::memcpy(plan.h_params + bg.coeff_off, coeffs[g], bg.K * sizeof(double));
+		}
+		plan.h_params[bg.scale_off] = (scale != nullptr) ? scale[g] : 1.0;
+	}
+	GLI_HIP(hipGraphLaunch(plan.exec, c.stream));
+	GLI_HIP(hipEventRecord(plan.done, c.stream));
+	plan.launched = true;
 }
 
 }  // namespace gli
@@ -788,13 +1380,70 @@ int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, cons
 	try {
 		Context& c = ctx();
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
-		for (int b = 0; b < num_grids; ++b) {
-			eri_values_dev(coeffs[b], K[b], grids + 6 * b, d_idx[b], num[b], tol, method, d_values[b]);
+		if (method == GLI_ERI_DIRECT_FP64) {
+			for (int b = 0; b < num_grids; ++b) {
+				eri_values_dev(coeffs[b], K[b], grids + 6 * b, d_idx[b], num[b], tol, method, d_values[b]);
+			}
+			return 0;
 		}
+		std::unique_ptr<BatchPlan> plan(batch_create(num_grids, coeffs, K, grids, d_idx, num, tol, method, d_values, 0));
+		batch_run(*plan, nullptr, nullptr);
+		GLI_HIP(hipEventSynchronize(plan->done));
 		return 0;
 	}
 	catch (const Failure&) {
 		return -1;
+	}
+}
+
+int gli_eri_batch_create(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,
+                         const int64_t* const* d_idx, const int64_t* num, double tol, int method, double* const* d_values,
+                         int autotune, gli_eri_batch** plan)
+{
+	try {
+		GLI_REQUIRE(plan != nullptr, "null plan pointer");
+		*plan = nullptr;
+		*plan = reinterpret_cast<gli_eri_batch*>(batch_create(num_grids, coeffs, K, grids, d_idx, num, tol, method, d_values, autotune));
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_eri_batch_run(gli_eri_batch* plan, const double* const* coeffs, const double* scale)
+{
+	try {
+		GLI_REQUIRE(plan != nullptr, "null plan");
+		batch_run(*reinterpret_cast<BatchPlan*>(plan), coeffs, scale);
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_eri_batch_describe(const gli_eri_batch* plan, char* buf, int len)
+{
+	if (plan == nullptr) { return -1; }
+	const std::string& r = reinterpret_cast<const BatchPlan*>(plan)->report;
+	if (buf != nullptr && len > 0) {
+		const size_t k = std::min(r.size(), (size_t)len - 1);
+		std::memcpy(buf, r.data(), k);
+		buf[k] = '\0';
+	}
+	return (int)r.size();
+}
+
+void gli_eri_batch_destroy(gli_eri_batch* plan)
+{
+	if (plan == nullptr) { return; }
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		delete reinterpret_cast<BatchPlan*>(plan);
+	}
+	catch (const Failure&) {
 	}
 }
 
@@ -827,6 +1476,7 @@ int gli_eri_key_count(int64_t R, int symmetric)
 
 void gli_release_workspace(void)
 {
+	std::lock_guard<std::recursive_mutex> lk(ctx().mu);
 	EriWorkspace& w = eri_ws();
 	w.coeffs.release(); w.f.release(); w.P.release(); w.B.release();
 	w.Y1.release(); w.Y2.release(); w.V.release(); w.T3.release();

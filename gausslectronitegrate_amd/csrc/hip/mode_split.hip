@@ -201,6 +201,16 @@ static SplitWorkspace& split_ws()
 	return *w;
 }
 
+SplitWorkspace* split_workspace_new()
+{
+	return new SplitWorkspace();
+}
+
+void split_workspace_delete(SplitWorkspace* w)
+{
+	delete w;
+}
+
 // digits of a Kd x P operand into (D, e) with padding; returns Ppad
 static int64_t split_operand(const double* X, int64_t ld, int64_t P, int Kd, int Kp, DevBuf<uint16_t>& D, DevBuf<int>& e)
 {
@@ -215,18 +225,19 @@ static int64_t split_operand(const double* X, int64_t ld, int64_t P, int Kd, int
 }
 
 // the B operand (factor matrix) is split once per evaluation and reused by all modes
-void split_factor(const double* B, int64_t ldb, int C, int Kd)
+void split_factor(const double* B, int64_t ldb, int C, int Kd, SplitWorkspace* ws)
 {
-	SplitWorkspace& w = split_ws();
+	SplitWorkspace& w = ws ? *ws : split_ws();
 	const int Kp = (int)round_up(Kd, SBK);
 	split_operand(B, ldb, C, Kd, Kp, w.bd, w.be);
 }
 
 void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, double* Y, int64_t ldy,
-                        int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, bool cache_x)
+                        int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, bool cache_x,
+                        SplitWorkspace* ws, int tile_order)
 {
 	if (P <= 0 || C <= 0) { return; }
-	SplitWorkspace& w = split_ws();
+	SplitWorkspace& w = ws ? *ws : split_ws();
 	const int Kp = (int)round_up(Kd, SBK);
 	const int64_t Cpad = round_up(C, SBN);
 	const uint16_t* xd;
@@ -248,7 +259,7 @@ void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, 
 		xd = w.xd.p;
 		xe = w.xe.p;
 	}
-	const TileOrder order{(int)(Ppad / SBM), (int)(Cpad / SBN), tile_order_grouped()};
+	const TileOrder order{(int)(Ppad / SBM), (int)(Cpad / SBN), tile_order >= 0 ? tile_order : tile_order_grouped()};
 	const unsigned grid = (unsigned)order.blocks();
 	if (Kp <= KC) {
 		k_mode_product_split<true><<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,

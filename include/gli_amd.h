@@ -37,8 +37,12 @@ void* gli_stream(void);
 int gli_synchronize(void);
 
 /* Kernel tuning knobs (set per workload by an autotuner; the defaults are tuned for the 11^3 grid):
- *   "tile_order"  output-tile order of the mode products: 0 row tiles fastest, 1 XCD-grouped (default)
+ *   "tile_order"  output-tile order of the mode products: 0 row tiles fastest (default), 1 XCD-grouped
+ *   "gemm_shape"  FP64 mode-product tile: 0 64x64 (default), 1 128x64, 2 64x128, 3 128x128 (a shape
+ *                 that does not divide the key padding of a triangular product falls back to 0)
  *   "gather_ept"  elements per lane in the per-element gather: 2, 4 (default) or 8
+ *   "gather_order" 0: workgroups take consecutive pieces of the list; 1 (default): each XCD takes one
+ *                 contiguous chunk of the list (its L2 then holds the key-tensor entries it reuses)
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
 int gli_set_tuning(const char* knob, int value);
 int gli_get_tuning(const char* knob);
@@ -66,10 +70,31 @@ int gli_eri_values_device(const double* coeffs, int64_t K, const int64_t grid[6]
 int gli_erida_values_device(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_indices,
                             int64_t num, double tol, double* d_values);
 
-/* several grids in one call (for example 5^3, 7^3, 9^3 and 11^3 with different K) */
+/* several grids in one call (for example 5^3, 7^3, 9^3 and 11^3 with different K): a batch plan
+ * without autotuning, run once; synchronous */
 int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,
                                 const int64_t* const* d_indices, const int64_t* num, double tol, int method,
                                 double* const* d_values);
+
+/* Batch plan for repeated evaluation of up to 16 grids (grids[6*g .. 6*g+5], K[g] coefficients,
+ * d_indices[g] / num[g] device-resident four-indices, d_values[g] device outputs; method
+ * GLI_ERI_TENSOR_FP64 or GLI_ERI_TENSOR_MIXED).  The plan owns every intermediate buffer.  With
+ * autotune != 0 it times the candidate kernels on its own shapes (mode-product tile shape and
+ * order, FP64 or split-precision products under the mixed method, gather variant) and keeps the
+ * fastest; then it captures the whole evaluation (parameter upload, tables, three grouped mode
+ * products, one grouped gather over all grids) into one hipGraph.
+ * gli_eri_batch_run replays it with new coefficients (NULL: those of the previous run, which must
+ * keep each grid's mirror symmetry) and per-grid output scales (NULL: 1; e.g. 1/a for a grid
+ * spacing a); it is asynchronous on gli_stream().  gli_eri_batch_describe writes the autotuner's
+ * timings and choices as JSON (returns the full length).  The index and value arrays must stay
+ * allocated until the plan is destroyed. */
+typedef struct gli_eri_batch gli_eri_batch;
+int gli_eri_batch_create(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,
+                         const int64_t* const* d_indices, const int64_t* num, double tol, int method,
+                         double* const* d_values, int autotune, gli_eri_batch** plan);
+int gli_eri_batch_run(gli_eri_batch* plan, const double* const* coeffs, const double* scale);
+int gli_eri_batch_describe(const gli_eri_batch* plan, char* buf, int len);
+void gli_eri_batch_destroy(gli_eri_batch* plan);
 
 /* Y[p][k] = sum_{mu < Kd} X[mu*ldx + p] * B[mu*ldb + k] (FP64 MFMA; device pointers, synchronous) */
 int gli_mode_product_device(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int64_t C, int64_t Kd,

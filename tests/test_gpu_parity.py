@@ -343,3 +343,156 @@ def test_eri_mixed_full_7(L):
     v64 = eri_values_device(c, sym(7), idx, 1e-16, 0)
     vmx = eri_values_device(c, sym(7), idx, 1e-16, 1)
     assert np.max(np.abs(vmx - v64)) <= 1e-10
+
+
+# ---------------------------------------------------------------------------
+# tuning knobs: every variant computes the same terms in the same order per element
+
+def test_tuning_variants_bit_identical(L):
+    """tile order of the mode products, gather order and elements per lane change the schedule only:
+    the full 7^3 list (and an unsorted, ragged copy of it) must come out bit-identical"""
+    say("tuning variants")
+    c = golden("coeffs_K57_s0.npy")
+    d, n = eri_enumerate_device(7)
+    idx = d.to_numpy(np.int64, n)
+    rng = np.random.default_rng(5)
+    shuffled = idx[rng.permutation(n)][: n - 77]
+    knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1)), (b"gather_ept", (2, 4, 8))]
+    saved = {k: L.gli_get_tuning(k) for k, _ in knobs}
+    try:
+        ref = ref_s = None
+        for to in (0, 1):
+            for go in (0, 1):
+                for ept in (2, 4, 8):
+                    for k, v in zip((b"tile_order", b"gather_order", b"gather_ept"), (to, go, ept)):
+                        assert L.gli_set_tuning(k, v) == 0
+                    for method in (0, 1):
+                        got = eri_values_device(c, sym(7), idx, 1e-16, method)
+                        got_s = eri_values_device(c, sym(7), shuffled, 1e-16, method)
+                        if ref is None:
+                            ref, ref_s = {}, {}
+                        if method not in ref:
+                            ref[method], ref_s[method] = got, got_s
+                        assert np.array_equal(got, ref[method]), (to, go, ept, method)
+                        assert np.array_equal(got_s, ref_s[method]), (to, go, ept, method)
+        assert np.array_equal(ref_s[0], ref[0][np.searchsorted(idx, shuffled)])
+        assert L.gli_set_tuning(b"gather_ept", 3) == -1
+        assert L.gli_set_tuning(b"no_such_knob", 1) == -1
+        assert L.gli_get_tuning(b"no_such_knob") == -1
+    finally:
+        L.gli_clear_error()
+        for k, v in saved.items():
+            L.gli_set_tuning(k, v)
+
+
+# ---------------------------------------------------------------------------
+# batch plans: several grids, grouped launches, autotuner, one replayed hipGraph
+
+def _batch_setup(specs):
+    """specs: list of (grid spec, coefficients, indices or None for the full enumerated list)"""
+    entries = []
+    for spec, c, idx in specs:
+        if idx is None:
+            n = spec[0][1]
+            d, num = eri_enumerate_device(n)
+            idx = d.to_numpy(np.int64, num)
+        di = DevArray.from_numpy(np.ascontiguousarray(idx, dtype=np.int64))
+        dv = DevArray(8 * max(len(idx), 1))
+        entries.append((spec, np.ascontiguousarray(c, dtype=np.float64), np.ascontiguousarray(idx), di, dv))
+    return entries
+
+
+def _batch_arrays(entries, coeffs=None):
+    n = len(entries)
+    cs = coeffs if coeffs is not None else [e[1] for e in entries]
+    cptr = (_abi.P_double * n)(*[c.ctypes.data_as(_abi.P_double) for c in cs])
+    Ks = np.array([len(c) for c in cs], dtype=np.int64)
+    grids = np.concatenate([np.array([v for ax in e[0] for v in ax], dtype=np.int64) for e in entries])
+    dptr = (C.c_void_p * n)(*[e[3].ptr for e in entries])
+    nums = np.array([len(e[2]) for e in entries], dtype=np.int64)
+    vptr = (C.c_void_p * n)(*[e[4].ptr for e in entries])
+    return cptr, Ks, grids, dptr, nums, vptr
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_batch_plan_matches_single_grids(L, method):
+    """a 4-grid plan (uneven K, one non-cubic grid with an unsorted sample): every replay equals the
+    single-grid path bit for bit (FP64; the same kernels in the same summation order) or within
+    1e-10 of FP64 (mixed, whose autotuner may keep FP64 products where they are faster); output
+    scales, new coefficients and a symmetry change behave as documented"""
+    say("batch plan method", method)
+    rng = np.random.default_rng(11)
+    ragged = rng.integers(0, (5 * 3 * 4) ** 4, size=5000)
+    specs = [(sym(3), golden("coeffs_K29_s0.npy"), None),
+             (sym(5), golden("coeffs_K43_s0.npy"), None),
+             (sym(7), golden("coeffs_K57_s1.npy"), None),
+             (((-2, 5), (-1, 3), (-2, 4)), golden("coeffs_K57_s0.npy"), ragged)]
+    entries = _batch_setup(specs)
+    cptr, Ks, grids, dptr, nums, vptr = _batch_arrays(entries)
+    plan = C.c_void_p()
+    _lib.check(L.gli_eri_batch_create(len(entries), cptr, Ks.ctypes.data_as(_abi.P_glong), grids.ctypes.data_as(_abi.P_glong),
+                                      dptr, nums.ctypes.data_as(_abi.P_glong), 1e-16, method, vptr, 1, C.byref(plan)),
+               "gli_eri_batch_create")
+    try:
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        assert L.gli_eri_batch_describe(plan, buf, len(buf)) > 0
+        report = json.loads(buf.value.decode())
+        assert len(report["modes"]) == 3 and report["gather"]
+        singles = [eri_values_device(e[1], e[0], e[2], 1e-16, 0) for e in entries]
+        for rep in range(3):
+            _lib.check(L.gli_eri_batch_run(plan, None, None), "gli_eri_batch_run")
+            assert L.gli_synchronize() == 0
+            for e, ref in zip(entries, singles):
+                got = e[4].to_numpy(np.float64, len(e[2]))
+                if method == 0:
+                    assert np.array_equal(got, ref), (rep, e[0])
+                else:
+                    assert np.max(np.abs(got - ref)) <= 1e-10, (rep, e[0])
+            # eager work between replays
+            eri_values_device(entries[0][1], entries[0][0], entries[0][2][:100], 1e-16, 0)
+        scales = np.array([0.5, 1.0, 2.0, 1.0 / 1.5])
+        _lib.check(L.gli_eri_batch_run(plan, None, scales.ctypes.data_as(C.POINTER(C.c_double))), "run scaled")
+        assert L.gli_synchronize() == 0
+        for e, ref, sc in zip(entries, singles, scales):
+            got = e[4].to_numpy(np.float64, len(e[2]))
+            if method == 0:
+                assert np.array_equal(got, ref * sc)
+            else:
+                assert np.max(np.abs(got - ref * sc)) <= 2e-10
+        # new coefficients with the same mirror symmetry
+        new = [golden("coeffs_K29_s0.npy")[::-1].copy(), golden("coeffs_K43_s0.npy") * 0.5,
+               golden("coeffs_K57_s0.npy"), golden("coeffs_K57_s1.npy")]
+        assert all(np.array_equal(c, c[::-1]) == np.array_equal(e[1], e[1][::-1]) for c, e in zip(new, entries))
+        ncptr = (_abi.P_double * len(new))(*[c.ctypes.data_as(_abi.P_double) for c in new])
+        _lib.check(L.gli_eri_batch_run(plan, ncptr, None), "run new coefficients")
+        assert L.gli_synchronize() == 0
+        for e, c in zip(entries, new):
+            ref = eri_values_device(c, e[0], e[2], 1e-16, 0)
+            got = e[4].to_numpy(np.float64, len(e[2]))
+            if method == 0:
+                assert np.array_equal(got, ref)
+            else:
+                assert np.max(np.abs(got - ref)) <= 1e-10
+        # a coefficient set without mirror symmetry does not fit a plan built for symmetric ones
+        bad = [c.copy() for c in new]
+        bad[1][0] += 1e-3
+        bcptr = (_abi.P_double * len(bad))(*[c.ctypes.data_as(_abi.P_double) for c in bad])
+        assert L.gli_eri_batch_run(plan, bcptr, None) == -1
+        assert b"symmetry" in L.gli_last_error()
+        L.gli_clear_error()
+    finally:
+        L.gli_eri_batch_destroy(plan)
+
+
+def test_batch_one_shot_entry(L):
+    """gli_eri_values_batch_device (a plan without autotuning, run once) equals the single-grid path"""
+    say("batch one shot")
+    specs = [(sym(5), golden("coeffs_K29_s0.npy"), None), (sym(3), golden("coeffs_K57_s0.npy"), None)]
+    entries = _batch_setup(specs)
+    cptr, Ks, grids, dptr, nums, vptr = _batch_arrays(entries)
+    _lib.check(L.gli_eri_values_batch_device(len(entries), cptr, Ks.ctypes.data_as(_abi.P_glong), grids.ctypes.data_as(_abi.P_glong),
+                                             dptr, nums.ctypes.data_as(_abi.P_glong), 1e-16, 0, vptr), "batch")
+    for e in entries:
+        ref = eri_values_device(e[1], e[0], e[2], 1e-16, 0)
+        assert np.array_equal(e[4].to_numpy(np.float64, len(e[2])), ref)
