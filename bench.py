@@ -29,7 +29,8 @@ from gausslectronitegrate_amd.coeffs import synthetic_coefficients  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
-METHODS = {"tensor": 0, "mixed": 1, "direct": 2}
+METHODS = {"tensor": 0, "mixed": 1, "direct": 2, "bf16x2": 3, "fp32": 4}
+DTYPES = {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64", "bf16x2": "bf16x2+f32acc", "fp32": "f32"}
 
 
 def log(*a):
@@ -231,7 +232,7 @@ def batch_bench(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64"}[args.method],
+        "dtype": DTYPES[args.method],
         "data": "synthetic (seeded gausslet coefficients)",
         "config": {"workload": "eri_batch_5-7-9-11", "grids": BATCH, "tol": args.tol, "method": args.method,
                    "unique_elements": total},
@@ -258,8 +259,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--asymmetric", action="store_true",
                     help="coefficients without mirror symmetry (general axis-key set, GEMM-dominated)")
-    ap.add_argument("--no-compare", action="store_true",
-                    help="skip timing the split-precision method beside the FP64 one")
+    ap.add_argument("--compare", default="mixed",
+                    help="comma-separated methods timed beside the main one, each with its error against the main "
+                         "FP64 values over every element (e.g. mixed,bf16x2,fp32)")
+    ap.add_argument("--no-compare", action="store_true", help="time no other method")
     ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
                     help="set a kernel tuning knob (gli_set_tuning) before timing; repeatable")
     ap.add_argument("--batch", action="store_true",
@@ -355,34 +358,36 @@ def main():
     direct_rate = len(s_idx) / (direct_ms / 1e3)
     log("direct FP64 sample: %d elements in %.2f ms; max abs err %.3g" % (len(s_idx), direct_ms, err_abs))
 
-    # the split-precision method on the same workload: time and error over every element
-    compare = None
-    if not args.no_compare and method != METHODS["mixed"]:
-        mixed = METHODS["mixed"]
-        d_mix = Dev(L, 8 * num)
-        eri_step(L, c, g, d_idx, num, args.tol, mixed, d_mix.ptr)
+    # other methods on the same workload: time and error over every element
+    compare = []
+    for name in ([] if args.no_compare else [m for m in args.compare.split(",") if m and m != args.method]):
+        other = METHODS[name]
+        d_oth = Dev(L, 8 * num)
+        eri_step(L, c, g, d_idx, num, args.tol, other, d_oth.ptr)
         L.gli_synchronize()
         torch.cuda.synchronize()
         m0 = torch.cuda.Event(enable_timing=True)
         m1 = torch.cuda.Event(enable_timing=True)
         m0.record(stream)
         for _ in range(args.steps):
-            eri_step(L, c, g, d_idx, num, args.tol, mixed, d_mix.ptr)
+            eri_step(L, c, g, d_idx, num, args.tol, other, d_oth.ptr)
         m1.record(stream)
         L.gli_synchronize()
         torch.cuda.synchronize()
-        mix_ms = m0.elapsed_time(m1) / args.steps
-        mix_vals = np.empty(num)
-        _lib.check(L.gli_memcpy(mix_vals.ctypes.data, d_mix.ptr, 8 * num, 1), "copy mixed values")
-        del d_mix
-        diff = np.abs(mix_vals - vals)
+        oth_ms = m0.elapsed_time(m1) / args.steps
+        oth_vals = np.empty(num)
+        _lib.check(L.gli_memcpy(oth_vals.ctypes.data, d_oth.ptr, 8 * num, 1), "copy values")
+        del d_oth
+        diff = np.abs(oth_vals - vals)
         big = np.abs(vals) >= 1e-10
-        compare = {"method": "mixed", "ms_per_step": mix_ms, "value": num / (mix_ms / 1e3),
-                   "max_abs_err_vs_fp64_all_elements": float(diff.max()),
-                   "max_rel_err_vs_fp64_where_abs_ge_1e-10": float((diff[big] / np.abs(vals[big])).max()) if big.any() else None,
-                   "max_abs_err_vs_fp64_direct_sample": float(np.max(np.abs(mix_vals[samp] - direct_vals)))}
-        del mix_vals, diff, big
-        log("mixed: %.3f ms/step, max abs err vs FP64 over all elements %.3g" % (mix_ms, compare["max_abs_err_vs_fp64_all_elements"]))
+        compare.append({"method": name, "dtype": DTYPES[name], "ms_per_step": oth_ms, "value": num / (oth_ms / 1e3),
+                        "max_abs_err_vs_%s_all_elements" % args.method: float(diff.max()),
+                        "max_rel_err_vs_%s_where_abs_ge_1e-10" % args.method:
+                            float((diff[big] / np.abs(vals[big])).max()) if big.any() else None,
+                        "max_abs_err_vs_fp64_direct_sample": float(np.max(np.abs(oth_vals[samp] - direct_vals)))})
+        del oth_vals, diff, big
+        log("%s: %.3f ms/step, max abs err vs %s over all elements %.3g"
+            % (name, oth_ms, args.method, compare[-1]["max_abs_err_vs_%s_all_elements" % args.method]))
 
     # roofline of the dominant kernel
     workload = "eri_%dx%dx%d_K%d_seed%d%s" % (args.n, args.n, args.n, args.K, args.seed, "_asym" if args.asymmetric else "")
@@ -426,7 +431,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64"}[args.method],
+        "dtype": DTYPES[args.method],
         "data": "synthetic (seeded gausslet coefficients)",
         "config": {"workload": workload,
                    "grid": args.n, "K": args.K, "seed": args.seed, "tol": args.tol, "method": args.method,
@@ -441,7 +446,7 @@ def main():
         "max_abs_err_vs_cpu_reference": cpu_err,
         "speedup_vs_fp64_direct": value / direct_rate,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "split_precision": compare,
+        "compare": compare,
     }
     L.gli_device_free(d_idx)
     print(json.dumps(line), flush=True)

@@ -193,11 +193,13 @@ void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int6
 // split-precision (BF16 digit slices, exact FP32 slice sums, FP64 combination) variant:
 // split_factor() prepares B once; mode_product_split() then contracts X with it.  Both use the
 // library's shared split workspace unless a caller-owned one is given (a captured graph needs
-// buffers that no later call can reallocate); tile_order < 0 takes the context knob.
+// buffers that no later call can reallocate); tile_order < 0 takes the context knob; ns is the
+// number of base-256 digit slices (5: mixed method, 2: BF16 hi/lo comparison method), fixed by
+// split_factor for the products that follow it.
 struct SplitWorkspace;
 SplitWorkspace* split_workspace_new();
 void split_workspace_delete(SplitWorkspace* w);
-void split_factor(const double* B, int64_t ldb, int C, int Kd, SplitWorkspace* ws = nullptr);
+void split_factor(const double* B, int64_t ldb, int C, int Kd, SplitWorkspace* ws = nullptr, int ns = 5);
 void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, double* Y, int64_t ldy,
                         int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1, bool cache_x,
                         SplitWorkspace* ws = nullptr, int tile_order = -1);

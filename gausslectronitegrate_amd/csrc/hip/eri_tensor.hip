@@ -242,6 +242,72 @@ __global__ __launch_bounds__(256) void k_mode_product(ModeArgs a, TileOrder orde
 	mode_product_tile<BM, BN>(a, rt, ct, As, Bs);
 }
 
+// Plain FP32 variant of the mode product (v_mfma_f32_16x16x4f32, FP32 products and FP32
+// accumulation over the whole mu range, FP64 output): the comparison path of the precision study,
+// not accurate enough for the 1e-10 target.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_mode_product_f32(ModeArgs a, TileOrder order)
+{
+	__shared__ float As[GBK][64];
+	__shared__ float Bs[GBK][64];
+	int rt, ct;
+	if (!order.tile(blockIdx.x, rt, ct)) { return; }
+	const int tid = threadIdx.x;
+	const int lane = tid & 63, wave = tid >> 6;
+	const int64_t p0 = (int64_t)rt * 64;
+	const int k0 = ct * 64;
+	if (a.tri_cols || a.tri_rows) {
+		const int64_t gp = a.p_base + p0;
+		const int64_t outer = gp / a.Cp;
+		const int64_t rlo = gp % a.Cp, rhi = rlo + 63;
+		if (a.tri_cols && k0 + 63 < rlo) { return; }
+		if (a.tri_rows && rhi < outer) { return; }
+	}
+	const int wm = wave >> 1, wn = wave & 1;
+	f4v acc[2][2];
+	for (int i = 0; i < 2; ++i) {
+		for (int j = 0; j < 2; ++j) { acc[i][j] = f4v{0, 0, 0, 0}; }
+	}
+	for (int mu0 = 0; mu0 < a.Kd; mu0 += GBK) {
+		for (int r = 0; r < (GBK * 64) / 256; ++r) {
+			const int id = tid + 256 * r;
+			const int mu = id / 64, q = id % 64;
+			As[mu][q] = (mu0 + mu < a.Kd && p0 + q < a.P) ? (float)a.X[(int64_t)(mu0 + mu) * a.ldx + p0 + q] : 0.0f;
+			Bs[mu][q] = (mu0 + mu < a.Kd && k0 + q < a.C) ? (float)a.B[(int64_t)(mu0 + mu) * a.ldb + k0 + q] : 0.0f;
+		}
+		__syncthreads();
+		for (int kk = 0; kk < GBK; kk += 4) {
+			float av[2], bv[2];
+			for (int i = 0; i < 2; ++i) { av[i] = As[kk + lane / 16][wm * 32 + i * 16 + lane % 16]; }
+			for (int j = 0; j < 2; ++j) { bv[j] = Bs[kk + lane / 16][wn * 32 + j * 16 + lane % 16]; }
+			for (int i = 0; i < 2; ++i) {
+				for (int j = 0; j < 2; ++j) { acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0); }
+			}
+		}
+		__syncthreads();
+	}
+	// FP32 16x16 accumulator layout: lane holds column lane%16 and rows 4*(lane/16) + v
+	for (int i = 0; i < 2; ++i) {
+		for (int j = 0; j < 2; ++j) {
+			const int col = k0 + wn * 32 + j * 16 + lane % 16;
+			for (int v = 0; v < 4; ++v) {
+				const int64_t row = p0 + wm * 32 + i * 16 + 4 * (lane / 16) + v;
+				if (row >= a.P || col >= a.C) { continue; }
+				const double val = (double)acc[i][j][v];
+				if (a.tet_C1 > 0) {
+					const int64_t gp = a.p_base + row;
+					const int k1 = (int)(gp / a.Cp), k2 = (int)(gp % a.Cp);
+					if (k1 <= k2 && k2 <= col && col < a.tet_C1) { a.Y[tet_index(k1, k2, col)] = val; }
+				}
+				else {
+					a.Y[row * a.ldy + col] = val;
+				}
+			}
+		}
+	}
+}
+
 // several independent mode products (one per grid of a batch) in one launch; each problem's
 // workgroups start at a multiple of 8 so that the XCD-grouped order keeps its meaning
 constexpr int kMaxGroup = 16;
@@ -874,6 +940,16 @@ void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int6
 	mode_product_shaped(a, shape, tile_order_grouped());
 }
 
+static void mode_product_f32(const ModeArgs& a)
+{
+	if (a.P <= 0 || a.C <= 0) { return; }
+	GLI_REQUIRE(!(a.tri_cols || a.tri_rows || a.tet_C1 > 0) || (a.Cp % 64 == 0 && a.p_base % 64 == 0),
+	            "mode-product tile does not fit the key padding");
+	const TileOrder o{(int)ceil_div(a.P, 64), (int)ceil_div(a.C, 64), 0};
+	k_mode_product_f32<<<(unsigned)o.blocks(), 256, 0, ctx().stream>>>(a, o);
+	GLI_HIP(hipGetLastError());
+}
+
 void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped)
 {
 	GLI_REQUIRE(n >= 1 && n <= kMaxGroup, "too many problems in one grouped mode product");
@@ -910,8 +986,7 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	GLI_REQUIRE(num >= 0, "negative number of indices");
-	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64 || method == GLI_ERI_DIRECT_FP64 || method == GLI_ERI_TENSOR_MIXED,
-	            "unknown ERI method");
+	GLI_REQUIRE(method >= GLI_ERI_TENSOR_FP64 && method <= GLI_ERI_TENSOR_FP32, "unknown ERI method");
 	GLI_REQUIRE(coeffs != nullptr && K >= 3, "the number of gausslet coefficients must be odd and at least 3");
 	phase_begin();
 	const EriSetup s = eri_shape(K, grid, mirror_symmetric(coeffs, K));
@@ -936,8 +1011,15 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 		phase_mark("direct");
 		return;
 	}
-	const bool mixed = (method == GLI_ERI_TENSOR_MIXED);
+	const bool mixed = (method == GLI_ERI_TENSOR_MIXED || method == GLI_ERI_TENSOR_BF16X2);
+	const int ns = (method == GLI_ERI_TENSOR_BF16X2) ? 2 : 5;
+	const bool fp32 = (method == GLI_ERI_TENSOR_FP32);
 	GLI_REQUIRE(!mixed || s.Mp <= 256, "the split-precision method needs 6(n-1) + 2(K-1) < 256; use the FP64 method");
+	// the mode products of the comparison FP32 path
+	auto mode = [&](const double* X, int64_t ldx, int64_t P, double* Y, int64_t ldy, int tri_cols, int tri_rows, int tet_C1) {
+		if (fp32) { mode_product_f32(ModeArgs{X, ldx, P, w.B.p, Cp, s.Cp, s.Mp, Y, ldy, s.Cp, tri_cols, tri_rows, 0, tet_C1}); }
+		else { mode_product(X, ldx, P, w.B.p, Cp, s.Cp, s.Mp, Y, ldy, s.Cp, tri_cols, tri_rows, 0, tet_C1); }
+	};
 
 	// kernel tensor, cached per folded range
 	if (w.T3_M != s.M) {
@@ -948,24 +1030,24 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	}
 	phase_mark("kernel_tensor");
 	if (mixed) {
-		split_factor(w.B.p, Cp, s.Cp, s.Mp);
+		split_factor(w.B.p, Cp, s.Cp, s.Mp, nullptr, ns);
 		phase_mark("split_factor");
 	}
 	// Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]
 	w.Y1.reserve((size_t)Mp * Mp * Cp);
 	if (mixed) { mode_product_split(w.T3.p, Mp * Mp, Mp * Mp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0, true); }
-	else { mode_product(w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0); }
+	else { mode(w.T3.p, Mp * Mp, Mp * Mp, w.Y1.p, Cp, 0, 0, 0); }
 	phase_mark("mode1");
 	// Y2[m3][k1][k2] = sum_m2 Y1[m2][m3][k1] B[m2][k2]   (k2 >= k1 only)
 	w.Y2.reserve((size_t)Mp * Cp * Cp);
 	if (mixed) { mode_product_split(w.Y1.p, Mp * Cp, Mp * Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0, false); }
-	else { mode_product(w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0); }
+	else { mode(w.Y1.p, Mp * Cp, Mp * Cp, w.Y2.p, Cp, 1, 0, 0); }
 	phase_mark("mode2");
 	// V[k1 <= k2 <= k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3], written in tetrahedral layout
 	const int64_t nV = tet_index(0, 0, s.C1);
 	w.V.reserve((size_t)nV);
 	if (mixed) { mode_product_split(w.Y2.p, Cp * Cp, Cp * Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1, false); }
-	else { mode_product(w.Y2.p, Cp * Cp, Cp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1); }
+	else { mode(w.Y2.p, Cp * Cp, Cp * Cp, w.V.p, 0, 1, 1, s.C1); }
 	phase_mark("mode3");
 	launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept, c.gather_xcd);
 	phase_mark("gather");

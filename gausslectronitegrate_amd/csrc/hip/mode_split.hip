@@ -20,7 +20,8 @@ namespace gli {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int NS = 5;                         // digit slices per operand
+// NS = digit slices per operand: 5 for the mixed method (dropped terms below ~2^-40 of the column
+// scales), 2 for the BF16 hi/lo comparison method (~2^-16)
 constexpr int SBM = 64, SBN = 64, SBK = 32;   // block tile; K step = one MFMA 16x16x32
 constexpr int SPAD = 8;                       // LDS row padding (bf16 elements)
 
@@ -40,6 +41,7 @@ __global__ void k_column_exponents(const double* __restrict__ X, int64_t ld, int
 
 // base-256 digits of a 32 (mu) x 64 (p) tile, written as D[(s * Ppad + p) * Kp + mu]
 // (mu contiguous per column: the GEMM loads 16-byte runs of 8 digits)
+template <int NS>
 __global__ __launch_bounds__(256) void k_split_tile(const double* __restrict__ X, int64_t ld, int64_t P, int Kd,
 	int64_t Ppad, int Kp, const int* __restrict__ e, uint16_t* __restrict__ D)
 {
@@ -67,9 +69,9 @@ __global__ __launch_bounds__(256) void k_split_tile(const double* __restrict__ X
 }
 
 // Y[p][k] from the digits of X (Ppad columns) and B (Cpad columns)
-constexpr int KC = 192;  // exactness bound of an FP32 group accumulator: NS * 2^14 * KC < 2^24
+constexpr int KC = 192;  // exactness bound of an FP32 group accumulator: NS * 2^14 * KC < 2^24 for NS <= 5
 
-template <bool kOneChunk>  // true when the whole sum fits one exact chunk (Kp <= KC)
+template <int NS, bool kOneChunk>  // kOneChunk: the whole sum fits one exact chunk (Kp <= KC)
 __global__ __launch_bounds__(256) void k_mode_product_split(
 	const uint16_t* __restrict__ Xd, int64_t Ppad, const int* __restrict__ ex,
 	const uint16_t* __restrict__ Bd, int64_t Cpad, const int* __restrict__ eb,
@@ -193,6 +195,8 @@ struct SplitWorkspace {
 	DevBuf<int> xe, be, t3e;
 	const double* t3_src = nullptr;
 	int64_t t3_P = -1;
+	int t3_ns = 0;
+	int b_ns = 0;   // digit count of the factor held in bd
 };
 
 static SplitWorkspace& split_ws()
@@ -212,24 +216,29 @@ void split_workspace_delete(SplitWorkspace* w)
 }
 
 // digits of a Kd x P operand into (D, e) with padding; returns Ppad
-static int64_t split_operand(const double* X, int64_t ld, int64_t P, int Kd, int Kp, DevBuf<uint16_t>& D, DevBuf<int>& e)
+static int64_t split_operand(const double* X, int64_t ld, int64_t P, int Kd, int Kp, DevBuf<uint16_t>& D, DevBuf<int>& e,
+                             int ns)
 {
+	GLI_REQUIRE(ns == 2 || ns == 5, "digit slices must be 2 or 5");
 	const int64_t Ppad = round_up(P, SBM);
-	D.reserve((size_t)NS * Ppad * Kp);
+	D.reserve((size_t)ns * Ppad * Kp);
 	e.reserve((size_t)Ppad);
 	hipStream_t st = ctx().stream;
 	k_column_exponents<<<(unsigned)ceil_div(Ppad, 256), 256, 0, st>>>(X, ld, P, Kd, Ppad, e.p);
-	k_split_tile<<<dim3((unsigned)(Ppad / 64), (unsigned)(Kp / SBK)), 256, 0, st>>>(X, ld, P, Kd, Ppad, Kp, e.p, D.p);
+	const dim3 grid((unsigned)(Ppad / 64), (unsigned)(Kp / SBK));
+	if (ns == 2) { k_split_tile<2><<<grid, 256, 0, st>>>(X, ld, P, Kd, Ppad, Kp, e.p, D.p); }
+	else { k_split_tile<5><<<grid, 256, 0, st>>>(X, ld, P, Kd, Ppad, Kp, e.p, D.p); }
 	GLI_HIP(hipGetLastError());
 	return Ppad;
 }
 
 // the B operand (factor matrix) is split once per evaluation and reused by all modes
-void split_factor(const double* B, int64_t ldb, int C, int Kd, SplitWorkspace* ws)
+void split_factor(const double* B, int64_t ldb, int C, int Kd, SplitWorkspace* ws, int ns)
 {
 	SplitWorkspace& w = ws ? *ws : split_ws();
 	const int Kp = (int)round_up(Kd, SBK);
-	split_operand(B, ldb, C, Kd, Kp, w.bd, w.be);
+	split_operand(B, ldb, C, Kd, Kp, w.bd, w.be, ns);
+	w.b_ns = ns;
 }
 
 void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, double* Y, int64_t ldy,
@@ -238,6 +247,8 @@ void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, 
 {
 	if (P <= 0 || C <= 0) { return; }
 	SplitWorkspace& w = ws ? *ws : split_ws();
+	const int ns = w.b_ns;
+	GLI_REQUIRE(ns == 2 || ns == 5, "split_factor must run before mode_product_split");
 	const int Kp = (int)round_up(Kd, SBK);
 	const int64_t Cpad = round_up(C, SBN);
 	const uint16_t* xd;
@@ -245,30 +256,36 @@ void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, 
 	int64_t Ppad;
 	if (cache_x) {
 		// the kernel tensor operand is the same for every coefficient set of a grid
-		if (w.t3_src != X || w.t3_P != P) {
-			split_operand(X, ldx, P, Kd, Kp, w.t3d, w.t3e);
+		if (w.t3_src != X || w.t3_P != P || w.t3_ns != ns) {
+			split_operand(X, ldx, P, Kd, Kp, w.t3d, w.t3e, ns);
 			w.t3_src = X;
 			w.t3_P = P;
+			w.t3_ns = ns;
 		}
 		xd = w.t3d.p;
 		xe = w.t3e.p;
 		Ppad = round_up(P, SBM);
 	}
 	else {
-		Ppad = split_operand(X, ldx, P, Kd, Kp, w.xd, w.xe);
+		Ppad = split_operand(X, ldx, P, Kd, Kp, w.xd, w.xe, ns);
 		xd = w.xd.p;
 		xe = w.xe.p;
 	}
 	const TileOrder order{(int)(Ppad / SBM), (int)(Cpad / SBN), tile_order >= 0 ? tile_order : tile_order_grouped()};
 	const unsigned grid = (unsigned)order.blocks();
-	if (Kp <= KC) {
-		k_mode_product_split<true><<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,
-		                                                           tri_cols, tri_rows, p_base, tet_C1, order);
+	hipStream_t st = ctx().stream;
+#define GLI_SPLIT_LAUNCH(NSV, ONE) \
+	k_mode_product_split<NSV, ONE><<<grid, 256, 0, st>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp, tri_cols, \
+	                                                     tri_rows, p_base, tet_C1, order)
+	if (ns == 2) {
+		if (Kp <= KC) { GLI_SPLIT_LAUNCH(2, true); }
+		else { GLI_SPLIT_LAUNCH(2, false); }
 	}
 	else {
-		k_mode_product_split<false><<<grid, 256, 0, ctx().stream>>>(xd, Ppad, xe, w.bd.p, Cpad, w.be.p, P, C, Kp, Y, ldy, Cp,
-		                                                            tri_cols, tri_rows, p_base, tet_C1, order);
+		if (Kp <= KC) { GLI_SPLIT_LAUNCH(5, true); }
+		else { GLI_SPLIT_LAUNCH(5, false); }
 	}
+#undef GLI_SPLIT_LAUNCH
 	GLI_HIP(hipGetLastError());
 }
 

@@ -21,9 +21,11 @@ extern "C" {
 #define GLI_ERI_TENSOR_FP64  0  /* key-tensor contraction in FP64 (default, exact re-association) */
 #define GLI_ERI_TENSOR_MIXED 1  /* key-tensor contraction, split-precision MFMA bulk, FP64 final sums */
 #define GLI_ERI_DIRECT_FP64  2  /* per-element FP64 evaluation, no reuse across elements */
+#define GLI_ERI_TENSOR_BF16X2 3 /* comparison only: BF16 hi/lo two-term split, FP32 accumulation (~1e-6) */
+#define GLI_ERI_TENSOR_FP32  4  /* comparison only: FP32 MFMA products and accumulation (~1e-7) */
 
 /* method used by compute_sparse_eri_gausslet_integrals / compute_eri_gausslet_integral;
- * the environment variable GLI_ERI_METHOD (tensor | mixed | direct) sets the initial value */
+ * the environment variable GLI_ERI_METHOD (tensor | mixed | direct | bf16x2 | fp32) sets the initial value */
 int gli_set_eri_method(int method);
 int gli_get_eri_method(void);
 

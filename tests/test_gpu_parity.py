@@ -496,3 +496,22 @@ def test_batch_one_shot_entry(L):
     for e in entries:
         ref = eri_values_device(e[1], e[0], e[2], 1e-16, 0)
         assert np.array_equal(e[4].to_numpy(np.float64, len(e[2])), ref)
+
+
+def test_precision_ladder_7(L):
+    """the comparison methods on every unique 7^3 element against FP64: the 5-digit split stays within
+    1e-10, the BF16 hi/lo split and plain FP32 land near their unit roundoffs (bounds 1e-3 and 1e-4
+    catch a wrong accumulator layout, which gives O(1) errors), and the ladder is ordered"""
+    say("precision ladder 7")
+    c = golden("coeffs_K57_s1.npy")
+    d, n = eri_enumerate_device(7)
+    idx = d.to_numpy(np.int64, n)
+    v64 = eri_values_device(c, sym(7), idx, 1e-16, 0)
+    err = {}
+    for method, bound in ((1, 1e-10), (3, 1e-3), (4, 1e-4)):
+        got = eri_values_device(c, sym(7), idx, 1e-16, method)
+        err[method] = float(np.max(np.abs(got - v64)))
+        say("method", method, "max abs err", err[method])
+        assert err[method] <= bound, (method, err[method])
+    assert err[1] < err[4] and err[1] < err[3]
+    assert err[3] > 0 and err[4] > 0
