@@ -36,8 +36,9 @@ int gli_hip_kinetic_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 /* dense reconstruction and contractions of a sparse ERI / ERIDA set */
 int gli_hip_eri_fill_dense(const int64_t gs[6], const int64_t* keys, const double* vals, int64_t num,
                            const int64_t gd[6], double* out);
-int gli_hip_erida_fill_dense(const int64_t gs[6], const int64_t* keys, const double* vals, int64_t num,
-                             const int64_t gd[6], double* out);
+/* ret[(i N + k) S + s] = sum_{j,l} (ij|kl) states[(j N + l) S + s] over the sparse grid */
+int gli_hip_eri_apply(const int64_t gs[6], const int64_t* keys, const double* vals, int64_t num, const double* states,
+                      int64_t S, double* ret);
 
 #ifdef __cplusplus
 }

@@ -418,11 +418,293 @@ static int to_host(void (*dev)(int64_t, int64_t**, int64_t*), int64_t n, int64_t
 	}
 }
 
+// ---------------------------------------------------------------------------
+// dense views of a sparse ERI set (fill_dense_eri_tensor, apply_sparse_eri_tensor and
+// project_sparse_eri_gausslet_integrals, eri_integrals.c:714-1100): every tuple of a grid is mapped
+// to the canonical four-index of its orbit, whose value is found by binary search in the sorted
+// sparse list (0 when absent, as sparse_eri_gausslet_integrals_get_value).
+
+// smallest image and the elements reaching it, per point of a symmetric n^3 grid
+static void point_tables(int n, std::vector<int>& minimg, std::vector<unsigned long long>& canon)
+{
+	static const signed char col[48][3] = GLI_OH_COL;
+	static const signed char sgn[48][3] = GLI_OH_SGN;
+	const int h = (n - 1) / 2, N = n * n * n;
+	minimg.assign(N, 0);
+	canon.assign(N, 0);
+	for (int idx = 0; idx < N; ++idx) {
+		const int q[3] = { idx / (n * n) - h, (idx / n) % n - h, idx % n - h };
+		int img[48];
+		int best = N;
+		for (int e = 0; e < 48; ++e) {
+			const int x = sgn[e][0] * q[col[e][0]], y = sgn[e][1] * q[col[e][1]], z = sgn[e][2] * q[col[e][2]];
+			img[e] = ((x + h) * n + (y + h)) * n + (z + h);
+			best = std::min(best, img[e]);
+		}
+		minimg[idx] = best;
+		for (int e = 0; e < 48; ++e) {
+			if (img[e] == best) { canon[idx] |= 1ull << e; }
+		}
+	}
+}
+
+// The canonical four-index of a tuple with absolute coordinates A (relative to the centre of the
+// symmetric sparse grid): the box is translated so that twice its centre lies in {-1, 0, 1} per
+// axis (trans = centre / 2 truncated, bias = centre - 2 trans, eri_integrals.c:727-733), every bias
+// placement is tried, and the minimum over O_h and the 8 label swaps is taken.  Only the images
+// whose smallest point is the overall smallest orbit minimum m1 can reach the minimum; those are
+// the elements in canon[q] of the points q with minimg[q] == m1.  -1 when a placed point leaves
+// the sparse grid.
+__device__ int64_t eri_canonical_index(const EnumGrid& g, const int A[4][3], const int* __restrict__ minimg,
+                                       const unsigned long long* __restrict__ canon)
+{
+	int trans[3], bias[3];
+	for (int a = 0; a < 3; ++a) {
+		const int mn = min(min(A[0][a], A[1][a]), min(A[2][a], A[3][a]));
+		const int mx = max(max(A[0][a], A[1][a]), max(A[2][a], A[3][a]));
+		const int centre = mn + mx;
+		trans[a] = centre / 2;
+		bias[a] = centre - 2 * trans[a];
+	}
+	int m1 = 0x7fffffff;
+	for (int bx = 0; bx <= abs(bias[0]); ++bx) {
+		for (int by = 0; by <= abs(bias[1]); ++by) {
+			for (int bz = 0; bz <= abs(bias[2]); ++bz) {
+				const int b[3] = { bx, by, bz };
+				for (int p = 0; p < 4; ++p) {
+					int Q[3];
+					for (int a = 0; a < 3; ++a) {
+						Q[a] = A[p][a] - trans[a] - b[a] * bias[a];
+						if (Q[a] < -g.h || Q[a] > g.h) { return -1; }
+					}
+					m1 = min(m1, minimg[lin_c(g, Q[0], Q[1], Q[2])]);
+				}
+			}
+		}
+	}
+	const int64_t N = g.N, N2 = (int64_t)g.N * g.N;
+	int64_t best = INT64_MAX;
+	for (int bx = 0; bx <= abs(bias[0]); ++bx) {
+		for (int by = 0; by <= abs(bias[1]); ++by) {
+			for (int bz = 0; bz <= abs(bias[2]); ++bz) {
+				const int b[3] = { bx, by, bz };
+				int Q[4][3];
+				for (int p = 0; p < 4; ++p) {
+					for (int a = 0; a < 3; ++a) { Q[p][a] = A[p][a] - trans[a] - b[a] * bias[a]; }
+				}
+				for (int p = 0; p < 4; ++p) {
+					const int lq = lin_c(g, Q[p][0], Q[p][1], Q[p][2]);
+					if (minimg[lq] != m1) { continue; }
+					for (unsigned long long m = canon[lq]; m; m &= m - 1) {
+						const int e = __builtin_ctzll(m);
+						const int c0 = c_oh_col[e][0], c1 = c_oh_col[e][1], c2 = c_oh_col[e][2];
+						const int s0 = c_oh_sgn[e][0], s1 = c_oh_sgn[e][1], s2 = c_oh_sgn[e][2];
+						int L[4];
+						for (int r = 0; r < 4; ++r) { L[r] = lin_c(g, s0 * Q[r][c0], s1 * Q[r][c1], s2 * Q[r][c2]); }
+						const int64_t x0 = min(L[0], L[1]), x1 = max(L[0], L[1]);
+						const int64_t y0 = min(L[2], L[3]), y1 = max(L[2], L[3]);
+						int64_t ab = x0 * N + x1, cd = y0 * N + y1;
+						if (ab > cd) { const int64_t t = ab; ab = cd; cd = t; }
+						best = min(best, ab * N2 + cd);
+					}
+				}
+			}
+		}
+	}
+	return best;
+}
+
+__device__ inline double sparse_lookup(const int64_t* __restrict__ keys, const double* __restrict__ vals, int64_t num, int64_t key)
+{
+	if (key < 0) { return 0.0; }
+	int64_t lo = 0, hi = num;
+	while (lo < hi) {
+		const int64_t m = (lo + hi) >> 1;
+		const int64_t v = keys[m];
+		if (key < v) { hi = m; }
+		else if (key > v) { lo = m + 1; }
+		else { return vals[m]; }
+	}
+	return 0.0;
+}
+
+struct DenseGrid {
+	int istart[3];
+	int n[3];
+	int64_t N;
+};
+
+__device__ inline void dense_point(const DenseGrid& d, int64_t q, int A[3])
+{
+	const int z = (int)(q % d.n[2]);
+	const int64_t r = q / d.n[2];
+	const int y = (int)(r % d.n[1]);
+	const int x = (int)(r / d.n[1]);
+	A[0] = d.istart[0] + x;
+	A[1] = d.istart[1] + y;
+	A[2] = d.istart[2] + z;
+}
+
+// out[q] = value of dense tuple t0 + q, for q < count (grid-stride loop)
+__global__ __launch_bounds__(256) void k_eri_fill_dense(EnumGrid gs, DenseGrid gd, const int* __restrict__ minimg,
+	const unsigned long long* __restrict__ canon, const int64_t* __restrict__ keys, const double* __restrict__ vals, int64_t num,
+	int64_t t0, int64_t count, double* __restrict__ out)
+{
+	for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
+		int64_t t = t0 + q;
+		int A[4][3];
+		for (int p = 3; p >= 0; --p) {
+			dense_point(gd, t % gd.N, A[p]);
+			t /= gd.N;
+		}
+		out[q] = sparse_lookup(keys, vals, num, eri_canonical_index(gs, A, minimg, canon));
+	}
+}
+
+// ret[(i N + k) S + s0 + s] = sum_{j,l} (ij|kl) states[(j N + l) S + s0 + s] for s < sn <= APPLY_S:
+// one workgroup per (i, k), threads over (j, l), tree reduction in LDS
+constexpr int APPLY_S = 8;
+
+__global__ __launch_bounds__(256) void k_eri_apply(EnumGrid g, const int* __restrict__ minimg, const unsigned long long* __restrict__ canon,
+	const int64_t* __restrict__ keys, const double* __restrict__ vals, int64_t num, const double* __restrict__ states,
+	int64_t S, int s0, int sn, int64_t ik0, double* __restrict__ ret)
+{
+	__shared__ double red[APPLY_S][256];
+	const int64_t N = g.N, N2 = (int64_t)g.N * g.N;
+	const int64_t ik = ik0 + blockIdx.x;
+	const int i = (int)(ik / N), k = (int)(ik % N);
+	int A[4][3];
+	A[0][0] = i / (g.n * g.n) - g.h; A[0][1] = (i / g.n) % g.n - g.h; A[0][2] = i % g.n - g.h;
+	A[2][0] = k / (g.n * g.n) - g.h; A[2][1] = (k / g.n) % g.n - g.h; A[2][2] = k % g.n - g.h;
+	double acc[APPLY_S];
+	for (int s = 0; s < APPLY_S; ++s) { acc[s] = 0; }
+	for (int64_t jl = threadIdx.x; jl < N2; jl += blockDim.x) {
+		const int j = (int)(jl / N), l = (int)(jl % N);
+		A[1][0] = j / (g.n * g.n) - g.h; A[1][1] = (j / g.n) % g.n - g.h; A[1][2] = j % g.n - g.h;
+		A[3][0] = l / (g.n * g.n) - g.h; A[3][1] = (l / g.n) % g.n - g.h; A[3][2] = l % g.n - g.h;
+		const double v = sparse_lookup(keys, vals, num, eri_canonical_index(g, A, minimg, canon));
+		const double* x = states + jl * S + s0;
+		for (int s = 0; s < APPLY_S; ++s) {
+			if (s < sn) { acc[s] += v * x[s]; }
+		}
+	}
+	for (int s = 0; s < APPLY_S; ++s) { red[s][threadIdx.x] = acc[s]; }
+	__syncthreads();
+	for (int w = 128; w > 0; w >>= 1) {
+		if ((int)threadIdx.x < w) {
+			for (int s = 0; s < APPLY_S; ++s) { red[s][threadIdx.x] += red[s][threadIdx.x + w]; }
+		}
+		__syncthreads();
+	}
+	if ((int)threadIdx.x < sn) { ret[ik * S + s0 + threadIdx.x] = red[threadIdx.x][0]; }
+}
+
+struct SparseDev {
+	EnumGrid g;
+	DevBuf<int> minimg;
+	DevBuf<unsigned long long> canon;
+	DevBuf<int64_t> keys;
+	DevBuf<double> vals;
+};
+
+static void sparse_to_device(const int64_t gs[6], const int64_t* keys, const double* vals, int64_t num, SparseDev& d)
+{
+	GLI_REQUIRE(gs[1] == gs[3] && gs[1] == gs[5] && (gs[1] % 2) == 1 && gs[0] == -(gs[1] - 1) / 2 && gs[2] == gs[0] && gs[4] == gs[0],
+	            "dense views need the sparse set of a symmetric grid with an odd number of points per axis");
+	GLI_REQUIRE(gs[1] <= 31, "grid too large for the dense views (at most 31 points per axis)");
+	GLI_REQUIRE(num >= 0, "negative number of entries");
+	const int n = (int)gs[1];
+	d.g.n = n;
+	d.g.h = (n - 1) / 2;
+	d.g.N = n * n * n;
+	std::vector<int> minimg;
+	std::vector<unsigned long long> canon;
+	point_tables(n, minimg, canon);
+	hipStream_t st = ctx().stream;
+	d.minimg.alloc(minimg.size());
+	d.canon.alloc(canon.size());
+	d.keys.alloc(std::max<int64_t>(num, 1));
+	d.vals.alloc(std::max<int64_t>(num, 1));
+	GLI_HIP(hipMemcpyAsync(d.minimg.p, minimg.data(), minimg.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(d.canon.p, canon.data(), canon.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
+	if (num > 0) {
+		GLI_HIP(hipMemcpyAsync(d.keys.p, keys, num * sizeof(int64_t), hipMemcpyHostToDevice, st));
+		GLI_HIP(hipMemcpyAsync(d.vals.p, vals, num * sizeof(double), hipMemcpyHostToDevice, st));
+	}
+	GLI_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace gli
 
 using namespace gli;
 
 extern "C" {
+
+int gli_hip_eri_fill_dense(const int64_t gs[6], const int64_t* keys, const double* vals, int64_t num, const int64_t gd[6], double* out)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		SparseDev sd;
+		sparse_to_device(gs, keys, vals, num, sd);
+		DenseGrid dg;
+		for (int a = 0; a < 3; ++a) {
+			GLI_REQUIRE(gd[2 * a + 1] > 0 && gd[2 * a + 1] < 1024, "dense grid axes need 1 to 1023 points");
+			dg.istart[a] = (int)gd[2 * a];
+			dg.n[a] = (int)gd[2 * a + 1];
+		}
+		dg.N = (int64_t)dg.n[0] * dg.n[1] * dg.n[2];
+		GLI_REQUIRE((double)dg.N * dg.N * dg.N * dg.N < 9.0e15, "dense tensor too large");
+		const int64_t total = dg.N * dg.N * dg.N * dg.N;
+		const int64_t chunk = std::min<int64_t>(total, (int64_t)1 << 26);
+		DevBuf<double> buf(std::max<int64_t>(chunk, 1));
+		for (int64_t t0 = 0; t0 < total; t0 += chunk) {
+			const int64_t cnt = std::min(chunk, total - t0);
+			const unsigned blocks = (unsigned)std::min<int64_t>(ceil_div(cnt, 256), 65536);
+			k_eri_fill_dense<<<blocks, 256, 0, c.stream>>>(sd.g, dg, sd.minimg.p, sd.canon.p, sd.keys.p, sd.vals.p, num, t0, cnt, buf.p);
+			GLI_HIP(hipGetLastError());
+			GLI_HIP(hipMemcpyAsync(out + t0, buf.p, cnt * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+			GLI_HIP(hipStreamSynchronize(c.stream));
+		}
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_hip_eri_apply(const int64_t gs[6], const int64_t* keys, const double* vals, int64_t num, const double* states,
+                      int64_t S, double* ret)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		GLI_REQUIRE(S >= 0, "negative number of states");
+		SparseDev sd;
+		sparse_to_device(gs, keys, vals, num, sd);
+		const int64_t N2 = (int64_t)sd.g.N * sd.g.N;
+		if (S == 0) { return 0; }
+		DevBuf<double> dst(N2 * S), dret(N2 * S);
+		GLI_HIP(hipMemcpyAsync(dst.p, states, N2 * S * sizeof(double), hipMemcpyHostToDevice, c.stream));
+		const int64_t rows_per_launch = 1 << 20;
+		for (int64_t s0 = 0; s0 < S; s0 += APPLY_S) {
+			const int sn = (int)std::min<int64_t>(APPLY_S, S - s0);
+			for (int64_t ik0 = 0; ik0 < N2; ik0 += rows_per_launch) {
+				const unsigned blocks = (unsigned)std::min(rows_per_launch, N2 - ik0);
+				k_eri_apply<<<blocks, 256, 0, c.stream>>>(sd.g, sd.minimg.p, sd.canon.p, sd.keys.p, sd.vals.p, num, dst.p, S, (int)s0, sn,
+				                                          ik0, dret.p);
+				GLI_HIP(hipGetLastError());
+			}
+		}
+		GLI_HIP(hipMemcpyAsync(ret, dret.p, N2 * S * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
 
 int gli_eri_enumerate_device(int64_t n, int64_t** d_out, int64_t* num)
 {

@@ -1,6 +1,7 @@
 /* Reference-compatible ERI entry points (eri_integrals.h) over the GPU path.
- * Enumeration and values run on the GPU; lookups and the dense/contracted
- * views of a sparse set are host restatements (eri_integrals.c:685-1100). */
+ * Enumeration, values and the dense / applied / projected views of a sparse
+ * set run on the GPU; single lookups are host binary searches
+ * (eri_integrals.c:685-707). */
 #include <stdlib.h>
 #include <string.h>
 #include "eri_integrals.h"
@@ -152,125 +153,74 @@ void delete_sparse_eri_gausslet_integrals(struct sparse_eri_gausslet_integrals* 
 }
 
 /* ------------------------------------------------------------------------- */
-/* dense views (eri_integrals.c:714-1100)                                     */
-
-static void perms_of(const struct cartesian_grid_3d* g, glong* perm[48])
-{
-	evaluate_octahedral_grid_permutations(g, perm);
-}
-
-static void free_perms(glong* perm[48])
-{
-	for (int e = 0; e < 48; ++e) { aligned_free(perm[e]); }
-}
-
-/* value of the dense tuple (i, j, k, l) of grid gd, looked up in the sparse set */
-static double dense_value(const struct sparse_eri_gausslet_integrals* eri, const glong* const perm[48],
-	const struct cartesian_grid_3d* gd, glong i, glong j, glong k, glong l)
-{
-	union cartesian_grid_point_3d pts[4];
-	linear_index_to_cartesian_grid_point_3d(gd, i, &pts[0]);
-	linear_index_to_cartesian_grid_point_3d(gd, j, &pts[1]);
-	linear_index_to_cartesian_grid_point_3d(gd, k, &pts[2]);
-	linear_index_to_cartesian_grid_point_3d(gd, l, &pts[3]);
-	const glong key = gli_eri_sparse_key(&eri->grid, perm, pts);
-	return key < 0 ? 0 : sparse_eri_gausslet_integrals_get_value(eri, key);
-}
+/* dense views (eri_integrals.c:714-1100): on the GPU, every tuple canonicalised there and looked  */
+/* up by binary search in the sorted sparse list                                                    */
 
 void fill_dense_eri_tensor(const struct sparse_eri_gausslet_integrals* eri_sparse, const struct cartesian_grid_3d* grid_dense, double* eri_tensor)
 {
-	const glong Nd = cartesian_grid_3d_num_points(grid_dense);
-	glong* perm[48];
-	perms_of(&eri_sparse->grid, perm);
-	#pragma omp parallel for schedule(dynamic) collapse(2)
-	for (glong i = 0; i < Nd; ++i) {
-		for (glong j = 0; j < Nd; ++j) {
-			for (glong k = 0; k < Nd; ++k) {
-				for (glong l = 0; l < Nd; ++l) {
-					eri_tensor[((i * Nd + j) * Nd + k) * Nd + l] = dense_value(eri_sparse, (const glong* const*)perm, grid_dense, i, j, k, l);
-				}
-			}
-		}
-	}
-	free_perms(perm);
+	int64_t gs[6], gd[6];
+	gli_require_symmetric_cubic(&eri_sparse->grid, "fill_dense_eri_tensor");
+	gli_grid_array(&eri_sparse->grid, gs);
+	gli_grid_array(grid_dense, gd);
+	gli_check(gli_hip_eri_fill_dense(gs, eri_sparse->four_indices, eri_sparse->integral_values, eri_sparse->num_entries, gd, eri_tensor),
+	          "fill_dense_eri_tensor");
 }
 
-/* eri_proj[p,q,r,s] = sum_{ijkl} b[i,p] b[j,q] b[k,r] b[l,s] (ij|kl) over the sparse grid,
- * contracted one (i, j) block at a time */
+/* ret[(i,k), s] = sum_{j,l} (ij|kl) states[(j,l), s]  (physicist's grouping, eri_integrals.c:974-1100) */
+void apply_sparse_eri_tensor(const struct sparse_eri_gausslet_integrals* eri, const double* states, const glong num_states, double* ret)
+{
+	int64_t gs[6];
+	gli_require_symmetric_cubic(&eri->grid, "apply_sparse_eri_tensor");
+	gli_grid_array(&eri->grid, gs);
+	gli_check(gli_hip_eri_apply(gs, eri->four_indices, eri->integral_values, eri->num_entries, states, num_states, ret),
+	          "apply_sparse_eri_tensor");
+}
+
+/* eri_proj[p,q,r,s] = sum_{ijkl} b[i,p] b[j,q] b[k,r] b[l,s] (ij|kl)  (eri_integrals.c:714-830):
+ * the GPU applies the tensor to the S^2 product states x[(j,l),(q,s)] = b[j,q] b[l,s], which gives
+ * R[(i,k),(q,s)]; the two remaining modes i, k are contracted here */
 void project_sparse_eri_gausslet_integrals(const struct sparse_eri_gausslet_integrals* eri, const double* basis, const glong num_states, double* eri_proj)
 {
 	const glong N = cartesian_grid_3d_num_points(&eri->grid);
-	const glong S = num_states;
-	const glong S4 = S * S * S * S;
-	glong* perm[48];
-	perms_of(&eri->grid, perm);
-	memset(eri_proj, 0, S4 * sizeof(double));
-	#pragma omp parallel
-	{
-		double* acc = aligned_calloc(S4 * sizeof(double));
-		double* V = aligned_malloc(N * N * sizeof(double));
-		double* VB = aligned_malloc(N * S * sizeof(double));
-		double* W = aligned_malloc(S * S * sizeof(double));
-		#pragma omp for schedule(dynamic) collapse(2)
-		for (glong i = 0; i < N; ++i) {
-			for (glong j = 0; j < N; ++j) {
-				for (glong k = 0; k < N; ++k) {
-					for (glong l = 0; l < N; ++l) {
-						V[k * N + l] = dense_value(eri, (const glong* const*)perm, &eri->grid, i, j, k, l);
-					}
-				}
-				/* VB[k,s] = sum_l V[k,l] b[l,s];  W[r,s] = sum_k b[k,r] VB[k,s] */
-				for (glong k = 0; k < N; ++k) {
-					for (glong s = 0; s < S; ++s) {
-						double t = 0;
-						for (glong l = 0; l < N; ++l) { t += V[k * N + l] * basis[l * S + s]; }
-						VB[k * S + s] = t;
-					}
-				}
-				for (glong r = 0; r < S; ++r) {
-					for (glong s = 0; s < S; ++s) {
-						double t = 0;
-						for (glong k = 0; k < N; ++k) { t += basis[k * S + r] * VB[k * S + s]; }
-						W[r * S + s] = t;
-					}
-				}
-				for (glong p = 0; p < S; ++p) {
-					for (glong q = 0; q < S; ++q) {
-						const double bpq = basis[i * S + p] * basis[j * S + q];
-						for (glong rs = 0; rs < S * S; ++rs) { acc[(p * S + q) * S * S + rs] += bpq * W[rs]; }
-					}
-				}
+	const glong S = num_states, S2 = S * S;
+	memset(eri_proj, 0, S2 * S2 * sizeof(double));
+	if (S == 0) { return; }
+	double* x = aligned_malloc(N * N * S2 * sizeof(double));
+	double* R = aligned_malloc(N * N * S2 * sizeof(double));
+	double* T = aligned_calloc(N * S * S2 * sizeof(double));
+	#pragma omp parallel for
+	for (glong j = 0; j < N; ++j) {
+		for (glong l = 0; l < N; ++l) {
+			for (glong q = 0; q < S; ++q) {
+				for (glong s = 0; s < S; ++s) { x[(j * N + l) * S2 + q * S + s] = basis[j * S + q] * basis[l * S + s]; }
 			}
 		}
-		#pragma omp critical
-		{
-			for (glong x = 0; x < S4; ++x) { eri_proj[x] += acc[x]; }
-		}
-		aligned_free(W); aligned_free(VB); aligned_free(V); aligned_free(acc);
 	}
-	free_perms(perm);
-}
-
-/* ret[(i,k), s] = sum_{j,l} (ij|kl) states[(j,l), s]  (physicist's grouping) */
-void apply_sparse_eri_tensor(const struct sparse_eri_gausslet_integrals* eri, const double* states, const glong num_states, double* ret)
-{
-	const glong N = cartesian_grid_3d_num_points(&eri->grid);
-	const glong S = num_states;
-	glong* perm[48];
-	perms_of(&eri->grid, perm);
-	#pragma omp parallel for schedule(dynamic) collapse(2)
+	apply_sparse_eri_tensor(eri, x, S2, R);
+	/* T[i, r, qs] = sum_k b[k, r] R[(i, k), qs] */
+	#pragma omp parallel for
 	for (glong i = 0; i < N; ++i) {
 		for (glong k = 0; k < N; ++k) {
-			double* r = &ret[(i * N + k) * S];
-			for (glong s = 0; s < S; ++s) { r[s] = 0; }
-			for (glong j = 0; j < N; ++j) {
-				for (glong l = 0; l < N; ++l) {
-					const double v = dense_value(eri, (const glong* const*)perm, &eri->grid, i, j, k, l);
-					const double* x = &states[(j * N + l) * S];
-					for (glong s = 0; s < S; ++s) { r[s] += v * x[s]; }
+			const double* Rik = &R[(i * N + k) * S2];
+			for (glong r = 0; r < S; ++r) {
+				const double b = basis[k * S + r];
+				double* Tir = &T[(i * S + r) * S2];
+				for (glong qs = 0; qs < S2; ++qs) { Tir[qs] += b * Rik[qs]; }
+			}
+		}
+	}
+	/* proj[p, q, r, s] = sum_i b[i, p] T[i, r, qs] */
+	for (glong i = 0; i < N; ++i) {
+		for (glong p = 0; p < S; ++p) {
+			const double b = basis[i * S + p];
+			for (glong r = 0; r < S; ++r) {
+				for (glong q = 0; q < S; ++q) {
+					for (glong s = 0; s < S; ++s) { eri_proj[((p * S + q) * S + r) * S + s] += b * T[(i * S + r) * S2 + q * S + s]; }
 				}
 			}
 		}
 	}
-	free_perms(perm);
+	aligned_free(T);
+	aligned_free(R);
+	aligned_free(x);
 }

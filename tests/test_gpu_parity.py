@@ -515,3 +515,48 @@ def test_precision_ladder_7(L):
         assert err[method] <= bound, (method, err[method])
     assert err[1] < err[4] and err[1] < err[3]
     assert err[3] > 0 and err[4] > 0
+
+
+# ---------------------------------------------------------------------------
+# dense views on the GPU: canonicalisation of arbitrary tuples + lookup
+
+def test_dense_fill_7_sub_box_bit_exact(L, orc):
+    """fill_tensor of the full 7^3 sparse set on an off-centre 3x2x4 dense box equals the oracle's
+    fill (reference canonicalisation with bias placements, eri_integrals.c:837-967) bit for bit"""
+    say("dense fill 7 sub box")
+    c = golden("coeffs_K57_s0.npy")
+    eri = gli.compute_sparse_eri_gausslet_integrals(c, sym(7), 1e-16)
+    box = ((-3, 3), (-1, 2), (0, 4))
+    got = eri.fill_tensor(box)
+    ref = orc.fill_dense_eri(sym(7), eri.four_indices, eri.integral_values, box)
+    assert got.shape == ref.shape
+    assert np.array_equal(got.ravel(), np.asarray(ref).ravel())
+
+
+def test_apply_5_columns_and_linearity(L, orc):
+    """apply_sparse_eri_tensor on 5^3 (2.4e8 tuples): a unit state at (j0, l0) returns the column of
+    values (i j0 | k l0), checked through the oracle's canonical keys, and apply is linear"""
+    say("apply 5")
+    c = golden("coeffs_K43_s0.npy")
+    eri = gli.compute_sparse_eri_gausslet_integrals(c, sym(5), 1e-16)
+    keys, vals = eri.four_indices, eri.integral_values
+    N = 125
+    rng = np.random.default_rng(41)
+    jl = rng.integers(0, N * N, size=2)
+    X = np.zeros((N * N, 4))
+    X[jl[0], 0] = 1.0
+    X[jl[1], 1] = 1.0
+    X[:, 2] = rng.standard_normal(N * N)
+    X[:, 3] = 2.0 * X[:, 0] - 3.0 * X[:, 2]
+    out = eri.apply(X)
+    for u in range(2):
+        j0, l0 = divmod(int(jl[u]), N)
+        ik = np.arange(N * N)
+        i, k = ik // N, ik % N
+        t = ((i * N + j0) * N + k) * N + l0
+        ck = orc.eri_canonical_keys(sym(5), t)
+        pos = np.clip(np.searchsorted(keys, ck), 0, len(keys) - 1)
+        col = np.where(keys[pos] == ck, vals[pos], 0.0)
+        assert np.array_equal(out[:, u], col), u
+    scale = np.max(np.abs(out[:, 2])) + 1.0
+    assert np.max(np.abs(out[:, 3] - (2.0 * out[:, 0] - 3.0 * out[:, 2]))) <= 1e-13 * scale
