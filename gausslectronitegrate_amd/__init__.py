@@ -10,5 +10,6 @@ from .pymodule import (  # noqa: F401
     compute_eri_gausslet_integral, compute_sparse_eri_gausslet_integrals,
     compute_erida_gausslet_integral, compute_sparse_erida_gausslet_integrals,
     get_max_openmp_threads, SparseERI, SparseERIDA,
+    load_sparse_eri_gausslet_integrals, save_sparse_eri_gausslet_integrals,
 )
 from ._lib import ERI_TENSOR_FP64, ERI_TENSOR_MIXED, ERI_DIRECT_FP64  # noqa: F401

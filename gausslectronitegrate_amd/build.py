@@ -18,6 +18,8 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB_DIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIB_DIR, "libgli_amd.so")
+LIB_HDF5 = os.path.join(LIB_DIR, "libgli_amd_hdf5.so")
+HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
 INCLUDE = os.path.join(REPO, "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -67,7 +69,26 @@ def build(verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-Wl,-Bsymbolic", "-Wl,--no-undefined", "-lgomp", "-lm"])
     os.replace(tmp, LIB)
+    build_hdf5(incs)
     return LIB
+
+
+def build_hdf5(incs):
+    """libgli_amd_hdf5.so: the reference's HDF5 helpers and sparse ERI file layout, kept out of the
+    core library so that libgli_amd.so needs no HDF5 installation; skipped when HDF5 is absent."""
+    h5inc = os.path.join(HDF5_ROOT, "include")
+    h5lib = os.path.join(HDF5_ROOT, "lib")
+    if not os.path.exists(os.path.join(h5inc, "hdf5.h")):
+        print("HDF5 not found under %s: libgli_amd_hdf5.so not built" % HDF5_ROOT, file=sys.stderr)
+        return None
+    src = os.path.join(CSRC, "hdf5", "hdf5_io.c")
+    obj = os.path.join(BUILD, "hdf5_io.c.o")
+    _run([CC] + C_FLAGS + incs + ["-I" + h5inc, "-c", src, "-o", obj])
+    tmp = LIB_HDF5 + ".tmp"
+    _run([CC, "-shared", "-fPIC", "-o", tmp, obj, "-L" + LIB_DIR, "-lgli_amd", "-Wl,-rpath,$ORIGIN",
+          "-L" + h5lib, "-lhdf5", "-Wl,-rpath," + h5lib, "-Wl,--no-undefined"])
+    os.replace(tmp, LIB_HDF5)
+    return LIB_HDF5
 
 
 if __name__ == "__main__":

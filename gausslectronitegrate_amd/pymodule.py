@@ -12,6 +12,7 @@ import os
 import numpy as np
 
 from . import _abi
+from . import _lib
 from ._lib import lib, require_gpu
 
 
@@ -211,6 +212,16 @@ class SparseERI:
                                                 self._tol, C.byref(self._eri))
         L.delete_sparse_eri_indices(C.byref(idx))
 
+    @classmethod
+    def _from_arrays(cls, eri_struct, coeffs, tol):
+        """a set read from a file: the struct owns its arrays (released by delete_*)"""
+        self = cls.__new__(cls)
+        self._tol = float(tol)
+        self._gd, self._coeffs = _coeffs(coeffs, "load_sparse_eri_gausslet_integrals(filename)")
+        self._perm = _Perms(eri_struct.grid)
+        self._eri = eri_struct
+        return self
+
     def __del__(self):
         if getattr(self, "_eri", None) is not None and self._eri.integral_values:
             lib().delete_sparse_eri_gausslet_integrals(C.byref(self._eri))
@@ -278,6 +289,58 @@ class SparseERI:
 
 def compute_sparse_eri_gausslet_integrals(gausslet_coeffs, grid, tol):
     return SparseERI(gausslet_coeffs, grid, tol)
+
+
+_HDF5 = None
+
+
+def _hdf5_lib():
+    """lib/libgli_amd_hdf5.so: the reference's sparse ERI file layout (built when HDF5 is present)."""
+    global _HDF5
+    if _HDF5 is None:
+        path = os.path.join(os.path.dirname(_lib.LIB_PATH), "libgli_amd_hdf5.so")
+        if not os.path.exists(path):
+            raise RuntimeError("libgli_amd_hdf5.so is not built (no HDF5 installation was found at build time)")
+        L = C.CDLL(path)
+        L.gli_eri_load_hdf5.restype = C.c_int
+        L.gli_eri_load_hdf5.argtypes = [C.c_char_p, C.POINTER(_abi.SparseEri), C.POINTER(_abi.P_double),
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.gli_eri_save_hdf5.restype = C.c_int
+        L.gli_eri_save_hdf5.argtypes = [C.c_char_p, C.c_int64, C.c_double, _abi.P_double, C.c_int64,
+                                        C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+        _HDF5 = L
+    return _HDF5
+
+
+def load_sparse_eri_gausslet_integrals(filename):
+    """Load a sparse ERI set from an HDF5 file in the reference's layout (pymodule.c:1181-1326)."""
+    syntax = "load_sparse_eri_gausslet_integrals(filename)"
+    if not isinstance(filename, str):
+        raise SyntaxError("error parsing input; syntax: %s" % syntax)
+    L = _hdf5_lib()
+    eri = _abi.SparseEri()
+    pc = _abi.P_double()
+    K = C.c_int64()
+    tol = C.c_double()
+    if L.gli_eri_load_hdf5(filename.encode(), C.byref(eri), C.byref(pc), C.byref(K), C.byref(tol)) != 0:
+        raise RuntimeError("reading the sparse ERI set from HDF5 file '%s' failed; syntax: %s" % (filename, syntax))
+    coeffs = np.ctypeslib.as_array(pc, shape=(K.value,)).copy()
+    C.CDLL(None).free(C.cast(pc, C.c_void_p))
+    return SparseERI._from_arrays(eri, coeffs, tol.value)
+
+
+def save_sparse_eri_gausslet_integrals(eri, filename):
+    """Write a sparse ERI set in the reference's HDF5 layout (the layout cli/gli_eri_integrals.c writes)."""
+    if not isinstance(eri, SparseERI) or not isinstance(filename, str):
+        raise SyntaxError("error parsing input; syntax: save_sparse_eri_gausslet_integrals(eri, filename)")
+    g = eri._eri.grid.spec()
+    if not (g[0] == g[1] == g[2] and g[0][0] == -(g[0][1] - 1) // 2):
+        raise ValueError("the HDF5 layout stores a centred cubic grid")
+    c = np.ascontiguousarray(eri._coeffs, dtype=np.float64)
+    if _hdf5_lib().gli_eri_save_hdf5(filename.encode(), g[0][1], eri.tol, c.ctypes.data_as(_abi.P_double), len(c),
+                                     C.cast(eri._eri.four_indices, C.c_void_p), C.cast(eri._eri.integral_values, C.c_void_p),
+                                     eri.num_entries, 0) != 0:
+        raise RuntimeError("writing HDF5 file '%s' failed" % filename)
 
 
 # ---------------------------------------------------------------------------

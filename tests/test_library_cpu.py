@@ -13,9 +13,14 @@ from gausslectronitegrate_amd import _abi, _lib
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+HDF5_HEADERS = ("hdf5_util.h", "gli_hdf5.h")  # exported by lib/libgli_amd_hdf5.so
+
+
+def declared_functions(headers=None):
     names = set()
     for h in os.listdir(os.path.join(REPO, "include")):
+        if (h in HDF5_HEADERS) != (headers == "hdf5"):
+            continue
         text = open(os.path.join(REPO, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)
@@ -32,6 +37,15 @@ def test_library_exports_every_declared_symbol():
     L = _lib.lib()
     names = declared_functions()
     assert "compute_sparse_eri_gausslet_integrals" in names and "gli_eri_values_device" in names
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_hdf5_library_exports_every_declared_symbol():
+    path = os.path.join(os.path.dirname(_lib.LIB_PATH), "libgli_amd_hdf5.so")
+    L = ctypes.CDLL(path)
+    names = declared_functions("hdf5")
+    assert {"read_hdf5_dataset", "write_hdf5_vector_attribute", "gli_eri_load_hdf5", "gli_eri_save_hdf5"} <= names
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert missing == []
 
