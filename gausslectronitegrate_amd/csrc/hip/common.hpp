@@ -118,7 +118,8 @@ struct Context {
 	int stage_next = 0;
 	// kernel tuning knobs (gli_set_tuning)
 	int tile_grouped = 0;   // mode-product tile order (the XCD-grouped order measured slower on the ERI shapes)
-	int gather_ept = 4;     // gather elements per lane
+	int gather_ept = 8;     // gather elements per lane
+	int gather_lds = 1;     // gather keeps its key tables in LDS (when they fit)
 	int gather_xcd = 1;     // gather walks one contiguous chunk of the list per XCD
 	int gemm_shape = 0;     // FP64 mode-product tile shape (0: 64x64, 1: 128x64, 2: 64x128, 3: 128x128)
 };

@@ -219,8 +219,12 @@ int gli_set_tuning(const char* knob, int value)
 			c.tile_grouped = value;
 		}
 		else if (k == "gather_ept") {
-			GLI_REQUIRE(value == 2 || value == 4 || value == 8, "gather_ept must be 2, 4 or 8");
+			GLI_REQUIRE(value == 2 || value == 4 || value == 8 || value == 16, "gather_ept must be 2, 4, 8 or 16");
 			c.gather_ept = value;
+		}
+		else if (k == "gather_lds") {
+			GLI_REQUIRE(value == 0 || value == 1, "gather_lds must be 0 or 1");
+			c.gather_lds = value;
 		}
 		else if (k == "gemm_shape") {
 			GLI_REQUIRE(value >= 0 && value < kGemmShapes, "gemm_shape must be 0..3");
@@ -248,6 +252,7 @@ int gli_get_tuning(const char* knob)
 		if (k == "tile_order") { return c.tile_grouped; }
 		if (k == "gather_ept") { return c.gather_ept; }
 		if (k == "gather_order") { return c.gather_xcd; }
+		if (k == "gather_lds") { return c.gather_lds; }
 		if (k == "gemm_shape") { return c.gemm_shape; }
 		GLI_REQUIRE(false, "unknown tuning knob");
 	}

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <map>
 #include <tuple>
+#include <type_traits>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -504,15 +505,24 @@ struct GatherDesc {
 	double* out;
 	const double* scale;
 	int64_t piece_begin;   // first workgroup-sized piece of this problem in a grouped launch
+	int nlut, C1;          // entries of the key table and of the tetrahedral offsets
 };
 
+// LDS bytes a workgroup needs to hold the key table and the tetrahedral offsets
+__host__ __device__ inline int gather_lds_bytes(int nlut, int C1)
+{
+	return 4 * ((C1 + 3) & ~3) + 2 * nlut;
+}
+constexpr int kGatherLdsMax = 48 * 1024;
+
+// lut and tet3 point to the descriptor's tables in global memory, or to their copies in LDS
 template <int GATHER_EPT>
-__device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base)
+__device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const short* __restrict__ lut,
+                                   const uint32_t* __restrict__ tet3)
 {
 	const int lane = threadIdx.x & 63;
 	const GridDev& g = d.g;
 	const DecodeConsts& dc = d.dc;
-	const short* __restrict__ lut = d.lut;
 	const double* __restrict__ V = d.V;
 	int64_t t[GATHER_EPT];
 #pragma unroll
@@ -574,7 +584,7 @@ __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base)
 			vi[u] = (__umul24(k2, k2 + 1u) >> 1) + k1;
 		}
 #pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] += d.tet3[k3[u]]; }
+		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] += tet3[k3[u]]; }
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = V[vi[u]]; }
 	}
@@ -606,26 +616,49 @@ __device__ inline int64_t gather_piece(int64_t xcd_chunks)
 	return (xcd_chunks > 0) ? (int64_t)(blockIdx.x & 7) * xcd_chunks + (blockIdx.x >> 3) : (int64_t)blockIdx.x;
 }
 
-template <int GATHER_EPT>
+// With kLds the workgroup first copies the key table and the tetrahedral offsets into LDS: their
+// reads then no longer compete with the V reads for the vector L1's miss path (the bound of this
+// kernel); the copy pays off from 2048 elements per workgroup on.
+template <int GATHER_EPT, bool kLds>
+__device__ inline void gather_block(const GatherDesc& d, int64_t wave_base)
+{
+	extern __shared__ uint32_t gather_lds[];
+	const short* lut = d.lut;
+	const uint32_t* tet3 = d.tet3;
+	if (kLds) {
+		uint32_t* st = gather_lds;
+		short* sl = (short*)(gather_lds + ((d.C1 + 3) & ~3));
+		for (int q = threadIdx.x; q < d.C1; q += blockDim.x) { st[q] = d.tet3[q]; }
+		for (int q = threadIdx.x; q < d.nlut; q += blockDim.x) { sl[q] = d.lut[q]; }
+		__syncthreads();
+		lut = sl;
+		tet3 = st;
+	}
+	if (wave_base >= d.num) { return; }
+	gather_wave<GATHER_EPT>(d, wave_base, lut, tet3);
+}
+
+template <int GATHER_EPT, bool kLds>
 __global__ __launch_bounds__(256) void k_gather(GatherDesc d, int64_t xcd_chunks)
 {
 	const int64_t piece = gather_piece(xcd_chunks);
 	const int64_t wave_base = (piece * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * GATHER_EPT);
-	if (wave_base >= d.num) { return; }
-	gather_wave<GATHER_EPT>(d, wave_base);
+	if (!kLds && wave_base >= d.num) { return; }
+	if (kLds && piece * (blockDim.x >> 6) * (64 * GATHER_EPT) >= d.num) { return; }   // whole workgroup idle
+	gather_block<GATHER_EPT, kLds>(d, wave_base);
 }
 
 // all grids of a batch in one launch: descriptors in device memory, pieces concatenated
-template <int GATHER_EPT>
+template <int GATHER_EPT, bool kLds>
 __global__ __launch_bounds__(256) void k_gather_grouped(const GatherDesc* __restrict__ descs, int n, int64_t xcd_chunks)
 {
 	const int64_t piece = gather_piece(xcd_chunks);
 	int gi = 0;
 	while (gi + 1 < n && piece >= descs[gi + 1].piece_begin) { ++gi; }
 	const GatherDesc d = descs[gi];
-	const int64_t wave_base = ((piece - d.piece_begin) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * GATHER_EPT);
-	if (piece < d.piece_begin || wave_base >= d.num) { return; }
-	gather_wave<GATHER_EPT>(d, wave_base);
+	const int64_t first = (piece - d.piece_begin) * (blockDim.x >> 6) * (64 * GATHER_EPT);
+	if (piece < d.piece_begin || first >= d.num) { return; }   // whole workgroup idle
+	gather_block<GATHER_EPT, kLds>(d, first + (threadIdx.x >> 6) * (64 * GATHER_EPT));
 }
 
 // per-element FP64 sum over the folded columns, no reuse across elements:
@@ -862,6 +895,8 @@ static GatherDesc gather_desc(const EriSetup& s, const int64_t* d_idx, int64_t n
 	d.out = out;
 	d.scale = scale;
 	d.piece_begin = 0;
+	d.nlut = s.R * s.R * (4 * s.R - 3);
+	d.C1 = s.C1;
 	return d;
 }
 
@@ -870,30 +905,55 @@ static int64_t gather_pieces(int64_t num, int ept)
 	return ceil_div(num, 256 * (int64_t)ept);
 }
 
-static void launch_gather(const GatherDesc& d, int ept, int xcd)
+static bool gather_lds_fits(const GatherDesc& d)
+{
+	return gather_lds_bytes(d.nlut, d.C1) <= kGatherLdsMax;
+}
+
+template <typename F>
+static void gather_dispatch(int ept, bool lds, F&& f)
+{
+	switch (ept * 2 + (lds ? 1 : 0)) {
+		case 4: f(std::integral_constant<int, 2>(), std::false_type()); break;
+		case 5: f(std::integral_constant<int, 2>(), std::true_type()); break;
+		case 9: f(std::integral_constant<int, 4>(), std::true_type()); break;
+		case 16: f(std::integral_constant<int, 8>(), std::false_type()); break;
+		case 17: f(std::integral_constant<int, 8>(), std::true_type()); break;
+		case 32: f(std::integral_constant<int, 16>(), std::false_type()); break;
+		case 33: f(std::integral_constant<int, 16>(), std::true_type()); break;
+		default: f(std::integral_constant<int, 4>(), std::false_type()); break;
+	}
+}
+
+static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 {
 	const int64_t pieces = gather_pieces(d.num, ept);
 	if (pieces == 0) { return; }
 	const int64_t chunks = xcd ? ceil_div(pieces, 8) : 0;
 	const int64_t blocks = xcd ? 8 * chunks : pieces;
 	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
+	const bool use_lds = lds && gather_lds_fits(d);
+	const size_t shm = use_lds ? (size_t)gather_lds_bytes(d.nlut, d.C1) : 0;
 	hipStream_t st = ctx().stream;
-	if (ept == 2) { k_gather<2><<<(unsigned)blocks, 256, 0, st>>>(d, chunks); }
-	else if (ept == 8) { k_gather<8><<<(unsigned)blocks, 256, 0, st>>>(d, chunks); }
-	else { k_gather<4><<<(unsigned)blocks, 256, 0, st>>>(d, chunks); }
+	gather_dispatch(ept, use_lds, [&](auto E, auto Lds) {
+		k_gather<decltype(E)::value, decltype(Lds)::value><<<(unsigned)blocks, 256, shm, st>>>(d, chunks);
+	});
 	GLI_HIP(hipGetLastError());
 }
 
-static void launch_gather_grouped(const GatherDesc* d_descs, int n, int64_t total_pieces, int ept, int xcd)
+static void launch_gather_grouped(const GatherDesc* d_descs, int n, int64_t total_pieces, int ept, int xcd, int lds,
+                                  int lds_bytes)
 {
 	if (total_pieces == 0) { return; }
 	const int64_t chunks = xcd ? ceil_div(total_pieces, 8) : 0;
 	const int64_t blocks = xcd ? 8 * chunks : total_pieces;
 	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
+	const bool use_lds = lds && lds_bytes <= kGatherLdsMax;
+	const size_t shm = use_lds ? (size_t)lds_bytes : 0;
 	hipStream_t st = ctx().stream;
-	if (ept == 2) { k_gather_grouped<2><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); }
-	else if (ept == 8) { k_gather_grouped<8><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); }
-	else { k_gather_grouped<4><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); }
+	gather_dispatch(ept, use_lds, [&](auto E, auto Lds) {
+		k_gather_grouped<decltype(E)::value, decltype(Lds)::value><<<(unsigned)blocks, 256, shm, st>>>(d_descs, n, chunks);
+	});
 	GLI_HIP(hipGetLastError());
 }
 
@@ -1049,7 +1109,7 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	if (mixed) { mode_product_split(w.Y2.p, Cp * Cp, Cp * Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1, false); }
 	else { mode(w.Y2.p, Cp * Cp, Cp * Cp, w.V.p, 0, 1, 1, s.C1); }
 	phase_mark("mode3");
-	launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept, c.gather_xcd);
+	launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept, c.gather_xcd, c.gather_lds);
 	phase_mark("gather");
 }
 
@@ -1078,8 +1138,9 @@ struct BatchChoice {
 	int gemm_shape[3] = { 0, 0, 0 };
 	int gemm_order[3] = { 1, 1, 1 };
 	bool split[3] = { false, false, false };
-	int gather_ept = 4;
+	int gather_ept = 8;
 	int gather_order = 1;
+	int gather_lds = 1;
 };
 
 struct BatchPlan {
@@ -1093,6 +1154,7 @@ struct BatchPlan {
 	DevBuf<GatherDesc> d_gdesc;
 	int gdesc_ept = -1;
 	int64_t total_pieces = 0;
+	int lds_bytes = 0;   // LDS for the largest key table of the batch
 	SplitWorkspace* split_ws = nullptr;
 	BatchChoice choice;
 	std::string report;
@@ -1175,7 +1237,8 @@ static void batch_compute(BatchPlan& plan, const BatchChoice& ch)
 		}
 	}
 	batch_gather_descs(plan, ch.gather_ept);
-	launch_gather_grouped(plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ch.gather_ept, ch.gather_order);
+	launch_gather_grouped(plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ch.gather_ept, ch.gather_order,
+	                      ch.gather_lds, plan.lds_bytes);
 }
 
 // median time (ms) of a few repetitions of f on the library stream
@@ -1283,21 +1346,28 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 	rep += "], \"gather\": [";
 	double best = 1e30;
 	bool first = true;
-	for (int ept : { 2, 4, 8 }) {
-		for (int order = 0; order < 2; ++order) {
-			batch_gather_descs(plan, ept);
-			const double t = time_ms([&] { launch_gather_grouped(plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ept, order); });
-			rep += std::string(first ? "" : ", ") + "{\"ept\": " + std::to_string(ept) + ", \"order\": " + std::to_string(order) +
-			       ", \"ms\": " + fmt("%.4f", t) + "}";
-			first = false;
-			if (t < best) {
-				best = t;
-				ch.gather_ept = ept;
-				ch.gather_order = order;
+	for (int lds = 0; lds < 2; ++lds) {
+		if (lds && plan.lds_bytes > kGatherLdsMax) { continue; }
+		for (int ept : { 2, 4, 8, 16 }) {
+			for (int order = 0; order < 2; ++order) {
+				batch_gather_descs(plan, ept);
+				const double t = time_ms([&] {
+					launch_gather_grouped(plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ept, order, lds, plan.lds_bytes);
+				});
+				rep += std::string(first ? "" : ", ") + "{\"ept\": " + std::to_string(ept) + ", \"order\": " + std::to_string(order) +
+				       ", \"lds\": " + std::to_string(lds) + ", \"ms\": " + fmt("%.4f", t) + "}";
+				first = false;
+				if (t < best) {
+					best = t;
+					ch.gather_ept = ept;
+					ch.gather_order = order;
+					ch.gather_lds = lds;
+				}
 			}
 		}
 	}
-	rep += "], \"gather_chosen\": {\"ept\": " + std::to_string(ch.gather_ept) + ", \"order\": " + std::to_string(ch.gather_order) + "}}";
+	rep += "], \"gather_chosen\": {\"ept\": " + std::to_string(ch.gather_ept) + ", \"order\": " + std::to_string(ch.gather_order) +
+	       ", \"lds\": " + std::to_string(ch.gather_lds) + "}}";
 	plan.report = rep;
 	return ch;
 }
@@ -1350,7 +1420,11 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 		params += (size_t)K[g];
 		plan->grids.push_back(std::move(bg));
 	}
-	for (auto& g : plan->grids) { g->scale_off = params++; }
+	for (auto& g : plan->grids) {
+		g->scale_off = params++;
+		const GatherDesc gd = gather_desc(g->s, nullptr, 0, nullptr, nullptr, nullptr);
+		plan->lds_bytes = std::max(plan->lds_bytes, gather_lds_bytes(gd.nlut, gd.C1));
+	}
 	plan->n_params = params;
 	plan->d_params.alloc(params);
 	GLI_HIP(hipHostMalloc((void**)&plan->h_params, params * sizeof(double), hipHostMallocDefault));
@@ -1375,6 +1449,7 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 		}
 		ch.gather_ept = c.gather_ept;
 		ch.gather_order = c.gather_xcd;
+		ch.gather_lds = c.gather_lds;
 		plan->report = "{\"autotuned\": false}";
 	}
 	plan->choice = ch;

@@ -42,7 +42,8 @@ int gli_synchronize(void);
  *   "tile_order"  output-tile order of the mode products: 0 row tiles fastest (default), 1 XCD-grouped
  *   "gemm_shape"  FP64 mode-product tile: 0 64x64 (default), 1 128x64, 2 64x128, 3 128x128 (a shape
  *                 that does not divide the key padding of a triangular product falls back to 0)
- *   "gather_ept"  elements per lane in the per-element gather: 2, 4 (default) or 8
+ *   "gather_ept"  elements per lane in the per-element gather: 2, 4, 8 (default) or 16
+ *   "gather_lds"  1 (default): each gather workgroup copies the key tables into LDS first; 0: global reads
  *   "gather_order" 0: workgroups take consecutive pieces of the list; 1 (default): each XCD takes one
  *                 contiguous chunk of the list (its L2 then holds the key-tensor entries it reuses)
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
@@ -112,6 +113,13 @@ int gli_eri_key_count(int64_t R, int symmetric);
 
 /* release cached device workspace */
 void gli_release_workspace(void);
+
+/* Canonical sparse four-index (two-index for ERIDA) of grid points pts on the symmetric grid gs,
+ * with the O_h permutations of evaluate_octahedral_grid_permutations: the box is centred and the
+ * minimum over bias placements, O_h and label swaps is taken (eri_integrals.c:240-291); -1 when a
+ * centred point leaves the grid.  The Python surface uses them for single-element lookups. */
+glong gli_eri_sparse_key(const struct cartesian_grid_3d* gs, const glong* const perm[48], const union cartesian_grid_point_3d pts[4]);
+glong gli_erida_sparse_key(const struct cartesian_grid_3d* gs, const glong* const perm[48], const union cartesian_grid_point_3d pts[2]);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@ compared with the reference tests' own bounds (1e-13 ERI/nuclear/kinetic,
 1e-12 ERIDA dense, test/test_*.c).
 """
 import ctypes as C
+import itertools
 import math
 import sys
 
@@ -357,26 +358,27 @@ def test_tuning_variants_bit_identical(L):
     idx = d.to_numpy(np.int64, n)
     rng = np.random.default_rng(5)
     shuffled = idx[rng.permutation(n)][: n - 77]
-    knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1)), (b"gather_ept", (2, 4, 8))]
+    knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1)), (b"gather_ept", (2, 4, 8, 16)), (b"gather_lds", (0, 1)),
+             (b"gemm_shape", (0, 3))]
     saved = {k: L.gli_get_tuning(k) for k, _ in knobs}
     try:
-        ref = ref_s = None
-        for to in (0, 1):
-            for go in (0, 1):
-                for ept in (2, 4, 8):
-                    for k, v in zip((b"tile_order", b"gather_order", b"gather_ept"), (to, go, ept)):
-                        assert L.gli_set_tuning(k, v) == 0
-                    for method in (0, 1):
-                        got = eri_values_device(c, sym(7), idx, 1e-16, method)
-                        got_s = eri_values_device(c, sym(7), shuffled, 1e-16, method)
-                        if ref is None:
-                            ref, ref_s = {}, {}
-                        if method not in ref:
-                            ref[method], ref_s[method] = got, got_s
-                        assert np.array_equal(got, ref[method]), (to, go, ept, method)
-                        assert np.array_equal(got_s, ref_s[method]), (to, go, ept, method)
+        ref, ref_s = {}, {}
+        for to, go, ept, lds, shape in itertools.product((0, 1), (0, 1), (2, 4, 8, 16), (0, 1), (0, 3)):
+            if (to + go + ept + lds + shape) % 2 and ept not in (4, 16):
+                continue   # a spread of the combinations keeps the test short; every value of every knob is used
+            for k, v in zip((b"tile_order", b"gather_order", b"gather_ept", b"gather_lds", b"gemm_shape"),
+                            (to, go, ept, lds, shape)):
+                assert L.gli_set_tuning(k, v) == 0
+            for method in (0, 1):
+                got = eri_values_device(c, sym(7), idx, 1e-16, method)
+                got_s = eri_values_device(c, sym(7), shuffled, 1e-16, method)
+                if method not in ref:
+                    ref[method], ref_s[method] = got, got_s
+                assert np.array_equal(got, ref[method]), (to, go, ept, lds, shape, method)
+                assert np.array_equal(got_s, ref_s[method]), (to, go, ept, lds, shape, method)
         assert np.array_equal(ref_s[0], ref[0][np.searchsorted(idx, shuffled)])
         assert L.gli_set_tuning(b"gather_ept", 3) == -1
+        assert L.gli_set_tuning(b"gather_lds", 2) == -1
         assert L.gli_set_tuning(b"no_such_knob", 1) == -1
         assert L.gli_get_tuning(b"no_such_knob") == -1
     finally:

@@ -41,6 +41,14 @@ def test_library_exports_every_declared_symbol():
     assert missing == []
 
 
+def test_library_exports_nothing_else():
+    """kernels, C++ helpers and the private host/device interface stay local to the library"""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    assert exported - declared_functions() == set()
+
+
 def test_hdf5_library_exports_every_declared_symbol():
     path = os.path.join(os.path.dirname(_lib.LIB_PATH), "libgli_amd_hdf5.so")
     L = ctypes.CDLL(path)
