@@ -244,6 +244,94 @@ def batch_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def aux_bench(args):
+    """ERI-DA set, nuclear matrix (unit charges at (0, 0, +-0.7 / a) grid units, a = 1) and kinetic matrix on
+    an n^3 grid: GPU time and the max abs error against the reference C build on the same inputs (the
+    full ERI-DA set and kinetic matrix; a seeded sample of nuclear entries through the reference's
+    single-entry function, whose full matrix takes minutes on the host)."""
+    import gausslectronitegrate_amd as gli
+    from oracle import oracle as orc
+    _lib.require_gpu()
+    n, K = args.n, args.K
+    c = synthetic_coefficients(K, args.seed)
+    h = (n - 1) // 2
+    grid = ((-h, n), (-h, n), (-h, n))
+    N = n ** 3
+    pos = np.array([[0.0, 0.0, 0.7], [0.0, 0.0, -0.7]])
+    chg = np.array([1, 1], dtype=np.int32)
+    have_ref = os.path.exists(orc.REF_LIB_PATH)
+    ref = _abi.bind(C.CDLL(orc.REF_LIB_PATH)) if have_ref else None
+    gd = _abi.GaussletData()
+    cc = np.ascontiguousarray(c)
+    gd.coefficients = cc.ctypes.data_as(_abi.P_double)
+    gd.indices.istart = -(K - 1) // 2
+    gd.indices.num = K
+    g = _abi.Grid.of(grid)
+
+    def timed(f, reps=3):
+        f()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = f()
+        return r, (time.perf_counter() - t0) / reps * 1e3
+
+    out = {"metric": "one-body and ERI-DA assembly on an %d^3 grid (K=%d, seed %d), GPU vs reference C build" % (n, K, args.seed),
+           "grid": n, "K": K, "nuclei": pos.tolist(), "charges": chg.tolist(), "tol": args.tol}
+    # ERI-DA: all unique pairs
+    erida, t_g = timed(lambda: gli.compute_sparse_erida_gausslet_integrals(c, grid, args.tol))
+    rec = {"unique_elements": erida.num_entries, "gpu_ms_incl_host_copies": t_g}
+    if ref is not None:
+        idx = np.ascontiguousarray(erida.two_indices)
+        res = _abi.SparseErida()
+        t0 = time.perf_counter()
+        ref.compute_sparse_erida_gausslet_integrals(C.byref(gd), C.byref(g), idx.ctypes.data_as(_abi.P_glong), len(idx),
+                                                    args.tol, C.byref(res))
+        rec["reference_ms"] = (time.perf_counter() - t0) * 1e3
+        rv = np.ctypeslib.as_array(res.integral_values, shape=(len(idx),)).copy()
+        ref.delete_sparse_erida_gausslet_integrals(C.byref(res))
+        rec["max_abs_err_vs_reference"] = float(np.max(np.abs(erida.integral_values - rv)))
+    out["erida"] = rec
+    # kinetic: full matrix
+    kin, t_g = timed(lambda: gli.compute_kinetic_gausslet_integrals(c, grid))
+    rec = {"matrix": [N, N], "gpu_ms_incl_host_copies": t_g}
+    if ref is not None:
+        res = _abi.KineticIntegrals()
+        t0 = time.perf_counter()
+        ref.compute_kinetic_gausslet_integrals(C.byref(gd), C.byref(g), C.byref(res))
+        rec["reference_ms"] = (time.perf_counter() - t0) * 1e3
+        rk = np.ctypeslib.as_array(res.integral_values, shape=(N * N,)).reshape(N, N).copy()
+        ref.delete_kinetic_gausslet_integrals(C.byref(res))
+        rec["max_abs_err_vs_reference"] = float(np.max(np.abs(np.asarray(kin).reshape(N, N) - rk)))
+    out["kinetic"] = rec
+    # nuclear: full matrix on the GPU, seeded sample of entries through the reference
+    nuc, t_g = timed(lambda: gli.compute_nuclear_gausslet_integrals(c, grid, pos, chg, args.tol))
+    nuc = np.asarray(nuc).reshape(N, N)
+    rec = {"matrix": [N, N], "gpu_ms_incl_host_copies": t_g}
+    if ref is not None:
+        rng = np.random.default_rng(7)
+        pairs = rng.integers(0, N, size=(48, 2))
+        nucl = (_abi.Nucleus * 2)()
+        for q in range(2):
+            for a in range(3):
+                nucl[q].pos[a] = pos[q][a]
+            nucl[q].charge = int(chg[q])
+        pts = (_abi.GridPoint * 2)()
+        t0 = time.perf_counter()
+        errs = []
+        for i, j in pairs:
+            for slot, p in enumerate((i, j)):
+                x, y, z = p // (n * n) - h, (p // n) % n - h, p % n - h
+                pts[slot].c[0], pts[slot].c[1], pts[slot].c[2] = x, y, z
+            v = ref.compute_nuclear_gausslet_integral(C.byref(gd), pts, nucl, 2, args.tol)
+            errs.append(abs(nuc[i, j] - v))
+        rec["reference_ms_per_entry"] = (time.perf_counter() - t0) * 1e3 / len(pairs)
+        rec["reference_sample"] = len(pairs)
+        rec["max_abs_err_vs_reference_sample"] = float(max(errs))
+        rec["symmetric"] = bool(np.array_equal(nuc, nuc.T))
+    out["nuclear"] = rec
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,6 +356,8 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="mixed-size batch: 5^3 (K=29), 7^3 (K=43), 9^3 and 11^3 (K=57) in one autotuned plan, "
                          "replayed as one hipGraph per step")
+    ap.add_argument("--aux", action="store_true",
+                    help="ERI-DA set, nuclear and kinetic matrices on the --n grid against the reference C build")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
                     help="only run an elementwise kernel of known traffic (counter calibration) and exit")
     args = ap.parse_args()
@@ -276,6 +366,9 @@ def main():
         return
     if args.gpus == 1 and args.batch:
         batch_bench(args)
+        return
+    if args.gpus == 1 and args.aux:
+        aux_bench(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would take a contiguous, "

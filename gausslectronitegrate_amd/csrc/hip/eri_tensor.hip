@@ -128,11 +128,6 @@ constexpr int KEY_ALIGN = 64;   // key columns are padded to a multiple of this
 __host__ __device__ constexpr int shape_bm(int s) { return (s == 1 || s == 3) ? 128 : 64; }
 __host__ __device__ constexpr int shape_bn(int s) { return (s == 2 || s == 3) ? 128 : 64; }
 
-// position of the sorted key triple k1 <= k2 <= k3 in the tetrahedral layout of a symmetric 3-tensor
-__host__ __device__ inline int64_t tet_index(int k1, int k2, int k3)
-{
-	return (int64_t)k3 * (k3 + 1) * (k3 + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1;
-}
 
 template <int BM, int BN>
 __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, double (*As)[BM], double (*Bs)[BN])
@@ -857,6 +852,12 @@ static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol,
 	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, 0, st>>>(d_coeffs, s.K, f);
 	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, 0, st>>>(f, s.K, s.R, tol, P);
 	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.ks->keys_dev.p, s.C1, s.K, s.R, s.M, s.Cp, B);
+	GLI_HIP(hipGetLastError());
+}
+
+void kernel_tensor(const double* T, int M, int Mp, double* T3)
+{
+	k_kernel_tensor<<<dim3((unsigned)ceil_div(Mp, 64), (unsigned)Mp, (unsigned)Mp), 64, 0, ctx().stream>>>(T, M, Mp, T3);
 	GLI_HIP(hipGetLastError());
 }
 

@@ -8,8 +8,8 @@
 // d_s d'_t is at most 2^14; the products with the same digit sum s + t = d
 // (at most NS of them) share one FP32 MFMA accumulator, whose partial sums
 // stay integers below 2^24 over a chunk of KC = 192 terms, hence exact.  Each
-// chunk is flushed into one FP64 accumulator with weights 256^-(d+2), and the
-// column scales are applied at the end.  With NS = 5 the dropped terms are
+// chunk is flushed into a compensated FP64 accumulator (TwoSum, hi + lo) with
+// weights 256^-(d+2), and the column scales are applied at the end.  With NS = 5 the dropped terms are
 // below about 2^-40 of the column scales (near 1e-11 absolute on the ERI key
 // tensor of an 11^3 grid with K = 57).
 #include "common.hpp"
@@ -68,6 +68,25 @@ __global__ __launch_bounds__(256) void k_split_tile(const double* __restrict__ X
 	}
 }
 
+// error-free sum s + e = a + b (Knuth's TwoSum)
+__device__ inline void two_sum(double a, double b, double& s, double& e)
+{
+	s = a + b;
+	const double bb = s - a;
+	e = (a - (s - bb)) + (b - bb);
+}
+
+// compensated FP64 combination of the exact group sums of one chunk: sum_d grp[d] 256^-(d+2)
+template <int NS>
+__device__ inline void combine_groups(const float* g, double& hi, double& lo)
+{
+	for (int d = NS - 1; d >= 0; --d) {
+		double e;
+		two_sum(hi, ldexp((double)g[d], -8 * (d + 2)), hi, e);
+		lo += e;
+	}
+}
+
 // Y[p][k] from the digits of X (Ppad columns) and B (Cpad columns)
 constexpr int KC = 192;  // exactness bound of an FP32 group accumulator: NS * 2^14 * KC < 2^24 for NS <= 5
 
@@ -97,10 +116,11 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 
 	const int wm = wave >> 1, wn = wave & 1;
 	f4 grp[NS][2][2];                   // FP32 group accumulators, one per digit sum
-	double y[2][2][kOneChunk ? 1 : 4];  // FP64 accumulators across chunks
+	double y[2][2][kOneChunk ? 1 : 4];  // compensated FP64 accumulators (hi, lo) across chunks
+	double ylo[2][2][kOneChunk ? 1 : 4];
 	for (int i = 0; i < 2; ++i) {
 		for (int j = 0; j < 2; ++j) {
-			for (int v = 0; v < (kOneChunk ? 1 : 4); ++v) { y[i][j][v] = 0; }
+			for (int v = 0; v < (kOneChunk ? 1 : 4); ++v) { y[i][j][v] = 0; ylo[i][j][v] = 0; }
 			for (int d = 0; d < NS; ++d) { grp[d][i][j] = f4{0, 0, 0, 0}; }
 		}
 	}
@@ -149,9 +169,9 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 			for (int i = 0; i < 2; ++i) {
 				for (int j = 0; j < 2; ++j) {
 					for (int v = 0; v < 4; ++v) {
-						double acc = 0;
-						for (int d = NS - 1; d >= 0; --d) { acc += ldexp((double)grp[d][i][j][v], -8 * (d + 2)); }
-						y[i][j][kOneChunk ? 0 : v] += acc;
+						float g[NS];
+						for (int d = 0; d < NS; ++d) { g[d] = grp[d][i][j][v]; }
+						combine_groups<NS>(g, y[i][j][kOneChunk ? 0 : v], ylo[i][j][kOneChunk ? 0 : v]);
 					}
 					for (int d = 0; d < NS; ++d) { grp[d][i][j] = f4{0, 0, 0, 0}; }
 				}
@@ -168,11 +188,14 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 				if (row >= P || col >= C) { continue; }
 				double acc;
 				if (kOneChunk) {
-					acc = 0;
-					for (int d = NS - 1; d >= 0; --d) { acc += ldexp((double)grp[d][i][j][v], -8 * (d + 2)); }
+					float g[NS];
+					for (int d = 0; d < NS; ++d) { g[d] = grp[d][i][j][v]; }
+					double hi = 0, lo = 0;
+					combine_groups<NS>(g, hi, lo);
+					acc = hi + lo;
 				}
 				else {
-					acc = y[i][j][kOneChunk ? 0 : v];
+					acc = y[i][j][kOneChunk ? 0 : v] + ylo[i][j][kOneChunk ? 0 : v];
 				}
 				const double val = ldexp(acc, ex[row] + eb[col]);
 				if (tet_C1 > 0) {

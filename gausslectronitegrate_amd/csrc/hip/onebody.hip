@@ -227,88 +227,92 @@ __global__ void k_erida_factors(const double* __restrict__ c, int K, double tol,
 	F[q] = (fabs(acc) > tol) ? acc : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_erida_values(const int64_t* __restrict__ idx, int64_t num, int nx, int ny, int nz,
-	const double* __restrict__ F, int K, int M, int Mp, const double* __restrict__ T, double* __restrict__ out)
+// Folded ERI-DA columns: per axis the reference sums F[d] T'[(3s + d)^2 + ...] over d
+// (erida_integrals.c:482-535); grouping d by mu = |3s + d| gives B[mu][|s|] (F[-d] = F[d] for every
+// coefficient set, so the column depends on |s| only).
+__global__ void k_erida_fold(const double* __restrict__ F, int K, int R, int M, int Cp, double* __restrict__ B)
 {
-	extern __shared__ double smem[];
-	double* col = smem;           // 3 x Mp
-	double* red = smem + 3 * Mp;  // 256
-	__shared__ int win[3][2];
-	const int tid = threadIdx.x;
-	const int64_t N = (int64_t)nx * ny * nz;
-	for (int64_t e = blockIdx.x; e < num; e += gridDim.x) {
-		const int64_t i = idx[e] / N, j = idx[e] % N;
-		int s[3];
-		s[2] = (int)(i % nz) - (int)(j % nz);
-		s[1] = (int)((i / nz) % ny) - (int)((j / nz) % ny);
-		s[0] = (int)(i / ((int64_t)nz * ny)) - (int)(j / ((int64_t)nz * ny));
-		for (int a = 0; a < 3; ++a) {
-			for (int m = tid; m < Mp; m += blockDim.x) {
-				double v = 0;
-				if (m <= M) {
-					const int t1 = m - 3 * s[a] + (K - 1);
-					if (t1 >= 0 && t1 < 2 * K - 1) { v += F[t1]; }
-					if (m > 0) {
-						const int t2 = -m - 3 * s[a] + (K - 1);
-						if (t2 >= 0 && t2 < 2 * K - 1) { v += F[t2]; }
-					}
-				}
-				col[a * Mp + m] = v;
-			}
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	const int mu = blockIdx.y;
+	if (k >= Cp) { return; }
+	double v = 0;
+	if (k < R && mu <= M) {
+		int d = mu - 3 * k;
+		if (d >= -(K - 1) && d <= K - 1) { v += F[d + K - 1]; }
+		if (mu > 0) {
+			d = -mu - 3 * k;
+			if (d >= -(K - 1) && d <= K - 1) { v += F[d + K - 1]; }
 		}
-		__syncthreads();
-		if (tid < 3) {
-			int lo = M + 1, hi = -1;
-			for (int m = 0; m <= M; ++m) {
-				if (col[tid * Mp + m] != 0) { lo = min(lo, m); hi = m; }
-			}
-			win[tid][0] = lo;
-			win[tid][1] = hi;
-		}
-		__syncthreads();
-		const int n0 = win[0][1] - win[0][0] + 1, n1 = win[1][1] - win[1][0] + 1;
-		double acc = 0;
-		if (n0 > 0 && n1 > 0 && win[2][1] >= win[2][0]) {
-			for (int pq = tid; pq < n0 * n1; pq += blockDim.x) {
-				const int m1 = win[0][0] + pq / n1, m2 = win[1][0] + pq % n1;
-				const double w = col[m1] * col[Mp + m2];
-				if (w == 0) { continue; }
-				const int base = m1 * m1 + m2 * m2;
-				double sum = 0;
-				for (int m3 = win[2][0]; m3 <= win[2][1]; ++m3) { sum += col[2 * Mp + m3] * T[base + m3 * m3]; }
-				acc += w * sum;
-			}
-		}
-		red[tid] = acc;
-		__syncthreads();
-		for (int w = 128; w > 0; w >>= 1) {
-			if (tid < w) { red[tid] += red[tid + w]; }
-			__syncthreads();
-		}
-		if (tid == 0) { out[e] = red[0]; }
-		__syncthreads();
 	}
+	B[(int64_t)mu * Cp + k] = v;
 }
 
+// value of pair t = i N + j: the key tensor at the sorted |displacements| (NaN outside the grid)
+__global__ void k_erida_gather(const int64_t* __restrict__ idx, int64_t num, int nx, int ny, int nz,
+                               const double* __restrict__ V, double* __restrict__ out)
+{
+	const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= num) { return; }
+	const int64_t N = (int64_t)nx * ny * nz;
+	const int64_t t = idx[e];
+	if (t < 0 || t >= N * N) {
+		out[e] = __builtin_nan("");
+		return;
+	}
+	const int64_t i = t / N, j = t % N;
+	int k[3];
+	k[2] = abs((int)(i % nz) - (int)(j % nz));
+	k[1] = abs((int)((i / nz) % ny) - (int)((j / nz) % ny));
+	k[0] = abs((int)(i / ((int64_t)nz * ny)) - (int)(j / ((int64_t)nz * ny)));
+	const int lo = min(k[0], k[1]), hi = max(k[0], k[1]);
+	const int k3 = max(hi, k[2]), mid = min(hi, k[2]);
+	out[e] = V[tet_index(min(lo, mid), max(lo, mid), k3)];
+}
+
+struct EridaWorkspace {
+	DevBuf<double> coeffs, F, B, Y1, Y2, V, T3;
+	int T3_M = -1;
+};
+
+// ERI-DA values as a key-tensor contraction (the ERI method with one key per axis: |s|):
+// V[k1][k2][k3] = sum_mu T'[mu1^2 + mu2^2 + mu3^2] B[mu1][k1] B[mu2][k2] B[mu3][k3], then one read per
+// pair; the summed terms are the reference's (erida_integrals.c:429-542) in another order
 static void erida_values_dev(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* d_idx, int64_t num, double tol, double* d_vals)
 {
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	GLI_REQUIRE(K >= 1 && (K % 2) == 1, "the number of gausslet coefficients must be odd");
 	GLI_REQUIRE(g[1] > 0 && g[3] > 0 && g[5] > 0, "grid axes must have a positive number of points");
+	GLI_REQUIRE(g[1] < 1024 && g[3] < 1024 && g[5] < 1024, "at most 1023 points per axis");
 	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
+	static EridaWorkspace* ws = new EridaWorkspace();
+	EridaWorkspace& w = *ws;
 	hipStream_t st = c.stream;
 	const int R = (int)std::max(g[1], std::max(g[3], g[5]));
 	const int M = 3 * (R - 1) + ((int)K - 1);
-	const int Mp = (int)round_up(M + 1, 16);
+	const int64_t Mp = round_up(M + 1, 16), Cp = round_up(R, 64);
+	GLI_REQUIRE(Mp <= 2048, "grid too long for the ERI-DA key tensor");
 	const double* T = erida_table(3 * (int64_t)M * M + 1);
-	DevBuf<double> dc(K), F(2 * K - 1);
-	upload_async(dc.p, coeffs, K * sizeof(double));
-	k_erida_factors<<<(unsigned)ceil_div(2 * K - 1, 128), 128, 0, st>>>(dc.p, (int)K, tol, F.p);
+	w.coeffs.reserve(K);
+	w.F.reserve(2 * K - 1);
+	w.B.reserve(Mp * Cp);
+	upload_async(w.coeffs.p, coeffs, K * sizeof(double));
+	k_erida_factors<<<(unsigned)ceil_div(2 * K - 1, 128), 128, 0, st>>>(w.coeffs.p, (int)K, tol, w.F.p);
+	k_erida_fold<<<dim3((unsigned)ceil_div(Cp, 128), (unsigned)Mp), 128, 0, st>>>(w.F.p, (int)K, R, M, (int)Cp, w.B.p);
+	GLI_HIP(hipGetLastError());
+	if (w.T3_M != M) {
+		w.T3.reserve(Mp * Mp * Mp);
+		kernel_tensor(T, M, (int)Mp, w.T3.p);
+		w.T3_M = M;
+	}
+	w.Y1.reserve(Mp * Mp * Cp);
+	w.Y2.reserve(Mp * Cp * Cp);
+	w.V.reserve(tet_index(0, 0, R));
+	mode_product(w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, (int)Cp, (int)Mp, w.Y1.p, Cp, (int)Cp, 0, 0, 0, 0);
+	mode_product(w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, (int)Cp, (int)Mp, w.Y2.p, Cp, (int)Cp, 1, 0, 0, 0);
+	mode_product(w.Y2.p, Cp * Cp, Cp * Cp, w.B.p, Cp, (int)Cp, (int)Mp, w.V.p, 0, (int)Cp, 1, 1, 0, R);
 	if (num > 0) {
-		const size_t shm = (3 * (size_t)Mp + 256) * sizeof(double);
-		k_erida_values<<<(unsigned)std::min<int64_t>(num, 65536), 256, shm, st>>>(d_idx, num, (int)g[1], (int)g[3], (int)g[5],
-		                                                                        F.p, (int)K, M, Mp, T, d_vals);
+		k_erida_gather<<<(unsigned)ceil_div(num, 256), 256, 0, st>>>(d_idx, num, (int)g[1], (int)g[3], (int)g[5], w.V.p, d_vals);
 	}
 	GLI_HIP(hipGetLastError());
 	GLI_HIP(hipStreamSynchronize(st));
