@@ -170,10 +170,13 @@ void phase_mark(const char* name);
 // Y[p][k] = sum_{mu < Kd} X[mu * ldx + p] * B[mu * ldb + k] for p < P, k < C (FP64 MFMA);
 // with Cp > 0 and tri_cols / tri_rows, tiles outside k >= r / outer <= r (p = outer*Cp + r) are skipped;
 // with tet_C1 > 0 the output goes to the tetrahedral layout of the sorted triple (outer, r, k), all < tet_C1
-// position of the sorted key triple k1 <= k2 <= k3 in the tetrahedral layout of a symmetric 3-tensor
+// Position of the sorted key triple k1 <= k2 <= k3 in the tetrahedral layout of a symmetric 3-tensor:
+// blocks of growing k3 (C(k3+2, 3) entries before block k3), inside a block the runs k2 = k1..k3 of
+// each k1 stored contiguously.  The middle key is the contiguous one because the mode-3 product writes
+// output tiles of fixed k1 whose rows run over k2 (full-line stores).
 __host__ __device__ inline int64_t tet_index(int k1, int k2, int k3)
 {
-	return (int64_t)k3 * (k3 + 1) * (k3 + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1;
+	return (int64_t)k3 * (k3 + 1) * (k3 + 2) / 6 + (int64_t)k1 * (k3 + 1) - (int64_t)k1 * (k1 - 1) / 2 + (k2 - k1);
 }
 
 // T3[m1][m2][m3] = T[m1^2 + m2^2 + m3^2] for m <= M, 0 in the padding up to Mp (library stream)

@@ -351,26 +351,44 @@ __device__ bool erida_is_canonical(const EnumGrid& g, const int P[2][3], int t)
 	return true;
 }
 
+// one workgroup per first index i, lanes over j in rounds of 256 consecutive j; kind 0 counts, kind 1
+// writes each round's kept pairs in ascending j (wave ballots + a 4-wave prefix in LDS)
 template <int kWrite>
-__global__ void k_erida_rows(EnumGrid g, uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int64_t* __restrict__ out)
+__global__ __launch_bounds__(256) void k_erida_rows(EnumGrid g, uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets,
+                                                     int64_t* __restrict__ out)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= g.N) { return; }
+	__shared__ uint32_t wave_cnt[4];
+	const int i = blockIdx.x;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	int P[2][3];
 	P[0][0] = i / (g.n * g.n) - g.h; P[0][1] = (i / g.n) % g.n - g.h; P[0][2] = i % g.n - g.h;
-	uint32_t cnt = 0;
-	int64_t pos = kWrite ? offsets[i] : 0;
-	for (int j = i; j < g.N; ++j) {
-		P[1][0] = j / (g.n * g.n) - g.h; P[1][1] = (j / g.n) % g.n - g.h; P[1][2] = j % g.n - g.h;
-		bool centred = true;
-		for (int a = 0; a < 3; ++a) { centred = centred && abs(P[0][a] + P[1][a]) <= 1; }
-		if (!centred) { continue; }
-		if (erida_is_canonical(g, P, i * g.N + j)) {
-			if (kWrite) { out[pos++] = (int64_t)i * g.N + j; }
-			else { cnt++; }
+	int64_t base = kWrite ? offsets[i] : 0;
+	uint32_t total = 0;
+	for (int j0 = i; j0 < g.N; j0 += 256) {
+		const int j = j0 + (int)threadIdx.x;
+		bool keep = false;
+		if (j < g.N) {
+			P[1][0] = j / (g.n * g.n) - g.h; P[1][1] = (j / g.n) % g.n - g.h; P[1][2] = j % g.n - g.h;
+			bool centred = true;
+			for (int a = 0; a < 3; ++a) { centred = centred && abs(P[0][a] + P[1][a]) <= 1; }
+			keep = centred && erida_is_canonical(g, P, i * g.N + j);
 		}
+		const unsigned long long m = __ballot(keep);
+		if (lane == 0) { wave_cnt[wave] = (uint32_t)__popcll(m); }
+		__syncthreads();
+		uint32_t before = 0, round = 0;
+		for (int w = 0; w < 4; ++w) {
+			before += (w < wave) ? wave_cnt[w] : 0u;
+			round += wave_cnt[w];
+		}
+		if (kWrite && keep) {
+			const uint32_t rank = before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+			out[base + total + rank] = (int64_t)i * g.N + j;
+		}
+		total += round;
+		__syncthreads();
 	}
-	if (!kWrite) { counts[i] = cnt; }
+	if (!kWrite && threadIdx.x == 0) { counts[i] = total; }
 }
 
 static void erida_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
@@ -386,13 +404,12 @@ static void erida_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 	hipStream_t st = c.stream;
 	DevBuf<uint32_t> counts(g.N);
 	DevBuf<int64_t> offsets(g.N);
-	const unsigned blocks = (unsigned)ceil_div(g.N, 128);
-	k_erida_rows<0><<<blocks, 128, 0, st>>>(g, counts.p, nullptr, nullptr);
+	k_erida_rows<0><<<(unsigned)g.N, 256, 0, st>>>(g, counts.p, nullptr, nullptr);
 	GLI_HIP(hipGetLastError());
 	const int64_t total = exclusive_scan<uint32_t>(counts.p, g.N, offsets.p);
 	int64_t* out = nullptr;
 	GLI_HIP(hipMalloc(&out, (size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));
-	k_erida_rows<1><<<blocks, 128, 0, st>>>(g, nullptr, offsets.p, out);
+	k_erida_rows<1><<<(unsigned)g.N, 256, 0, st>>>(g, nullptr, offsets.p, out);
 	GLI_HIP(hipGetLastError());
 	GLI_HIP(hipStreamSynchronize(st));
 	*d_out = out;

@@ -576,7 +576,8 @@ __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const
 			const unsigned mid = min(hi, kk[u][2]);
 			k3[u] = max(hi, kk[u][2]);
 			const unsigned k1 = min(lo, mid), k2 = max(lo, mid);
-			vi[u] = (__umul24(k2, k2 + 1u) >> 1) + k1;
+			// tet_index without its k3 block offset (added from the table below); k1 (k3 + 1) < 2^32 for C1 < 2^15
+			vi[u] = __umul24(k1, k3[u] + 1u) - (__umul24(k1, k1 - 1u) >> 1) + (k2 - k1);
 		}
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] += tet3[k3[u]]; }

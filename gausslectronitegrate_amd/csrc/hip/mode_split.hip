@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 					const int64_t gp = p_base + row;
 					const int k1 = (int)(gp / Cp), k2 = (int)(gp % Cp);
 					if (k1 <= k2 && k2 <= col && col < tet_C1) {
-						Y[(int64_t)col * (col + 1) * (col + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1] = val;
+						Y[tet_index(k1, k2, col)] = val;
 					}
 				}
 				else {

@@ -582,3 +582,60 @@ def test_hdf5_streamed_save_from_device(L, tmp_path):
     assert back.num_entries == n and back.grid == sym(9)
     assert np.array_equal(back.four_indices, d.to_numpy(np.int64, n))
     assert np.array_equal(back.integral_values, dv.to_numpy(np.float64, n))
+
+
+def test_batch_split_products_in_graph(L):
+    """a mixed-method plan without autotuning keeps the split-precision products inside its captured
+    graph (per-grid digit extraction in a plan-owned workspace): every replay equals the single-grid
+    mixed path bit for bit"""
+    say("batch split in graph")
+    specs = [(sym(5), golden("coeffs_K29_s0.npy"), None), (sym(7), golden("coeffs_K57_s1.npy"), None)]
+    entries = _batch_setup(specs)
+    cptr, Ks, grids, dptr, nums, vptr = _batch_arrays(entries)
+    plan = C.c_void_p()
+    _lib.check(L.gli_eri_batch_create(len(entries), cptr, Ks.ctypes.data_as(_abi.P_glong), grids.ctypes.data_as(_abi.P_glong),
+                                      dptr, nums.ctypes.data_as(_abi.P_glong), 1e-16, 1, vptr, 0, C.byref(plan)),
+               "gli_eri_batch_create")
+    try:
+        refs = [eri_values_device(e[1], e[0], e[2], 1e-16, 1) for e in entries]
+        for rep in range(3):
+            _lib.check(L.gli_eri_batch_run(plan, None, None), "run")
+            assert L.gli_synchronize() == 0
+            for e, ref in zip(entries, refs):
+                assert np.array_equal(e[4].to_numpy(np.float64, len(e[2])), ref), rep
+            eri_values_device(entries[1][1], entries[1][0], entries[1][2][:1000], 1e-16, 1)   # eager work between replays
+    finally:
+        L.gli_eri_batch_destroy(plan)
+
+
+def test_additive_api_errors(L):
+    """bad arguments of the additive entry points return -1 with a message and leave the library usable"""
+    say("api errors")
+    c = golden("coeffs_K57_s0.npy")
+    d, n = eri_enumerate_device(3)
+    dv = DevArray(8 * n)
+    P = c.ctypes.data_as(_abi.P_double)
+    bad_grid = np.array([-1, 0, -1, 3, -1, 3], dtype=np.int64)
+    assert L.gli_eri_values_device(P, len(c), bad_grid.ctypes.data_as(_abi.P_glong), d.ptr, n, 1e-16, 0, dv.ptr) == -1
+    assert L.gli_last_error()
+    L.gli_clear_error()
+    g = np.array([-1, 3, -1, 3, -1, 3], dtype=np.int64)
+    assert L.gli_eri_values_device(P, len(c), g.ctypes.data_as(_abi.P_glong), d.ptr, n, 1e-16, 7, dv.ptr) == -1
+    assert L.gli_eri_values_device(P, len(c) - 1, g.ctypes.data_as(_abi.P_glong), d.ptr, n, 1e-16, 0, dv.ptr) == -1
+    assert L.gli_eri_values_device(P, len(c), g.ctypes.data_as(_abi.P_glong), d.ptr, n, -1.0, 0, dv.ptr) == -1
+    p = C.c_void_p()
+    num = C.c_int64()
+    assert L.gli_eri_enumerate_device(4, C.byref(p), C.byref(num)) == -1   # even grids have no centred enumeration
+    plan = C.c_void_p()
+    cptr = (_abi.P_double * 1)(P)
+    Ks = np.array([len(c)], dtype=np.int64)
+    dptr = (C.c_void_p * 1)(d.ptr)
+    nums = np.array([n], dtype=np.int64)
+    vptr = (C.c_void_p * 1)(dv.ptr)
+    assert L.gli_eri_batch_create(1, cptr, Ks.ctypes.data_as(_abi.P_glong), g.ctypes.data_as(_abi.P_glong), dptr,
+                                  nums.ctypes.data_as(_abi.P_glong), 1e-16, 2, vptr, 0, C.byref(plan)) == -1
+    assert b"batch" in L.gli_last_error()
+    L.gli_clear_error()
+    # still usable
+    _lib.check(L.gli_eri_values_device(P, len(c), g.ctypes.data_as(_abi.P_glong), d.ptr, n, 1e-16, 0, dv.ptr), "values")
+    assert np.array_equal(dv.to_numpy(np.float64, n), eri_values_device(c, sym(3), d.to_numpy(np.int64, n), 1e-16, 0))
