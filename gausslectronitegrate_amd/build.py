@@ -44,6 +44,28 @@ def sources():
     return hip, c
 
 
+HDF5_HEADERS = ("hdf5_util.h", "gli_hdf5.h")   # exported by libgli_amd_hdf5.so instead
+
+
+def exported_symbols():
+    """every function and table declared in include/*.h (but the HDF5 headers): the library's
+    whole dynamic interface; everything else (kernels, C++ helpers) stays local"""
+    import re
+    names = set()
+    for h in sorted(os.listdir(INCLUDE)):
+        if h in HDF5_HEADERS or not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"static inline[^{]*\{.*?\n\}", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            if m.group(1) not in ("if", "for", "while", "return", "sizeof"):
+                names.add(m.group(1))
+        for m in re.finditer(r"^extern const [^;]*?\b(\w+)\s*\[", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIB_DIR, exist_ok=True)
@@ -65,9 +87,12 @@ def build(verbose=False):
             if o.strip():
                 print(o)
     objs = [o for _, o in jobs]
+    vmap = os.path.join(BUILD, "exports.map")
+    with open(vmap, "w") as f:
+        f.write("{\n  global:\n%s  local: *;\n};\n" % "".join("    %s;\n" % n for n in sorted(exported_symbols())))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-Wl,-Bsymbolic", "-Wl,--no-undefined", "-lgomp", "-lm"])
+         ["-Wl,-Bsymbolic", "-Wl,--no-undefined", "-Wl,--version-script=" + vmap, "-lgomp", "-lm"])
     os.replace(tmp, LIB)
     build_hdf5(incs)
     return LIB

@@ -195,7 +195,7 @@ struct ModeArgs {
 	int64_t p_base;
 	int tet_C1;
 };
-constexpr int kGemmShapes = 4;
+constexpr int kGemmShapes = 6;
 bool gemm_shape_fits(int shape, int Cp, int tri, int64_t p_base);
 void mode_product_shaped(const ModeArgs& a, int shape, int grouped);
 void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped);

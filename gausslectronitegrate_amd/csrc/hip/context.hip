@@ -227,7 +227,7 @@ int gli_set_tuning(const char* knob, int value)
 			c.gather_lds = value;
 		}
 		else if (k == "gemm_shape") {
-			GLI_REQUIRE(value >= 0 && value < kGemmShapes, "gemm_shape must be 0..3");
+			GLI_REQUIRE(value >= 0 && value < kGemmShapes, "gemm_shape must be 0..5");
 			c.gemm_shape = value;
 		}
 		else if (k == "gather_order") {

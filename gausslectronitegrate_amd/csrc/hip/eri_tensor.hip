@@ -121,15 +121,16 @@ __global__ void k_kernel_tensor(const double* __restrict__ T, int M, int Mp, dou
 // Tiles entirely outside the needed region are not computed (never read later).  The skip test
 // needs (p_base + p0) / Cp fixed per tile: Cp and p_base multiples of BM.
 
-constexpr int GBK = 16;
+constexpr int GBK = 16;   // mu slice of the FP32 comparison kernel
 constexpr int KEY_ALIGN = 64;   // key columns are padded to a multiple of this
 
-// tile shapes (kGemmShapes of them): 0: 64x64, 1: 128x64, 2: 64x128, 3: 128x128
-__host__ __device__ constexpr int shape_bm(int s) { return (s == 1 || s == 3) ? 128 : 64; }
+// tile shapes (kGemmShapes of them), BM x BN output tile and BK-deep mu slices:
+// 0: 64x64x16, 1: 128x64x16, 2: 64x128x16, 3: 128x128x16, 4: 64x64x32, 5: 128x64x32
+__host__ __device__ constexpr int shape_bm(int s) { return (s == 1 || s == 3 || s == 5) ? 128 : 64; }
 __host__ __device__ constexpr int shape_bn(int s) { return (s == 2 || s == 3) ? 128 : 64; }
 
 
-template <int BM, int BN>
+template <int BM, int BN, int GBK>
 __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, double (*As)[BM], double (*Bs)[BN])
 {
 	constexpr int TM = BM / 32, TN = BN / 32;
@@ -228,14 +229,14 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 	}
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int GBK>
 __global__ __launch_bounds__(256) void k_mode_product(ModeArgs a, TileOrder order)
 {
 	__shared__ double As[GBK][BM];
 	__shared__ double Bs[GBK][BN];
 	int rt, ct;
 	if (!order.tile(blockIdx.x, rt, ct)) { return; }
-	mode_product_tile<BM, BN>(a, rt, ct, As, Bs);
+	mode_product_tile<BM, BN, GBK>(a, rt, ct, As, Bs);
 }
 
 // Plain FP32 variant of the mode product (v_mfma_f32_16x16x4f32, FP32 products and FP32
@@ -314,7 +315,7 @@ struct ModeGroup {
 	int n;
 };
 
-template <int BM, int BN>
+template <int BM, int BN, int GBK>
 __global__ __launch_bounds__(256) void k_mode_product_grouped(ModeGroup grp)
 {
 	__shared__ double As[GBK][BM];
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(256) void k_mode_product_grouped(ModeGroup grp)
 	while (g + 1 < grp.n && (int)blockIdx.x >= grp.block_begin[g + 1]) { ++g; }
 	int rt, ct;
 	if (!grp.order[g].tile((int)blockIdx.x - grp.block_begin[g], rt, ct)) { return; }
-	mode_product_tile<BM, BN>(grp.a[g], rt, ct, As, Bs);
+	mode_product_tile<BM, BN, GBK>(grp.a[g], rt, ct, As, Bs);
 }
 
 // ---------------------------------------------------------------------------
@@ -970,10 +971,10 @@ static TileOrder tile_order_for(int shape, int64_t P, int C, int grouped)
 	return TileOrder{(int)ceil_div(P, shape_bm(shape)), (int)ceil_div(C, shape_bn(shape)), grouped};
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int BK>
 static void launch_mode(const ModeArgs& a, const TileOrder& o, hipStream_t st)
 {
-	k_mode_product<BM, BN><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o);
+	k_mode_product<BM, BN, BK><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o);
 }
 
 void mode_product_shaped(const ModeArgs& a, int shape, int grouped)
@@ -985,10 +986,12 @@ void mode_product_shaped(const ModeArgs& a, int shape, int grouped)
 	const TileOrder o = tile_order_for(shape, a.P, a.C, grouped);
 	hipStream_t st = ctx().stream;
 	switch (shape) {
-		case 1: launch_mode<128, 64>(a, o, st); break;
-		case 2: launch_mode<64, 128>(a, o, st); break;
-		case 3: launch_mode<128, 128>(a, o, st); break;
-		default: launch_mode<64, 64>(a, o, st); break;
+		case 1: launch_mode<128, 64, 16>(a, o, st); break;
+		case 2: launch_mode<64, 128, 16>(a, o, st); break;
+		case 3: launch_mode<128, 128, 16>(a, o, st); break;
+		case 4: launch_mode<64, 64, 32>(a, o, st); break;
+		case 5: launch_mode<128, 64, 32>(a, o, st); break;
+		default: launch_mode<64, 64, 16>(a, o, st); break;
 	}
 	GLI_HIP(hipGetLastError());
 }
@@ -1033,10 +1036,12 @@ void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped)
 	if (blocks == 0) { return; }
 	hipStream_t st = ctx().stream;
 	switch (shape) {
-		case 1: k_mode_product_grouped<128, 64><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
-		case 2: k_mode_product_grouped<64, 128><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
-		case 3: k_mode_product_grouped<128, 128><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
-		default: k_mode_product_grouped<64, 64><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		case 1: k_mode_product_grouped<128, 64, 16><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		case 2: k_mode_product_grouped<64, 128, 16><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		case 3: k_mode_product_grouped<128, 128, 16><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		case 4: k_mode_product_grouped<64, 64, 32><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		case 5: k_mode_product_grouped<128, 64, 32><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
+		default: k_mode_product_grouped<64, 64, 16><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
 	}
 	GLI_HIP(hipGetLastError());
 }

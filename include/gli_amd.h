@@ -40,7 +40,8 @@ int gli_synchronize(void);
 
 /* Kernel tuning knobs (set per workload by an autotuner; the defaults are tuned for the 11^3 grid):
  *   "tile_order"  output-tile order of the mode products: 0 row tiles fastest (default), 1 XCD-grouped
- *   "gemm_shape"  FP64 mode-product tile: 0 64x64 (default), 1 128x64, 2 64x128, 3 128x128 (a shape
+ *   "gemm_shape"  FP64 mode-product tile (rows x columns x mu slice): 0 64x64x16 (default), 1 128x64x16,
+ *                 2 64x128x16, 3 128x128x16, 4 64x64x32, 5 128x64x32 (a shape
  *                 that does not divide the key padding of a triangular product falls back to 0)
  *   "gather_ept"  elements per lane in the per-element gather: 2, 4, 8 (default) or 16
  *   "gather_lds"  1 (default): each gather workgroup copies the key tables into LDS first; 0: global reads

@@ -359,11 +359,11 @@ def test_tuning_variants_bit_identical(L):
     rng = np.random.default_rng(5)
     shuffled = idx[rng.permutation(n)][: n - 77]
     knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1)), (b"gather_ept", (2, 4, 8, 16)), (b"gather_lds", (0, 1)),
-             (b"gemm_shape", (0, 3))]
+             (b"gemm_shape", (0, 3, 4, 5))]
     saved = {k: L.gli_get_tuning(k) for k, _ in knobs}
     try:
         ref, ref_s = {}, {}
-        for to, go, ept, lds, shape in itertools.product((0, 1), (0, 1), (2, 4, 8, 16), (0, 1), (0, 3)):
+        for to, go, ept, lds, shape in itertools.product((0, 1), (0, 1), (2, 4, 8, 16), (0, 1), (0, 3, 4, 5)):
             if (to + go + ept + lds + shape) % 2 and ept not in (4, 16):
                 continue   # a spread of the combinations keeps the test short; every value of every knob is used
             for k, v in zip((b"tile_order", b"gather_order", b"gather_ept", b"gather_lds", b"gemm_shape"),
