@@ -132,14 +132,29 @@ def cpu_baseline(c, n, all_idx, tol, target_s, seed, label):
     run(all_idx[probe_pos])
     per = (time.perf_counter() - t0) / len(probe_pos)
     count = int(min(len(all_idx), max(len(probe_pos), target_s / max(per, 1e-9))))
+    count = max(count, 8)
     pos = np.sort(rng.choice(len(all_idx), size=count, replace=False))
-    sub = np.ascontiguousarray(all_idx[pos])
-    t0 = time.perf_counter()
-    vals = run(sub)
-    dt = time.perf_counter() - t0
-    return {"value": count / dt, "unit": "elements/s", "cores": threads, "kind": kind,
-            "sample": "%d seeded uniformly drawn symmetry-unique %s four-indices (K=%d, seed %d, tol %g), %.1f s"
-                      % (count, label, len(c), seed, tol, dt)}, pos, vals
+    # 8 sub-batches (a seeded split of the sample) for a confidence interval of the rate
+    part = rng.permutation(count) % 8
+    vals = np.empty(count)
+    dt = 0.0
+    rates = []
+    for b in range(8):
+        sel = np.nonzero(part == b)[0]
+        sub = np.ascontiguousarray(all_idx[pos[sel]])
+        t0 = time.perf_counter()
+        vals[sel] = run(sub)
+        d = time.perf_counter() - t0
+        dt += d
+        rates.append(len(sel) / d)
+    rates = np.array(rates)
+    half = 2.365 * rates.std(ddof=1) / np.sqrt(8)   # Student t, 7 degrees of freedom, 95%
+    rate = count / dt
+    return {"value": rate, "unit": "elements/s", "cores": threads, "kind": kind,
+            "ci95": [float(rates.mean() - half), float(rates.mean() + half)],
+            "full_set_s": len(all_idx) / rate,
+            "sample": "%d seeded uniformly drawn symmetry-unique %s four-indices (K=%d, seed %d, tol %g) in 8 "
+                      "sub-batches, %.1f s" % (count, label, len(c), seed, tol, dt)}, pos, vals
 
 
 # grids of the mixed-size batch: (n, K, coefficient seed, spacing a in bohr); outputs are scaled by 1/a
@@ -342,7 +357,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-16)
     ap.add_argument("--method", choices=sorted(METHODS), default="tensor")
-    ap.add_argument("--direct-sample", type=int, default=16384)
+    ap.add_argument("--direct-sample", type=int, default=65536)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--asymmetric", action="store_true",
@@ -444,12 +459,19 @@ def main():
     L.gli_synchronize()
     torch.cuda.synchronize()
     direct_ms = e0.elapsed_time(e1)
+    # flops of the same run, counted on the device in a separate pass
+    _lib.check(L.gli_enable_direct_flop_count(1), "direct flop count")
+    eri_step(L, c, g, ds_idx.ptr, len(s_idx), args.tol, 2, ds_val.ptr)
+    w_ref, w_exe = C.c_double(), C.c_double()
+    _lib.check(L.gli_direct_flops(C.byref(w_ref), C.byref(w_exe)), "direct flops")
+    L.gli_enable_direct_flop_count(0)
     direct_vals = np.empty(len(s_idx))
     _lib.check(L.gli_memcpy(direct_vals.ctypes.data, ds_val.ptr, 8 * len(s_idx), 1), "copy direct")
     err_abs = float(np.max(np.abs(vals[samp] - direct_vals)))
     err_rel = float(np.max(np.abs(vals[samp] - direct_vals) / np.maximum(np.abs(direct_vals), 1e-300)))
     direct_rate = len(s_idx) / (direct_ms / 1e3)
-    log("direct FP64 sample: %d elements in %.2f ms; max abs err %.3g" % (len(s_idx), direct_ms, err_abs))
+    log("direct FP64 sample: %d elements in %.2f ms (%.3g reference-form TF/s, %.3g executed TF/s); max abs err %.3g"
+        % (len(s_idx), direct_ms, w_ref.value / direct_ms / 1e9, w_exe.value / direct_ms / 1e9, err_abs))
 
     # other methods on the same workload: time and error over every element
     compare = []
@@ -533,7 +555,12 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms": phases,
         "enumeration_ms": enum_ms,
-        "fp64_direct": {"elements_per_s": direct_rate, "sample": len(s_idx), "ms": direct_ms},
+        "fp64_direct": {"elements_per_s": direct_rate, "sample": len(s_idx), "ms": direct_ms,
+                        "reference_form_flops_per_element": w_ref.value / len(s_idx),
+                        "executed_flops_per_element": w_exe.value / len(s_idx),
+                        "achieved_tflops_reference_form": w_ref.value / (direct_ms / 1e3) / 1e12,
+                        "achieved_tflops_executed": w_exe.value / (direct_ms / 1e3) / 1e12,
+                        "frac_fp64_vector_peak_executed": w_exe.value / (direct_ms / 1e3) / 1e12 / FP64_PEAK_TFS},
         "max_abs_err_vs_fp64_direct": err_abs,
         "max_rel_err_vs_fp64_direct": err_rel,
         "max_abs_err_vs_cpu_reference": cpu_err,

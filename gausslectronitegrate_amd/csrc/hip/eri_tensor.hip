@@ -368,23 +368,19 @@ __device__ inline int axis_key(const int* __restrict__ lut, int R, int pi, int p
 	return lut[(a * R + b) * (4 * R - 3) + cd + 2 * (R - 1)];
 }
 
-__device__ inline void element_keys(const GridDev& g, const int* __restrict__ lut, const int* __restrict__ coords,
-                                    int64_t t, int& k1, int& k2, int& k3)
+// key id of each axis (x, y, z) of a four-index, and the axis shift pair (|s01|, |s23|) of each
+__device__ inline void element_axis_keys(const GridDev& g, const int* __restrict__ lut, const int* __restrict__ coords,
+                                         int64_t t, int k[3], int ab[3])
 {
-	int i, j, k, l;
-	decode_four(g, t, i, j, k, l);
-	const int ci = coords[i], cj = coords[j], ck = coords[k], cl = coords[l];
-	int kk[3];
+	int i, j, kk, l;
+	decode_four(g, t, i, j, kk, l);
+	const int ci = coords[i], cj = coords[j], ck = coords[kk], cl = coords[l];
 	for (int a = 0; a < 3; ++a) {
 		const int sh = 10 * a;
-		kk[a] = axis_key(lut, g.R, (ci >> sh) & 1023, (cj >> sh) & 1023, (ck >> sh) & 1023, (cl >> sh) & 1023);
+		const int pi = (ci >> sh) & 1023, pj = (cj >> sh) & 1023, pk = (ck >> sh) & 1023, pl = (cl >> sh) & 1023;
+		k[a] = axis_key(lut, g.R, pi, pj, pk, pl);
+		ab[a] = abs(pi - pj) * g.R + abs(pk - pl);
 	}
-	// sort ascending (V is symmetric in its three arguments)
-	int x = kk[0], y = kk[1], z = kk[2], s;
-	if (x > y) { s = x; x = y; y = s; }
-	if (y > z) { s = y; y = z; z = s; }
-	if (x > y) { s = x; x = y; y = s; }
-	k1 = x; k2 = y; k3 = z;
 }
 
 // quotient a / d for 0 <= a < 2^24 from a float reciprocal and one correction step
@@ -658,63 +654,161 @@ __global__ __launch_bounds__(256) void k_gather_grouped(const GatherDesc* __rest
 	gather_block<GATHER_EPT, kLds>(d, first + (threadIdx.x >> 6) * (64 * GATHER_EPT));
 }
 
-// per-element FP64 sum over the folded columns, no reuse across elements:
-// value = sum_{m1,m2} B1[m1] B2[m2] sum_{m3} B3[m3] T[m1^2 + m2^2 + m3^2]
-__global__ __launch_bounds__(256) void k_eri_direct(const int64_t* __restrict__ idx, int64_t num, GridDev g,
-	const int* __restrict__ lut, const int* __restrict__ coords, const double* __restrict__ B, int Cp, int M, int Mp, const double* __restrict__ T,
-	double* __restrict__ out)
+// ---------------------------------------------------------------------------
+// per-element FP64 path: the reference's binned sum (eri_integrals.c:608-669), one workgroup per
+// element, no reuse across elements.  On the folded columns B[.][kx], B[.][ky], B[.][kz]
+// (t -> |t| is an exact regrouping of the reference's terms):
+//   C[rho] = sum_{m1^2 + m2^2 = rho} Bx[m1] By[m2]                        (x, y bins)
+//   value  = sum_{rho: C[rho] != 0} C[rho] * sum_{m3} Bz[m3] T[rho + m3^2]
+// Bins are computed per rho from a table of its representations m1^2 + m2^2 = rho, so no bin is
+// written twice and the result does not depend on scheduling; the nonzero bins of a chunk of rho
+// are compacted into LDS and the z sums run with lanes over consecutive bins, so the table reads
+// of a wave fall on a few lines.
+
+constexpr int DIRECT_THREADS = 256;
+constexpr int DIRECT_CHUNK = 2048;   // rho values per compaction round
+
+struct DirectCount {
+	unsigned long long* sums;      // [0] reference-form flops, [1] executed flops; nullptr: not counted
+	const int* nnz_rows;           // nonzeros of P[a][b] per shift pair (a * R + b)
+};
+
+__device__ inline double wave_sum(double v)
+{
+	for (int o = 32; o > 0; o >>= 1) { v += __shfl_xor(v, o, 64); }
+	return v;
+}
+
+__global__ __launch_bounds__(DIRECT_THREADS) void k_eri_direct(const int64_t* __restrict__ idx, int64_t num, GridDev g,
+	const int* __restrict__ lut, const int* __restrict__ coords, const double* __restrict__ B, int Cp, int M, int Mp,
+	const double* __restrict__ T, const int* __restrict__ rep_off, const uint32_t* __restrict__ rep_list,
+	DirectCount cnt, double* __restrict__ out)
 {
 	extern __shared__ double smem[];
-	double* col = smem;               // 3 x Mp folded columns
-	double* red = smem + 3 * Mp;      // 256 partial sums
+	double* col = smem;                                   // 3 x Mp folded columns
+	double* list_c = smem + 3 * Mp;                       // compacted nonzero bins: values
+	int* list_rho = (int*)(list_c + DIRECT_CHUNK);        //                          and rho
 	__shared__ int win[3][2];
-	const int tid = threadIdx.x;
+	__shared__ int wave_tot[DIRECT_THREADS / 64];
+	__shared__ double wave_acc[DIRECT_THREADS / 64];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	constexpr int J = DIRECT_CHUNK / DIRECT_THREADS;
 	for (int64_t e = blockIdx.x; e < num; e += gridDim.x) {
 		const int64_t t = idx[e];
 		if (t < 0 || t >= g.N4) {
 			if (tid == 0) { out[e] = __builtin_nan(""); }
 			continue;
 		}
-		int k[3];
-		element_keys(g, lut, coords, t, k[0], k[1], k[2]);
-		for (int a = 0; a < 3; ++a) {
-			for (int m = tid; m < Mp; m += blockDim.x) { col[a * Mp + m] = (m <= M) ? B[(int64_t)m * Cp + k[a]] : 0.0; }
-		}
-		__syncthreads();
+		int k[3], ab[3];
+		element_axis_keys(g, lut, coords, t, k, ab);
 		if (tid < 3) {
-			int lo = M + 1, hi = -1;
-			for (int m = 0; m <= M; ++m) {
-				if (col[tid * Mp + m] != 0) { lo = min(lo, m); hi = m; }
-			}
-			win[tid][0] = lo;
-			win[tid][1] = hi;
+			win[tid][0] = M + 1;
+			win[tid][1] = -1;
 		}
 		__syncthreads();
-		const int n0 = win[0][1] - win[0][0] + 1, n1 = win[1][1] - win[1][0] + 1;
+		for (int a = 0; a < 3; ++a) {
+			for (int m = tid; m < Mp; m += DIRECT_THREADS) {
+				const double v = (m <= M) ? B[(int64_t)m * Cp + k[a]] : 0.0;
+				col[a * Mp + m] = v;
+				if (v != 0) {
+					atomicMin(&win[a][0], m);
+					atomicMax(&win[a][1], m);
+				}
+			}
+		}
+		__syncthreads();
+		const int lo0 = win[0][0], hi0 = win[0][1], lo1 = win[1][0], hi1 = win[1][1], lo2 = win[2][0], hi2 = win[2][1];
 		double acc = 0;
-		if (n0 > 0 && n1 > 0 && win[2][1] >= win[2][0]) {
-			const double* c0 = col;
-			const double* c1 = col + Mp;
-			const double* c2 = col + 2 * Mp;
-			for (int pq = tid; pq < n0 * n1; pq += blockDim.x) {
-				const int m1 = win[0][0] + pq / n1, m2 = win[1][0] + pq % n1;
-				const double w = c0[m1] * c1[m2];
-				if (w == 0) { continue; }
-				const int base = m1 * m1 + m2 * m2;
-				double s = 0;
-				for (int m3 = win[2][0]; m3 <= win[2][1]; ++m3) { s += c2[m3] * T[base + m3 * m3]; }
-				acc += w * s;
+		int my_bins = 0;
+		if (hi0 >= lo0 && hi1 >= lo1 && hi2 >= lo2) {
+			const double* cx = col;
+			const double* cy = col + Mp;
+			const double* cz = col + 2 * Mp;
+			const int rho_lo = lo0 * lo0 + lo1 * lo1, rho_hi = hi0 * hi0 + hi1 * hi1;
+			for (int base = rho_lo; base <= rho_hi; base += DIRECT_CHUNK) {
+				// bins of this chunk: thread tid takes rho = base + 64 * (J * wave + j) + lane
+				double cv[J];
+				uint64_t mask[J];
+				int count = 0;
+#pragma unroll
+				for (int j = 0; j < J; ++j) {
+					const int rho = base + 64 * (J * wave + j) + lane;
+					double c = 0;
+					if (rho <= rho_hi) {
+						const int r0 = rep_off[rho], r1 = rep_off[rho + 1];
+						for (int r = r0; r < r1; ++r) {
+							const uint32_t pq = rep_list[r];
+							const int m1 = (int)(pq & 0xffffu), m2 = (int)(pq >> 16);
+							if (m1 >= lo0 && m1 <= hi0 && m2 >= lo1 && m2 <= hi1) { c += cx[m1] * cy[m2]; }
+						}
+					}
+					cv[j] = c;
+					mask[j] = __ballot(c != 0);
+					count += __popcll(mask[j]);
+				}
+				if (lane == 0) { wave_tot[wave] = count; }
+				__syncthreads();
+				int pos = 0, total = 0;
+				for (int w = 0; w < DIRECT_THREADS / 64; ++w) {
+					pos += (w < wave) ? wave_tot[w] : 0;
+					total += wave_tot[w];
+				}
+#pragma unroll
+				for (int j = 0; j < J; ++j) {
+					const uint64_t below = mask[j] & ((1ull << lane) - 1ull);
+					if (cv[j] != 0) {
+						const int at = pos + __popcll(below);
+						list_c[at] = cv[j];
+						list_rho[at] = base + 64 * (J * wave + j) + lane;
+					}
+					pos += __popcll(mask[j]);
+				}
+				__syncthreads();
+				my_bins += (tid == 0) ? total : 0;
+				// z sums over the compacted bins, lanes over consecutive bins
+				// (zero z entries inside the window add exact zeros: T is finite)
+				for (int q = tid; q < total; q += DIRECT_THREADS) {
+					const double* Tr = T + list_rho[q];
+					double s = 0;
+#pragma unroll 4
+					for (int m3 = lo2; m3 <= hi2; ++m3) { s += cz[m3] * Tr[m3 * m3]; }
+					acc += s * list_c[q];
+				}
+				__syncthreads();
 			}
 		}
-		red[tid] = acc;
+		acc = wave_sum(acc);
+		if (lane == 0) { wave_acc[wave] = acc; }
 		__syncthreads();
-		for (int w = 128; w > 0; w >>= 1) {
-			if (tid < w) { red[tid] += red[tid + w]; }
-			__syncthreads();
+		if (tid == 0) {
+			double v = 0;
+			for (int w = 0; w < DIRECT_THREADS / 64; ++w) { v += wave_acc[w]; }
+			out[e] = v;
+			if (cnt.sums != nullptr && hi0 >= lo0 && hi1 >= lo1 && hi2 >= lo2) {
+				const unsigned long long nx = cnt.nnz_rows[ab[0]], ny = cnt.nnz_rows[ab[1]], nz = cnt.nnz_rows[ab[2]];
+				// executed: every in-window (m1, m2) pair, and per nonzero bin the z window + 1
+				const unsigned long long R = (unsigned long long)my_bins;
+				const unsigned long long wx = hi0 - lo0 + 1, wy = hi1 - lo1 + 1, ez = hi2 - lo2 + 1;
+				atomicAdd(&cnt.sums[0], 2ull * nx * ny + 2ull * R * (nz + 1ull));
+				atomicAdd(&cnt.sums[1], 2ull * wx * wy + 2ull * R * (ez + 1ull));
+			}
 		}
-		if (tid == 0) { out[e] = red[0]; }
 		__syncthreads();
 	}
+}
+
+// nonzeros of each P[a][b] row (the reference's n per axis: entries with |P| > tol)
+__global__ void k_nnz_rows(const double* __restrict__ P, int rows, int LP, int* __restrict__ nnz)
+{
+	const int r = blockIdx.x;
+	if (r >= rows) { return; }
+	int c = 0;
+	for (int t = threadIdx.x; t < LP; t += blockDim.x) { c += (P[(int64_t)r * LP + t] != 0) ? 1 : 0; }
+	for (int o = 32; o > 0; o >>= 1) { c += __shfl_xor(c, o, 64); }
+	__shared__ int part[4];
+	if ((threadIdx.x & 63) == 0) { part[threadIdx.x >> 6] = c; }
+	__syncthreads();
+	if (threadIdx.x == 0) { nnz[r] = part[0] + part[1] + part[2] + part[3]; }
 }
 
 // ---------------------------------------------------------------------------
@@ -791,6 +885,14 @@ struct EriWorkspace {
 	DevBuf<int> coords;
 	DevBuf<double> T3;
 	int T3_M = -1;
+	// per-element path: representations m1^2 + m2^2 = rho for m1, m2 <= rep_M, grouped by rho
+	DevBuf<int> rep_off;
+	DevBuf<uint32_t> rep_list;
+	int rep_M = -1;
+	// per-element path flop counters
+	bool count_direct = false;
+	DevBuf<unsigned long long> direct_sums;
+	DevBuf<int> nnz_rows;
 };
 
 static EriWorkspace& eri_ws()
@@ -1046,6 +1148,30 @@ void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped)
 	GLI_HIP(hipGetLastError());
 }
 
+// representations rho = m1^2 + m2^2 (0 <= m1, m2 <= M), grouped by rho with m1 then m2 ascending:
+// rep_list[rep_off[rho] .. rep_off[rho + 1]) = m1 | m2 << 16
+static void direct_representations(EriWorkspace& w, int M)
+{
+	if (w.rep_M == M) { return; }
+	const int64_t nrho = 2 * (int64_t)M * M + 1;
+	std::vector<int> off((size_t)nrho + 1, 0);
+	for (int a = 0; a <= M; ++a) {
+		for (int b = 0; b <= M; ++b) { off[(size_t)(a * a + b * b) + 1] += 1; }
+	}
+	for (int64_t r = 0; r < nrho; ++r) { off[(size_t)r + 1] += off[(size_t)r]; }
+	std::vector<uint32_t> list((size_t)(M + 1) * (M + 1));
+	std::vector<int> fill(off.begin(), off.end() - 1);
+	for (int a = 0; a <= M; ++a) {
+		for (int b = 0; b <= M; ++b) { list[(size_t)fill[(size_t)(a * a + b * b)]++] = (uint32_t)a | ((uint32_t)b << 16); }
+	}
+	GLI_HIP(hipStreamSynchronize(ctx().stream));   // the previous tables may still be read
+	w.rep_off.alloc(off.size());
+	w.rep_list.alloc(list.size());
+	GLI_HIP(hipMemcpy(w.rep_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+	GLI_HIP(hipMemcpy(w.rep_list.p, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+	w.rep_M = M;
+}
+
 // values of device-resident four-indices into device-resident values
 static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_idx, int64_t num,
                            double tol, int method, double* d_vals)
@@ -1071,9 +1197,19 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	if (method == GLI_ERI_DIRECT_FP64) {
 		w.coords.reserve((size_t)s.g.N);
 		k_point_coords<<<(unsigned)ceil_div(s.g.N, 256), 256, 0, c.stream>>>(s.g.n[0], s.g.n[1], s.g.n[2], w.coords.p);
-		const int64_t blocks = std::min<int64_t>(num, 65536);
-		const size_t shm = (3 * (size_t)s.Mp + 256) * sizeof(double);
-		k_eri_direct<<<(unsigned)blocks, 256, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.coords.p, w.B.p, s.Cp, s.M, s.Mp, s.T, d_vals);
+		direct_representations(w, s.M);
+		DirectCount cnt{nullptr, nullptr};
+		if (w.count_direct) {
+			const int rows = s.R * s.R;
+			w.nnz_rows.reserve((size_t)rows);
+			k_nnz_rows<<<(unsigned)rows, 256, 0, c.stream>>>(w.P.p, rows, 4 * s.K - 3, w.nnz_rows.p);
+			cnt = DirectCount{w.direct_sums.p, w.nnz_rows.p};
+		}
+		const int64_t blocks = std::min<int64_t>(num, 8192);
+		const size_t shm = 3 * (size_t)s.Mp * sizeof(double) + DIRECT_CHUNK * (sizeof(double) + sizeof(int));
+		GLI_REQUIRE(shm <= 65536, "grid too long for the per-element path: 6(n-1) + 2(K-1) must stay below 1700");
+		k_eri_direct<<<(unsigned)blocks, DIRECT_THREADS, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.coords.p, w.B.p,
+			s.Cp, s.M, s.Mp, s.T, w.rep_off.p, w.rep_list.p, cnt, d_vals);
 		GLI_HIP(hipGetLastError());
 		phase_mark("direct");
 		return;
@@ -1645,6 +1781,46 @@ void gli_release_workspace(void)
 	w.coeffs.release(); w.f.release(); w.P.release(); w.B.release();
 	w.Y1.release(); w.Y2.release(); w.V.release(); w.T3.release();
 	w.T3_M = -1;
+	w.rep_off.release(); w.rep_list.release();
+	w.rep_M = -1;
+}
+
+int gli_enable_direct_flop_count(int on)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		EriWorkspace& w = eri_ws();
+		if (on) {
+			w.direct_sums.reserve(2);
+			GLI_HIP(hipMemsetAsync(w.direct_sums.p, 0, 2 * sizeof(unsigned long long), c.stream));
+		}
+		w.count_direct = (on != 0);
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_direct_flops(double* reference_form, double* executed)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		EriWorkspace& w = eri_ws();
+		GLI_REQUIRE(w.count_direct, "flop counting of the per-element path is not enabled");
+		unsigned long long h[2];
+		GLI_HIP(hipMemcpyAsync(h, w.direct_sums.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+		GLI_HIP(hipMemsetAsync(w.direct_sums.p, 0, sizeof(h), c.stream));
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		if (reference_form) { *reference_form = (double)h[0]; }
+		if (executed) { *executed = (double)h[1]; }
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
 }
 
 }  // extern "C"

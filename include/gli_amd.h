@@ -58,6 +58,16 @@ int gli_get_tuning(const char* knob);
 int gli_enable_phase_timing(int on);
 int gli_phase_times(double* ms, const char** names, int max);
 
+/* FP64 flops of the per-element path (GLI_ERI_DIRECT_FP64).  While enabled, each evaluation adds per
+ * element, on the device:
+ *   reference_form  2 n_x n_y + 2 R_xy (n_z + 1), the terms the reference's loops execute
+ *                   (eri_integrals.c:624-669: n_a entries with |P| > tol per axis, R_xy nonzero bins);
+ *   executed        the same count on the folded columns this path sums over (every pair of the x and
+ *                   y windows, and per nonzero bin the z window + 1).
+ * gli_direct_flops returns the sums since counting was enabled or last queried (synchronises). */
+int gli_enable_direct_flop_count(int on);
+int gli_direct_flops(double* reference_form, double* executed);
+
 /* device memory helpers; kind: 0 host->device, 1 device->host, 2 device->device */
 int gli_device_malloc(void** p, int64_t bytes);
 void gli_device_free(void* p);

@@ -300,6 +300,55 @@ void orc_eri_sparse(const double* c, glong K, const glong* g, const glong* idx, 
 	free(T); free(pln); free(pst); free(prod);
 }
 
+/* flops the reference's loops execute per element (eri_integrals.c:624-669): 2 n_x n_y for the bins
+ * (n_a = entries with |P| > tol), then per nonzero bin 2 n_z for the z sum and 2 for the scaling and
+ * the accumulation: W = 2 n_x n_y + 2 R_xy (n_z + 1) */
+void orc_eri_flops(const double* c, glong K, const glong* g, const glong* idx, glong num, double tol, double* out)
+{
+	const glong R = lmx(lmx(g[1], g[3]), g[5]);
+	const glong s0 = -R + 1, ns = 2 * R - 1, LP = 4 * K - 3;
+	double* prod = malloc(ns * ns * LP * sizeof(double));
+	glong* pst = malloc(ns * ns * sizeof(glong));
+	glong* pln = malloc(ns * ns * sizeof(glong));
+	orc_eri_products(c, K, s0, ns, tol, prod, pst, pln);
+	const glong N = g_num(g);
+	const glong M = 6 * (R - 1) + 2 * (K - 1);
+	const glong cap = 2 * lsq(M) + 2;
+	#pragma omp parallel
+	{
+		double* bins = malloc(cap * sizeof(double));
+		#pragma omp for schedule(static)
+		for (glong n = 0; n < num; ++n) {
+			glong r = idx[n];
+			const glong il = r % N; r /= N;
+			const glong ik = r % N; r /= N;
+			const glong ij = r % N; r /= N;
+			const glong ii = r;
+			glong pi[3], pj[3], pk[3], pl[3];
+			g_pt(g, ii, pi); g_pt(g, ij, pj); g_pt(g, ik, pk); g_pt(g, il, pl);
+			glong cd6[3], st[3], nm[3], cnt[3];
+			const double* pr[3];
+			for (int a = 0; a < 3; ++a) {
+				cd6[a] = 3 * ((pi[a] + pj[a]) - (pk[a] + pl[a]));
+				const glong kl = ((pi[a] - pj[a]) - s0) * ns + ((pk[a] - pl[a]) - s0);
+				pr[a] = &prod[kl * LP];
+				st[a] = pst[kl];
+				nm[a] = pln[kl];
+				cnt[a] = 0;
+				for (glong q = 0; q < nm[a]; ++q) { cnt[a] += fabs(pr[a][q]) > tol; }
+			}
+			eri_bins(cd6, pr, st, nm, tol, bins);
+			const glong nb = lmx(lsq(cd6[0] + st[0]), lsq(cd6[0] + st[0] + nm[0] - 1))
+			               + lmx(lsq(cd6[1] + st[1]), lsq(cd6[1] + st[1] + nm[1] - 1)) + 1;
+			glong rxy = 0;
+			for (glong q = 0; q < nb; ++q) { rxy += bins[q] != 0; }
+			out[n] = 2.0 * (double)cnt[0] * (double)cnt[1] + 2.0 * (double)rxy * (double)(cnt[2] + 1);
+		}
+		free(bins);
+	}
+	free(pln); free(pst); free(prod);
+}
+
 /* one ERI for four points pts[4][3] (eri_integrals.c:133-232) */
 double orc_eri_single(const double* c, glong K, const glong* pts, double tol)
 {

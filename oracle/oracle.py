@@ -37,6 +37,7 @@ def lib():
         L.orc_erida_table_len.argtypes = [_i64, _i64]
         L.orc_erida_table_len.restype = _i64
         L.orc_eri_sparse.argtypes = [_pd, _i64, _pl, _pl, _i64, C.c_double, _pd]
+        L.orc_eri_flops.argtypes = [_pd, _i64, _pl, _pl, _i64, C.c_double, _pd]
         L.orc_eri_single.argtypes = [_pd, _i64, _pl, C.c_double]
         L.orc_eri_single.restype = C.c_double
         L.orc_min_orbit_eri.argtypes = [_i64, _pl, _i64, _i64, _i64, _i64]
@@ -132,6 +133,15 @@ def eri_sparse(c, grid, indices, tol):
     idx = np.ascontiguousarray(indices, dtype=np.int64)
     out = np.zeros(len(idx))
     lib().orc_eri_sparse(c, len(c), _g(grid), idx, len(idx), tol, out)
+    return out
+
+
+def eri_flops(c, grid, indices, tol):
+    """per-element flops of the reference's binned loops, 2 n_x n_y + 2 R_xy (n_z + 1)"""
+    c = _c(c)
+    idx = np.ascontiguousarray(indices, dtype=np.int64)
+    out = np.zeros(len(idx))
+    lib().orc_eri_flops(c, len(c), _g(grid), idx, len(idx), tol, out)
     return out
 
 

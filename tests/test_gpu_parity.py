@@ -197,10 +197,33 @@ def test_eri_arbitrary_grid_and_indices(L, orc):
     rng = np.random.default_rng(4)
     idx = np.unique(rng.integers(0, N ** 4, size=300))
     ref = orc.eri_sparse(c, grid, idx, 1e-10)
-    got = eri_values_device(c, grid, idx, 1e-10, 0)
-    assert np.max(np.abs(got - ref)) <= 1e-13
-    bad = eri_values_device(c, grid, np.array([-1, N ** 4], dtype=np.int64), 1e-10, 0)
-    assert np.all(np.isnan(bad))
+    for method in (0, 2):
+        got = eri_values_device(c, grid, idx, 1e-10, method)
+        assert np.max(np.abs(got - ref)) <= 1e-13
+        bad = eri_values_device(c, grid, np.array([-1, N ** 4], dtype=np.int64), 1e-10, method)
+        assert np.all(np.isnan(bad))
+
+
+@pytest.mark.parametrize("n,K,tol", [(5, 29, 1e-16), (7, 43, 1e-12)])
+def test_direct_flop_count_matches_oracle(L, orc, n, K, tol):
+    """the per-element path's device counter of the reference's loop flops equals the oracle's count
+    (eri_integrals.c:624-669) on a seeded sample; its values match the oracle's"""
+    say("direct flops", n, K)
+    c = golden("coeffs_K%d_s0.npy" % K)
+    idx = golden("golden_sample_n%d_K%ds0.npz" % (n, K))["eri_indices"]
+    assert L.gli_enable_direct_flop_count(1) == 0
+    try:
+        got = eri_values_device(c, sym(n), idx, tol, 2)
+        ref_form = C.c_double()
+        executed = C.c_double()
+        assert L.gli_direct_flops(C.byref(ref_form), C.byref(executed)) == 0
+    finally:
+        L.gli_enable_direct_flop_count(0)
+    want = orc.eri_flops(c, sym(n), idx, tol)
+    assert ref_form.value == pytest.approx(want.sum(), rel=1e-9)
+    assert 0 < executed.value
+    assert np.max(np.abs(got - orc.eri_sparse(c, sym(n), idx, tol))) <= 1e-13
+    assert L.gli_direct_flops(C.byref(ref_form), C.byref(executed)) == -1   # counting is off again
 
 
 def test_eri_7_full_list_tensor_vs_direct_sample(L, orc):

@@ -261,6 +261,62 @@ def test_eri_brute_force_tiny_K(orc):
     assert v == pytest.approx(brute, rel=1e-12, abs=1e-14)
 
 
+def test_eri_flop_count_matches_instrumented_loops(orc):
+    """orc.eri_flops (the closed count 2 n_x n_y + 2 R_xy (n_z + 1)) against the reference's loops
+    (eri_integrals.c:608-669) run in Python with one count per executed multiply and add; the loop's
+    values also reproduce orc.eri_sparse.  3^3 grid, K = 7, tol 1e-12, a seeded sample."""
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal(7)
+    n = 3
+    grid = orc.sym_grid(n)
+    R = n
+    s0, ns = -R + 1, 2 * R - 1
+    prod, st, ln = orc.eri_products(c, s0, ns, 1e-12)
+    T = orc.eri_table(orc.eri_table_len(R, len(c)))
+    N = n ** 3
+    idx = np.sort(rng.choice(N ** 4, size=40, replace=False)).astype(np.int64)
+    want_w = orc.eri_flops(c, grid, idx, 1e-12)
+    want_v = orc.eri_sparse(c, grid, idx, 1e-12)
+    for e, t in enumerate(idx):
+        t = int(t)
+        ps = []
+        for _ in range(4):
+            ps.append(t % N)
+            t //= N
+        l, k, j, i = ps
+        pts = [((p // (n * n)) - 1, ((p // n) % n) - 1, (p % n) - 1) for p in (i, j, k, l)]
+        arrs, cds = [], []
+        for a in range(3):
+            cds.append(3 * ((pts[0][a] + pts[1][a]) - (pts[2][a] + pts[3][a])))
+            kl = ((pts[0][a] - pts[1][a]) - s0) * ns + ((pts[2][a] - pts[3][a]) - s0)
+            arrs.append([(int(st[kl]) + q, prod[kl][q]) for q in range(int(ln[kl]))])
+        ops = 0
+        bins = {}
+        for tx, px in arrs[0]:
+            if abs(px) <= 1e-12:
+                continue
+            for ty, py in arrs[1]:
+                if abs(py) <= 1e-12:
+                    continue
+                r = (cds[0] + tx) ** 2 + (cds[1] + ty) ** 2
+                bins[r] = bins.get(r, 0.0) + px * py
+                ops += 2
+        val = 0.0
+        for r in sorted(bins):
+            if bins[r] == 0:
+                continue
+            acc = 0.0
+            for tz, pz in arrs[2]:
+                if abs(pz) <= 1e-12:
+                    continue
+                acc += pz * T[r + (cds[2] + tz) ** 2]
+                ops += 2
+            val += acc * bins[r]
+            ops += 2
+        assert ops == want_w[e]
+        assert val == pytest.approx(want_v[e], rel=1e-12, abs=1e-15)
+
+
 # ---------------------------------------------------------------------------
 # ERIDA
 
