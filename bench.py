@@ -29,8 +29,9 @@ from gausslectronitegrate_amd.coeffs import synthetic_coefficients  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
-METHODS = {"tensor": 0, "mixed": 1, "direct": 2, "bf16x2": 3, "fp32": 4}
-DTYPES = {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64", "bf16x2": "bf16x2+f32acc", "fp32": "f32"}
+METHODS = {"tensor": 0, "mixed": 1, "direct": 2, "bf16x2": 3, "fp32": 4, "fp32x2": 5}
+DTYPES = {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64", "bf16x2": "bf16x2+f32acc", "fp32": "f32",
+          "fp32x2": "f32x2+f32acc"}
 
 
 def log(*a):
@@ -364,7 +365,7 @@ def main():
                     help="coefficients without mirror symmetry (general axis-key set, GEMM-dominated)")
     ap.add_argument("--compare", default="mixed",
                     help="comma-separated methods timed beside the main one, each with its error against the main "
-                         "FP64 values over every element (e.g. mixed,bf16x2,fp32)")
+                         "FP64 values over every element (e.g. mixed,bf16x2,fp32,fp32x2)")
     ap.add_argument("--no-compare", action="store_true", help="time no other method")
     ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
                     help="set a kernel tuning knob (gli_set_tuning) before timing; repeatable")

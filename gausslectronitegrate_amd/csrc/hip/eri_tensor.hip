@@ -239,15 +239,21 @@ __global__ __launch_bounds__(256) void k_mode_product(ModeArgs a, TileOrder orde
 	mode_product_tile<BM, BN, GBK>(a, rt, ct, As, Bs);
 }
 
-// Plain FP32 variant of the mode product (v_mfma_f32_16x16x4f32, FP32 products and FP32
-// accumulation over the whole mu range, FP64 output): the comparison path of the precision study,
-// not accurate enough for the 1e-10 target.
+// FP32 variants of the mode product (v_mfma_f32_16x16x4f32, FP32 accumulation over the whole mu
+// range, FP64 output), the comparison paths of the precision study; neither reaches the 1e-10 target.
+//   kSplit = false: operands rounded to FP32, one product per term;
+//   kSplit = true:  operands split x = hi + lo with hi = fl32(x), lo = fl32(x - hi); three products
+//                   hi*hi + hi*lo + lo*hi per term into the same FP32 accumulator, so the operand
+//                   rounding is corrected and the FP32 accumulation error remains.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
+template <bool kSplit>
 __global__ __launch_bounds__(256) void k_mode_product_f32(ModeArgs a, TileOrder order)
 {
 	__shared__ float As[GBK][64];
 	__shared__ float Bs[GBK][64];
+	__shared__ float Al[kSplit ? GBK : 1][64];
+	__shared__ float Bl[kSplit ? GBK : 1][64];
 	int rt, ct;
 	if (!order.tile(blockIdx.x, rt, ct)) { return; }
 	const int tid = threadIdx.x;
@@ -270,14 +276,31 @@ __global__ __launch_bounds__(256) void k_mode_product_f32(ModeArgs a, TileOrder 
 		for (int r = 0; r < (GBK * 64) / 256; ++r) {
 			const int id = tid + 256 * r;
 			const int mu = id / 64, q = id % 64;
-			As[mu][q] = (mu0 + mu < a.Kd && p0 + q < a.P) ? (float)a.X[(int64_t)(mu0 + mu) * a.ldx + p0 + q] : 0.0f;
-			Bs[mu][q] = (mu0 + mu < a.Kd && k0 + q < a.C) ? (float)a.B[(int64_t)(mu0 + mu) * a.ldb + k0 + q] : 0.0f;
+			const double x = (mu0 + mu < a.Kd && p0 + q < a.P) ? a.X[(int64_t)(mu0 + mu) * a.ldx + p0 + q] : 0.0;
+			const double y = (mu0 + mu < a.Kd && k0 + q < a.C) ? a.B[(int64_t)(mu0 + mu) * a.ldb + k0 + q] : 0.0;
+			As[mu][q] = (float)x;
+			Bs[mu][q] = (float)y;
+			if (kSplit) {
+				Al[mu][q] = (float)(x - (double)(float)x);
+				Bl[mu][q] = (float)(y - (double)(float)y);
+			}
 		}
 		__syncthreads();
 		for (int kk = 0; kk < GBK; kk += 4) {
 			float av[2], bv[2];
 			for (int i = 0; i < 2; ++i) { av[i] = As[kk + lane / 16][wm * 32 + i * 16 + lane % 16]; }
 			for (int j = 0; j < 2; ++j) { bv[j] = Bs[kk + lane / 16][wn * 32 + j * 16 + lane % 16]; }
+			if (kSplit) {
+				float al[2], bl[2];
+				for (int i = 0; i < 2; ++i) { al[i] = Al[kk + lane / 16][wm * 32 + i * 16 + lane % 16]; }
+				for (int j = 0; j < 2; ++j) { bl[j] = Bl[kk + lane / 16][wn * 32 + j * 16 + lane % 16]; }
+				for (int i = 0; i < 2; ++i) {
+					for (int j = 0; j < 2; ++j) {
+						acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[i], bv[j], acc[i][j], 0, 0, 0);
+						acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bl[j], acc[i][j], 0, 0, 0);
+					}
+				}
+			}
 			for (int i = 0; i < 2; ++i) {
 				for (int j = 0; j < 2; ++j) { acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[i], bv[j], acc[i][j], 0, 0, 0); }
 			}
@@ -1107,13 +1130,14 @@ void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int6
 	mode_product_shaped(a, shape, tile_order_grouped());
 }
 
-static void mode_product_f32(const ModeArgs& a)
+static void mode_product_f32(const ModeArgs& a, bool split)
 {
 	if (a.P <= 0 || a.C <= 0) { return; }
 	GLI_REQUIRE(!(a.tri_cols || a.tri_rows || a.tet_C1 > 0) || (a.Cp % 64 == 0 && a.p_base % 64 == 0),
 	            "mode-product tile does not fit the key padding");
 	const TileOrder o{(int)ceil_div(a.P, 64), (int)ceil_div(a.C, 64), 0};
-	k_mode_product_f32<<<(unsigned)o.blocks(), 256, 0, ctx().stream>>>(a, o);
+	if (split) { k_mode_product_f32<true><<<(unsigned)o.blocks(), 256, 0, ctx().stream>>>(a, o); }
+	else { k_mode_product_f32<false><<<(unsigned)o.blocks(), 256, 0, ctx().stream>>>(a, o); }
 	GLI_HIP(hipGetLastError());
 }
 
@@ -1179,7 +1203,7 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	GLI_REQUIRE(num >= 0, "negative number of indices");
-	GLI_REQUIRE(method >= GLI_ERI_TENSOR_FP64 && method <= GLI_ERI_TENSOR_FP32, "unknown ERI method");
+	GLI_REQUIRE(method >= GLI_ERI_TENSOR_FP64 && method <= GLI_ERI_TENSOR_FP32X2, "unknown ERI method");
 	GLI_REQUIRE(coeffs != nullptr && K >= 3, "the number of gausslet coefficients must be odd and at least 3");
 	phase_begin();
 	const EriSetup s = eri_shape(K, grid, mirror_symmetric(coeffs, K));
@@ -1216,11 +1240,12 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	}
 	const bool mixed = (method == GLI_ERI_TENSOR_MIXED || method == GLI_ERI_TENSOR_BF16X2);
 	const int ns = (method == GLI_ERI_TENSOR_BF16X2) ? 2 : 5;
-	const bool fp32 = (method == GLI_ERI_TENSOR_FP32);
+	const bool fp32 = (method == GLI_ERI_TENSOR_FP32 || method == GLI_ERI_TENSOR_FP32X2);
+	const bool fp32_split = (method == GLI_ERI_TENSOR_FP32X2);
 	GLI_REQUIRE(!mixed || s.Mp <= 256, "the split-precision method needs 6(n-1) + 2(K-1) < 256; use the FP64 method");
 	// the mode products of the comparison FP32 path
 	auto mode = [&](const double* X, int64_t ldx, int64_t P, double* Y, int64_t ldy, int tri_cols, int tri_rows, int tet_C1) {
-		if (fp32) { mode_product_f32(ModeArgs{X, ldx, P, w.B.p, Cp, s.Cp, s.Mp, Y, ldy, s.Cp, tri_cols, tri_rows, 0, tet_C1}); }
+		if (fp32) { mode_product_f32(ModeArgs{X, ldx, P, w.B.p, Cp, s.Cp, s.Mp, Y, ldy, s.Cp, tri_cols, tri_rows, 0, tet_C1}, fp32_split); }
 		else { mode_product(X, ldx, P, w.B.p, Cp, s.Cp, s.Mp, Y, ldy, s.Cp, tri_cols, tri_rows, 0, tet_C1); }
 	};
 

@@ -23,9 +23,10 @@ extern "C" {
 #define GLI_ERI_DIRECT_FP64  2  /* per-element FP64 evaluation, no reuse across elements */
 #define GLI_ERI_TENSOR_BF16X2 3 /* comparison only: BF16 hi/lo two-term split, FP32 accumulation (~1e-6) */
 #define GLI_ERI_TENSOR_FP32  4  /* comparison only: FP32 MFMA products and accumulation (~1e-7) */
+#define GLI_ERI_TENSOR_FP32X2 5 /* comparison only: FP32 hi/lo split operands (three FP32 MFMA products), FP32 accumulation */
 
 /* method used by compute_sparse_eri_gausslet_integrals / compute_eri_gausslet_integral;
- * the environment variable GLI_ERI_METHOD (tensor | mixed | direct | bf16x2 | fp32) sets the initial value */
+ * the environment variable GLI_ERI_METHOD (tensor | mixed | direct | bf16x2 | fp32 | fp32x2) sets the initial value */
 int gli_set_eri_method(int method);
 int gli_get_eri_method(void);
 
