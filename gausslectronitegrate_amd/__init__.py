@@ -9,6 +9,7 @@ from .pymodule import (  # noqa: F401
     compute_nuclear_gausslet_integral, compute_nuclear_gausslet_integrals,
     compute_eri_gausslet_integral, compute_sparse_eri_gausslet_integrals,
     compute_erida_gausslet_integral, compute_sparse_erida_gausslet_integrals,
+    compute_eri_gausslet_integral_batch, compute_erida_gausslet_integral_batch,
     get_max_openmp_threads, SparseERI, SparseERIDA,
     load_sparse_eri_gausslet_integrals, save_sparse_eri_gausslet_integrals,
 )

@@ -28,6 +28,10 @@ EXTRA = {
     "gli_get_tuning": (C.c_int, [C.c_char_p]),
     "gli_phase_times": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_char_p), C.c_int]),
     "gli_enable_direct_flop_count": (C.c_int, [C.c_int]),
+    "gli_eri_integrals_points": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64), C.c_int64,
+                                           C.c_double, C.c_int, C.POINTER(C.c_double)]),
+    "gli_erida_integrals_points": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64), C.c_int64,
+                                             C.c_double, C.POINTER(C.c_double)]),
     "gli_direct_flops": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "gli_device_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
     "gli_device_free": (None, [C.c_void_p]),

@@ -20,6 +20,7 @@ extern "C" {
 #define GLI_ERI_DIRECT_FP64  2  /* per-element FP64 sum, no cross-element reuse */
 #define GLI_ERI_TENSOR_BF16X2 3 /* comparison only: BF16 hi/lo two-term split, FP32 accumulation (~1e-6) */
 #define GLI_ERI_TENSOR_FP32  4  /* comparison only: FP32 MFMA products and accumulation (~1e-7) */
+#define GLI_ERI_TENSOR_FP32X2 5 /* comparison only: FP32 hi/lo split operands, FP32 accumulation (~1e-7) */
 
 int gli_hip_eri_enumerate(int64_t n, int64_t** out, int64_t* num);
 int gli_hip_erida_enumerate(int64_t n, int64_t** out, int64_t* num);

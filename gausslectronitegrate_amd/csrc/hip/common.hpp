@@ -163,6 +163,12 @@ int tile_order_grouped();
 // be reused as soon as this returns)
 void upload_async(void* dst, const void* src, size_t bytes);
 
+// Tuples of np grid points (points[(e * np + p) * 3 + a]) as linear indices on one grid: each
+// tuple moves to its own minimum corner (the integrals depend on point differences only), and
+// the grid g[6] is the smallest box starting at 0 that holds every moved tuple.  Fails when the
+// indices would reach 2^53.
+void points_to_indices(const int64_t* points, int64_t num, int np, int64_t g[6], std::vector<int64_t>& idx);
+
 // phase timing (records an event on the library stream when enabled)
 void phase_begin();
 void phase_mark(const char* name);

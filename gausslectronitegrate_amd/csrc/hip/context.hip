@@ -1,5 +1,6 @@
 // Process-wide device context, error state and the Coulomb kernel tables.
 #include "common.hpp"
+#include <algorithm>
 #include "../gli_device.h"
 
 #include <cmath>
@@ -83,6 +84,47 @@ const double* erida_table(int64_t n)
 	Context& c = ctx();
 	const long double sqrt2_3 = 0.4714045207910316829338962414032327L;
 	return table(c.erida_table_host, c.erida_table_dev, n, sqrt2_3, 3.L);
+}
+
+void points_to_indices(const int64_t* points, int64_t num, int np, int64_t g[6], std::vector<int64_t>& idx)
+{
+	GLI_REQUIRE(num >= 0, "negative number of point tuples");
+	GLI_REQUIRE(num == 0 || points != nullptr, "no points given");
+	int64_t span[3] = { 0, 0, 0 };
+	for (int64_t e = 0; e < num; ++e) {
+		for (int a = 0; a < 3; ++a) {
+			int64_t mn = points[(e * np) * 3 + a], mx = mn;
+			for (int p = 1; p < np; ++p) {
+				mn = std::min(mn, points[(e * np + p) * 3 + a]);
+				mx = std::max(mx, points[(e * np + p) * 3 + a]);
+			}
+			GLI_REQUIRE(mx - mn < 1024, "points of one tuple lie 1024 or more grid steps apart");
+			span[a] = std::max(span[a], mx - mn);
+		}
+	}
+	for (int a = 0; a < 3; ++a) {
+		g[2 * a] = 0;
+		g[2 * a + 1] = span[a] + 1;
+	}
+	const double N = (double)g[1] * g[3] * g[5];
+	double lim = 1;
+	for (int p = 0; p < np; ++p) { lim *= N; }
+	GLI_REQUIRE(lim < 9007199254740992.0, "point tuples too far apart for one batch: indices would reach 2^53");
+	const int64_t Ni = g[1] * g[3] * g[5];
+	idx.resize((size_t)num);
+	for (int64_t e = 0; e < num; ++e) {
+		int64_t mn[3];
+		for (int a = 0; a < 3; ++a) {
+			mn[a] = points[(e * np) * 3 + a];
+			for (int p = 1; p < np; ++p) { mn[a] = std::min(mn[a], points[(e * np + p) * 3 + a]); }
+		}
+		int64_t t = 0;
+		for (int p = 0; p < np; ++p) {
+			const int64_t* q = points + (e * np + p) * 3;
+			t = t * Ni + ((q[0] - mn[0]) * g[3] + (q[1] - mn[1])) * g[5] + (q[2] - mn[2]);
+		}
+		idx[(size_t)e] = t;
+	}
 }
 
 void upload_async(void* dst, const void* src, size_t bytes)

@@ -1699,6 +1699,22 @@ int gli_hip_eri_values(const double* coeffs, int64_t K, const int64_t g[6], cons
 	}
 }
 
+int gli_eri_integrals_points(const double* coeffs, int64_t K, const int64_t* points, int64_t num, double tol, int method,
+                             double* values)
+{
+	try {
+		int64_t g[6];
+		std::vector<int64_t> idx;
+		points_to_indices(points, num, 4, g, idx);
+		GLI_REQUIRE(num == 0 || values != nullptr, "no output array given");
+		if (num == 0) { return 0; }
+		return gli_hip_eri_values(coeffs, K, g, idx.data(), num, tol, method, values);
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
 int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,
                                 const int64_t* const* d_idx, const int64_t* num, double tol, int method, double* const* d_values)
 {

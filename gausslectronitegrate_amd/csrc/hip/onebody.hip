@@ -380,4 +380,20 @@ int gli_hip_erida_values(const double* coeffs, int64_t K, const int64_t g[6], co
 	}
 }
 
+int gli_erida_integrals_points(const double* coeffs, int64_t K, const int64_t* points, int64_t num, double tol,
+                               double* values)
+{
+	try {
+		int64_t g[6];
+		std::vector<int64_t> idx;
+		points_to_indices(points, num, 2, g, idx);
+		GLI_REQUIRE(num == 0 || values != nullptr, "no output array given");
+		if (num == 0) { return 0; }
+		return gli_hip_erida_values(coeffs, K, g, idx.data(), num, tol, values);
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
 }  // extern "C"

@@ -179,6 +179,46 @@ def compute_eri_gausslet_integral(gausslet_coeffs, grid_points, tol):
     return float(require_gpu().compute_eri_gausslet_integral(C.byref(gd), pts, tol))
 
 
+def _point_tuples(points, count, syntax):
+    pts = np.ascontiguousarray(points, dtype=np.int64)
+    if pts.ndim != 3 or pts.shape[1:] != (count, 3):
+        raise ValueError("'grid_points' must have shape (n, %d, 3): n tuples of %d points with three integer "
+                         "coordinates each; syntax: %s" % (count, count, syntax))
+    return pts
+
+
+def compute_eri_gausslet_integral_batch(gausslet_coeffs, grid_points, tol, method=None):
+    """(ij|kl) for n tuples of four grid points (shape (n, 4, 3)) in one GPU evaluation: the batched
+    form of compute_eri_gausslet_integral (pymodule.c:414-640).  method: one of the ERI_* constants,
+    default the current library method (gli_get_eri_method).  The GIL is released during the call."""
+    syntax = "compute_eri_gausslet_integral_batch(gausslet_coeffs, grid_points, tol, method=None)"
+    _, c = _coeffs(gausslet_coeffs, syntax)
+    pts = _point_tuples(grid_points, 4, syntax)
+    tol = _tol(tol, syntax)
+    L = require_gpu()
+    m = L.gli_get_eri_method() if method is None else int(method)
+    out = np.empty(pts.shape[0])
+    _lib.check(L.gli_eri_integrals_points(c.ctypes.data_as(_abi.P_double), len(c), pts.ctypes.data_as(_abi.P_glong),
+                                          pts.shape[0], tol, m, out.ctypes.data_as(_abi.P_double)),
+               "compute_eri_gausslet_integral_batch")
+    return out
+
+
+def compute_erida_gausslet_integral_batch(gausslet_coeffs, grid_points, tol):
+    """(ii|jj) for n pairs of grid points (shape (n, 2, 3)) in one GPU evaluation: the batched form
+    of compute_erida_gausslet_integral.  The GIL is released during the call."""
+    syntax = "compute_erida_gausslet_integral_batch(gausslet_coeffs, grid_points, tol)"
+    _, c = _coeffs(gausslet_coeffs, syntax)
+    pts = _point_tuples(grid_points, 2, syntax)
+    tol = _tol(tol, syntax)
+    L = require_gpu()
+    out = np.empty(pts.shape[0])
+    _lib.check(L.gli_erida_integrals_points(c.ctypes.data_as(_abi.P_double), len(c), pts.ctypes.data_as(_abi.P_glong),
+                                            pts.shape[0], tol, out.ctypes.data_as(_abi.P_double)),
+               "compute_erida_gausslet_integral_batch")
+    return out
+
+
 class _Perms:
     """O_h grid permutations of a sparse grid, kept for lookups (pymodule.c:1169)."""
 

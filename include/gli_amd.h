@@ -85,6 +85,17 @@ int gli_eri_values_device(const double* coeffs, int64_t K, const int64_t grid[6]
 int gli_erida_values_device(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_indices,
                             int64_t num, double tol, double* d_values);
 
+/* Many single integrals in one GPU evaluation: the batched forms of compute_eri_gausslet_integral
+ * (eri_integrals.c:133-232) and compute_erida_gausslet_integral (erida_integrals.c:119-201).
+ * points[(e * 4 + p) * 3 + a] is axis a of point p (i, j, k, l) of element e (two points i, j for
+ * ERIDA); values[e] receives (ij|kl) resp. (ii|jj); host memory, synchronous.  The ERI method is
+ * one of GLI_ERI_*.  Each tuple is moved to its own minimum corner (the values depend on point
+ * differences only) and all tuples share the smallest grid that holds them. */
+int gli_eri_integrals_points(const double* coeffs, int64_t K, const int64_t* points, int64_t num, double tol, int method,
+                             double* values);
+int gli_erida_integrals_points(const double* coeffs, int64_t K, const int64_t* points, int64_t num, double tol,
+                               double* values);
+
 /* several grids in one call (for example 5^3, 7^3, 9^3 and 11^3 with different K): a batch plan
  * without autotuning, run once; synchronous */
 int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,

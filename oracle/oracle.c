@@ -349,13 +349,20 @@ void orc_eri_flops(const double* c, glong K, const glong* g, const glong* idx, g
 	free(pln); free(pst); free(prod);
 }
 
-/* one ERI for four points pts[4][3] (eri_integrals.c:133-232) */
+/* one ERI for four points pts[4][3] (eri_integrals.c:133-232).
+ * Deviation: the reference sizes its Coulomb table from the pair spreads |s01|, |s23| only
+ * (eri_integrals.c:134-159) and reads past it when the two pair centres lie further apart; the
+ * table here covers the box holding the four points, as the sparse path's does
+ * (eri_integrals.c:551-574).  Wherever the reference stays inside its table the values agree. */
 double orc_eri_single(const double* c, glong K, const glong* pts, double tol)
 {
-	glong r01 = 0, r23 = 0;
+	glong r01 = 0, r23 = 0, span = 0;
 	for (int a = 0; a < 3; ++a) {
 		r01 = lmx(r01, labs(pts[0 * 3 + a] - pts[1 * 3 + a]));
 		r23 = lmx(r23, labs(pts[2 * 3 + a] - pts[3 * 3 + a]));
+		const glong mn = lmn(lmn(pts[a], pts[3 + a]), lmn(pts[6 + a], pts[9 + a]));
+		const glong mx = lmx(lmx(pts[a], pts[3 + a]), lmx(pts[6 + a], pts[9 + a]));
+		span = lmx(span, mx - mn);
 	}
 	const glong R = lmx(r01 + 1, r23 + 1);
 	const glong s0 = -R + 1, ns = 2 * R - 1, LP = 4 * K - 3;
@@ -363,7 +370,7 @@ double orc_eri_single(const double* c, glong K, const glong* pts, double tol)
 	glong* pst = malloc(ns * ns * sizeof(glong));
 	glong* pln = malloc(ns * ns * sizeof(glong));
 	orc_eri_products(c, K, s0, ns, tol, prod, pst, pln);
-	const glong nT = orc_eri_table_len(R, K);
+	const glong nT = orc_eri_table_len(lmx(R, span + 1), K);
 	double* T = malloc(nT * sizeof(double));
 	orc_eri_coulomb_table(nT, T);
 	glong cd6[3], st[3], nm[3];

@@ -632,6 +632,58 @@ def test_batch_split_products_in_graph(L):
         L.gli_eri_batch_destroy(plan)
 
 
+def test_batched_single_elements(L, orc):
+    """compute_eri/erida_gausslet_integral_batch on seeded random point tuples (negative coordinates,
+    repeated points, tuples of different spans) against the oracle's single-element sums
+    (eri_integrals.c:133-232, erida_integrals.c:119-201), with both ERI methods; the notebook value of
+    four orbitals at the origin (eri_integrals.ipynb:59); shape and span errors"""
+    say("batched single elements")
+    c = golden("coeffs_K29_s0.npy")
+    rng = np.random.default_rng(31)
+    pts = rng.integers(-3, 4, size=(120, 4, 3))
+    pts[:10, 1] = pts[:10, 0]                      # repeated points
+    pts[10:20] = pts[10:20] // 3                   # small spans
+    ref = np.array([orc.eri_single(c, pts[e], 1e-12) for e in range(len(pts))])
+    for method in (None, 0, 2):
+        got = gli.compute_eri_gausslet_integral_batch(c, pts, 1e-12, method)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.abs(ref).max()), method
+    pairs = rng.integers(-4, 5, size=(60, 2, 3))
+    ref2 = np.array([orc.erida_single(c, pairs[e], 1e-12) for e in range(len(pairs))])
+    got2 = gli.compute_erida_gausslet_integral_batch(c, pairs, 1e-12)
+    assert np.max(np.abs(got2 - ref2)) <= 1e-13 * max(1.0, np.abs(ref2).max())
+    g10 = golden("ref_eri.npz")["coeffs"]
+    v = gli.compute_eri_gausslet_integral_batch(g10, np.zeros((2, 4, 3), dtype=np.int64), 1e-16)
+    assert v[0] == pytest.approx(1.6252009000829197, abs=1e-13) and v[1] == v[0]
+    assert len(gli.compute_eri_gausslet_integral_batch(c, np.zeros((0, 4, 3), dtype=np.int64), 1e-12)) == 0
+    with pytest.raises(ValueError):
+        gli.compute_eri_gausslet_integral_batch(c, np.zeros((3, 2, 3), dtype=np.int64), 1e-12)
+    far = np.zeros((1, 4, 3), dtype=np.int64)
+    far[0, 3] = (40, 40, 40)
+    with pytest.raises(RuntimeError):
+        gli.compute_eri_gausslet_integral_batch(c, far, 1e-12)
+
+
+def test_concurrent_host_threads(L):
+    """entry points called from several Python threads at once (ctypes releases the GIL; the library
+    serialises its stream work) give the same values as sequential calls"""
+    say("concurrent host threads")
+    from concurrent.futures import ThreadPoolExecutor
+    c = golden("coeffs_K29_s0.npy")
+    rng = np.random.default_rng(8)
+    jobs = [rng.integers(-2, 3, size=(64, 4, 3)) for _ in range(8)]
+
+    def work(q):
+        if q % 2 == 0:
+            return gli.compute_eri_gausslet_integral_batch(c, jobs[q], 1e-12)
+        return gli.compute_sparse_eri_gausslet_integrals(c, sym(5), 1e-12).integral_values[::97]
+
+    seq = [work(q) for q in range(8)]
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        par = list(ex.map(work, range(8)))
+    for a, b in zip(seq, par):
+        assert np.array_equal(a, b)
+
+
 def test_additive_api_errors(L):
     """bad arguments of the additive entry points return -1 with a message and leave the library usable"""
     say("api errors")

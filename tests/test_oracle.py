@@ -261,6 +261,25 @@ def test_eri_brute_force_tiny_K(orc):
     assert v == pytest.approx(brute, rel=1e-12, abs=1e-14)
 
 
+def test_eri_single_equals_sparse_for_distant_pairs(orc):
+    """orc.eri_single on tuples whose two pairs lie far apart (a table sized from the pair spreads
+    alone, as in eri_integrals.c:134-159, would be read past its end) equals the sparse path on the
+    box holding the points (eri_integrals.c:551-574, table sized from the grid)"""
+    rng = np.random.default_rng(31)
+    c = golden("coeffs_K29_s0.npy")
+    pts = rng.integers(-3, 4, size=(40, 4, 3))
+    pts[:, 1] = pts[:, 0]          # i = j and k = l: zero pair spreads, centres up to 6 apart
+    pts[:, 3] = pts[:, 2]
+    grid = ((-3, 7), (-3, 7), (-3, 7))
+    N = 343
+    lin = ((pts[..., 0] + 3) * 7 + (pts[..., 1] + 3)) * 7 + (pts[..., 2] + 3)
+    idx = ((lin[:, 0] * N + lin[:, 1]) * N + lin[:, 2]) * N + lin[:, 3]
+    want = orc.eri_sparse(c, grid, idx, 1e-12)
+    got = np.array([orc.eri_single(c, pts[e], 1e-12) for e in range(len(pts))])
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - want)) <= 1e-15 * max(1.0, np.abs(want).max())
+
+
 def test_eri_flop_count_matches_instrumented_loops(orc):
     """orc.eri_flops (the closed count 2 n_x n_y + 2 R_xy (n_z + 1)) against the reference's loops
     (eri_integrals.c:608-669) run in Python with one count per executed multiply and add; the loop's
