@@ -291,8 +291,39 @@ def aux_bench(args):
             r = f()
         return r, (time.perf_counter() - t0) / reps * 1e3
 
-    out = {"metric": "one-body and ERI-DA assembly on an %d^3 grid (K=%d, seed %d), GPU vs reference C build" % (n, K, args.seed),
+    out = {"metric": "enumeration, one-body and ERI-DA assembly on an %d^3 grid (K=%d, seed %d), GPU vs reference C build"
+                     % (n, K, args.seed),
            "grid": n, "K": K, "nuclei": pos.tolist(), "charges": chg.tolist(), "tol": args.tol}
+    # ERI enumeration (symmetry-unique four-index list): GPU against the reference on all host threads
+    L = _lib.require_gpu()
+    gpu_idx = {}
+
+    def enum_gpu():
+        p = C.c_void_p()
+        cnt = C.c_int64()
+        _lib.check(L.gli_eri_enumerate_device(n, C.byref(p), C.byref(cnt)), "enumeration")
+        L.gli_synchronize()
+        gpu_idx["p"], gpu_idx["n"] = p, cnt.value
+        return cnt.value
+
+    enum_gpu()
+    L.gli_device_free(gpu_idx["p"])
+    t0 = time.perf_counter()
+    num = enum_gpu()
+    rec = {"unique_elements": num, "gpu_ms": (time.perf_counter() - t0) * 1e3}
+    if ref is not None and n <= args.enum_ref_max:
+        host = np.empty(num, dtype=np.int64)
+        _lib.check(L.gli_memcpy(host.ctypes.data, gpu_idx["p"], 8 * num, 1), "copy indices")
+        ri = _abi.SparseEriIndices()
+        t0 = time.perf_counter()
+        ref.enumerate_symmetry_reduced_eri_indices(C.byref(g), C.byref(ri))
+        rec["reference_ms"] = (time.perf_counter() - t0) * 1e3
+        rec["reference_threads"] = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+        rl = np.ctypeslib.as_array(ri.four_indices, shape=(ri.num,))
+        rec["bit_exact"] = bool(ri.num == num and np.array_equal(rl, host))
+        ref.delete_sparse_eri_indices(C.byref(ri))
+    L.gli_device_free(gpu_idx["p"])
+    out["eri_enumeration"] = rec
     # ERI-DA: all unique pairs
     erida, t_g = timed(lambda: gli.compute_sparse_erida_gausslet_integrals(c, grid, args.tol))
     rec = {"unique_elements": erida.num_entries, "gpu_ms_incl_host_copies": t_g}
@@ -374,6 +405,9 @@ def main():
                          "replayed as one hipGraph per step")
     ap.add_argument("--aux", action="store_true",
                     help="ERI-DA set, nuclear and kinetic matrices on the --n grid against the reference C build")
+    ap.add_argument("--enum-ref-max", type=int, default=9,
+                    help="with --aux: largest grid whose enumeration is also timed on the reference build (9^3 takes "
+                         "~40 s on 8 host threads; the reference's list needs 64 B of host memory per element)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="BYTES",
                     help="only run an elementwise kernel of known traffic (counter calibration) and exit")
     args = ap.parse_args()
