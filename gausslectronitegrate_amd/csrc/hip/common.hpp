@@ -120,6 +120,7 @@ struct Context {
 	int tile_grouped = 0;   // mode-product tile order (the XCD-grouped order measured slower on the ERI shapes)
 	int gather_ept = 8;     // gather elements per lane
 	int gather_lds = 1;     // gather keeps its key tables in LDS (when they fit)
+	int mode1_half = 1;     // FP64 paths store the first mode's (m2, m3)-symmetric output once
 	int gather_xcd = 1;     // gather walks one contiguous chunk of the list per XCD
 	int gemm_shape = 0;     // FP64 mode-product tile shape (0: 64x64, 1: 128x64, 2: 64x128, 3: 128x128)
 };
@@ -200,6 +201,11 @@ struct ModeArgs {
 	int Cp, tri_cols, tri_rows;
 	int64_t p_base;
 	int tet_C1;
+	// symmetric first mode (T3[m1][m2][m3] = T3[m1][m3][m2]): with half_n = n > 0 the rows are
+	// p = m2 * n + m3 and only those with m3 >= m2 are computed and stored; a following product
+	// with sym_x = 1 reads X[m2][m3][k] for m2 > m3 from X[m3][m2][k] (X rows m3 * Cp + k)
+	int half_n = 0;
+	int sym_x = 0;
 };
 constexpr int kGemmShapes = 6;
 bool gemm_shape_fits(int shape, int Cp, int tri, int64_t p_base);
@@ -208,6 +214,8 @@ void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped);
 
 void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
                   double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1);
+// the same with every field of ModeArgs (shape and order from the context's tuning)
+void mode_product_args(const ModeArgs& a);
 
 // split-precision (BF16 digit slices, exact FP32 slice sums, FP64 combination) variant:
 // split_factor() prepares B once; mode_product_split() then contracts X with it.  Both use the

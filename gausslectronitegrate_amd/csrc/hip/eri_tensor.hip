@@ -146,6 +146,20 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 		if (a.tri_cols && k0 + BN - 1 < rlo) { return; }
 		if (a.tri_rows && rhi < outer) { return; }
 	}
+	if (a.half_n > 0) {
+		// rows p = m2 n + m3; the tile is needed when one of its rows has m3 >= m2
+		const int64_t n = a.half_n, phi = (p0 + BM < a.P ? p0 + BM : a.P) - 1;
+		bool need = false;
+		for (int64_t m2 = p0 / n; m2 <= phi / n && !need; ++m2) {
+			const int64_t lo = (p0 > m2 * n + m2) ? p0 : m2 * n + m2;
+			const int64_t hi = (phi < m2 * n + n - 1) ? phi : m2 * n + n - 1;
+			need = lo <= hi;
+		}
+		if (!need) { return; }
+	}
+	// mirrored reads of a half-stored X: the tile's rows share m3 (Cp is a multiple of BM, p_base = 0)
+	const int64_t sym_m3 = a.sym_x ? p0 / a.Cp : 0;
+	const int64_t sym_k = p0 - sym_m3 * a.Cp;
 
 	const int wm = wave >> 1, wn = wave & 1;
 	d4 acc[TM][TN];
@@ -164,7 +178,9 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 		for (int r = 0; r < PA; ++r) {
 			const int id = tid + 256 * r;
 			const int mu = id / BM, p = id % BM;
-			ra[r] = (mu0 + mu < a.Kd && p0 + p < a.P) ? a.X[(int64_t)(mu0 + mu) * a.ldx + p0 + p] : 0.0;
+			const int64_t m = mu0 + mu;
+			const int64_t off = (a.sym_x && m > sym_m3) ? sym_m3 * a.ldx + m * a.Cp + sym_k + p : m * a.ldx + p0 + p;
+			ra[r] = (m < a.Kd && p0 + p < a.P) ? a.X[off] : 0.0;
 		}
 #pragma unroll
 		for (int r = 0; r < PB; ++r) {
@@ -216,6 +232,7 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 			for (int v = 0; v < 4; ++v) {
 				const int64_t row = p0 + wm * (BM / 2) + i * 16 + (lane / 16) + 4 * v;
 				if (row >= a.P || col >= a.C) { continue; }
+				if (a.half_n > 0 && row % a.half_n < row / a.half_n) { continue; }   // m3 < m2: never read
 				if (a.tet_C1 > 0) {
 					const int64_t gp = a.p_base + row;
 					const int k1 = (int)(gp / a.Cp), k2 = (int)(gp % a.Cp);
@@ -1121,13 +1138,18 @@ void mode_product_shaped(const ModeArgs& a, int shape, int grouped)
 	GLI_HIP(hipGetLastError());
 }
 
+void mode_product_args(const ModeArgs& a)
+{
+	GLI_REQUIRE(!a.sym_x || (a.p_base == 0 && a.Cp % 64 == 0), "mirrored mode-product input needs rows aligned to the key padding");
+	int shape = ctx().gemm_shape;
+	if (!gemm_shape_fits(shape, a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0 || a.sym_x, a.p_base)) { shape = 0; }
+	mode_product_shaped(a, shape, tile_order_grouped());
+}
+
 void mode_product(const double* X, int64_t ldx, int64_t P, const double* B, int64_t ldb, int C, int Kd,
                   double* Y, int64_t ldy, int Cp, int tri_cols, int tri_rows, int64_t p_base, int tet_C1)
 {
-	const ModeArgs a{X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows, p_base, tet_C1};
-	int shape = ctx().gemm_shape;
-	if (!gemm_shape_fits(shape, Cp, tri_cols || tri_rows || tet_C1 > 0, p_base)) { shape = 0; }
-	mode_product_shaped(a, shape, tile_order_grouped());
+	mode_product_args(ModeArgs{X, ldx, P, B, ldb, C, Kd, Y, ldy, Cp, tri_cols, tri_rows, p_base, tet_C1});
 }
 
 static void mode_product_f32(const ModeArgs& a, bool split)
@@ -1261,14 +1283,26 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 		split_factor(w.B.p, Cp, s.Cp, s.Mp, nullptr, ns);
 		phase_mark("split_factor");
 	}
-	// Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]
+	// Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]; symmetric in (m2, m3), so the FP64 path
+	// computes m3 >= m2 only and mode 2 reads the other half mirrored
+	const bool half = !mixed && !fp32 && c.mode1_half;
 	w.Y1.reserve((size_t)Mp * Mp * Cp);
 	if (mixed) { mode_product_split(w.T3.p, Mp * Mp, Mp * Mp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0, true); }
+	else if (half) {
+		ModeArgs a{w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0};
+		a.half_n = (int)Mp;
+		mode_product_args(a);
+	}
 	else { mode(w.T3.p, Mp * Mp, Mp * Mp, w.Y1.p, Cp, 0, 0, 0); }
 	phase_mark("mode1");
 	// Y2[m3][k1][k2] = sum_m2 Y1[m2][m3][k1] B[m2][k2]   (k2 >= k1 only)
 	w.Y2.reserve((size_t)Mp * Cp * Cp);
 	if (mixed) { mode_product_split(w.Y1.p, Mp * Cp, Mp * Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0, false); }
+	else if (half) {
+		ModeArgs a{w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0};
+		a.sym_x = 1;
+		mode_product_args(a);
+	}
 	else { mode(w.Y1.p, Mp * Cp, Mp * Cp, w.Y2.p, Cp, 1, 0, 0); }
 	phase_mark("mode2");
 	// V[k1 <= k2 <= k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3], written in tetrahedral layout
@@ -1323,6 +1357,7 @@ struct BatchPlan {
 	int gdesc_ept = -1;
 	int64_t total_pieces = 0;
 	int lds_bytes = 0;   // LDS for the largest key table of the batch
+	bool half = false;   // half-stored first mode (FP64 plans)
 	SplitWorkspace* split_ws = nullptr;
 	BatchChoice choice;
 	std::string report;
@@ -1342,12 +1377,21 @@ struct BatchPlan {
 	}
 };
 
-static ModeArgs batch_mode_args(const BatchGrid& bg, int mode)
+// half: the first mode stores only m3 >= m2 and the second reads it mirrored (FP64 plans)
+static ModeArgs batch_mode_args(const BatchGrid& bg, int mode, bool half)
 {
 	const EriSetup& s = bg.s;
 	const int64_t Mp = s.Mp, Cp = s.Cp;
-	if (mode == 0) { return ModeArgs{bg.T3, Mp * Mp, Mp * Mp, bg.B.p, Cp, s.Cp, s.Mp, bg.Y1.p, Cp, s.Cp, 0, 0, 0, 0}; }
-	if (mode == 1) { return ModeArgs{bg.Y1.p, Mp * Cp, Mp * Cp, bg.B.p, Cp, s.Cp, s.Mp, bg.Y2.p, Cp, s.Cp, 1, 0, 0, 0}; }
+	if (mode == 0) {
+		ModeArgs a{bg.T3, Mp * Mp, Mp * Mp, bg.B.p, Cp, s.Cp, s.Mp, bg.Y1.p, Cp, s.Cp, 0, 0, 0, 0};
+		a.half_n = half ? (int)Mp : 0;
+		return a;
+	}
+	if (mode == 1) {
+		ModeArgs a{bg.Y1.p, Mp * Cp, Mp * Cp, bg.B.p, Cp, s.Cp, s.Mp, bg.Y2.p, Cp, s.Cp, 1, 0, 0, 0};
+		a.sym_x = half ? 1 : 0;
+		return a;
+	}
 	return ModeArgs{bg.Y2.p, Cp * Cp, Cp * Cp, bg.B.p, Cp, s.Cp, s.Mp, bg.V.p, 0, s.Cp, 1, 1, 0, s.C1};
 }
 
@@ -1374,13 +1418,13 @@ static void batch_mode(BatchPlan& plan, int mode, const BatchChoice& ch)
 {
 	ModeArgs a[kMaxGroup];
 	const int n = (int)plan.grids.size();
-	for (int g = 0; g < n; ++g) { a[g] = batch_mode_args(*plan.grids[g], mode); }
+	for (int g = 0; g < n; ++g) { a[g] = batch_mode_args(*plan.grids[g], mode, plan.half); }
 	mode_product_grouped(a, n, ch.gemm_shape[mode], ch.gemm_order[mode]);
 }
 
 static void batch_mode_split(BatchPlan& plan, int g, int mode, const BatchChoice& ch)
 {
-	const ModeArgs a = batch_mode_args(*plan.grids[g], mode);
+	const ModeArgs a = batch_mode_args(*plan.grids[g], mode, plan.half);
 	mode_product_split(a.X, a.ldx, a.P, a.C, a.Kd, a.Y, a.ldy, a.Cp, a.tri_cols, a.tri_rows, a.p_base, a.tet_C1, false,
 	                   plan.split_ws, ch.gemm_order[mode]);
 }
@@ -1400,7 +1444,7 @@ static void batch_compute(BatchPlan& plan, const BatchChoice& ch)
 			split_factor(bg.B.p, bg.s.Cp, bg.s.Cp, bg.s.Mp, plan.split_ws);
 			for (int m = 0; m < 3; ++m) {
 				if (ch.split[m]) { batch_mode_split(plan, (int)g, m, ch); }
-				else { mode_product_shaped(batch_mode_args(bg, m), ch.gemm_shape[m], ch.gemm_order[m]); }
+				else { mode_product_shaped(batch_mode_args(bg, m, plan.half), ch.gemm_shape[m], ch.gemm_order[m]); }
 			}
 		}
 	}
@@ -1455,8 +1499,8 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 		for (int shape = 0; shape < kGemmShapes; ++shape) {
 			bool fits = true;
 			for (auto& g : plan.grids) {
-				const ModeArgs a = batch_mode_args(*g, m);
-				fits = fits && gemm_shape_fits(shape, a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0, a.p_base);
+				const ModeArgs a = batch_mode_args(*g, m, plan.half);
+				fits = fits && gemm_shape_fits(shape, a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0 || a.sym_x, a.p_base);
 			}
 			if (!fits) { continue; }
 			for (int order = 0; order < 2; ++order) {
@@ -1552,6 +1596,7 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 	std::unique_ptr<BatchPlan> plan(new BatchPlan());
 	plan->method = method;
 	plan->tol = tol;
+	plan->half = (method == GLI_ERI_TENSOR_FP64) && ctx().mode1_half;
 	size_t params = 0;
 	for (int g = 0; g < n; ++g) {
 		GLI_REQUIRE(coeffs[g] != nullptr && num[g] >= 0, "bad batch entry");
@@ -1611,8 +1656,8 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 			ch.gemm_order[m] = c.tile_grouped;
 			ch.gemm_shape[m] = c.gemm_shape;
 			for (auto& g : plan->grids) {
-				const ModeArgs a = batch_mode_args(*g, m);
-				if (!gemm_shape_fits(ch.gemm_shape[m], a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0, a.p_base)) { ch.gemm_shape[m] = 0; }
+				const ModeArgs a = batch_mode_args(*g, m, plan->half);
+				if (!gemm_shape_fits(ch.gemm_shape[m], a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0 || a.sym_x, a.p_base)) { ch.gemm_shape[m] = 0; }
 			}
 		}
 		ch.gather_ept = c.gather_ept;

@@ -46,6 +46,8 @@ int gli_synchronize(void);
  *                 that does not divide the key padding of a triangular product falls back to 0)
  *   "gather_ept"  elements per lane in the per-element gather: 2, 4, 8 (default) or 16
  *   "gather_lds"  1 (default): each gather workgroup copies the key tables into LDS first; 0: global reads
+ *   "mode1_half"  1 (default): the FP64 paths compute the first mode product's (m2, m3)-symmetric output
+ *                 once (m3 >= m2) and the second reads it mirrored; 0: full output
  *   "gather_order" 0: workgroups take consecutive pieces of the list; 1 (default): each XCD takes one
  *                 contiguous chunk of the list (its L2 then holds the key-tensor entries it reuses)
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */

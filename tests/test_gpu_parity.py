@@ -382,26 +382,28 @@ def test_tuning_variants_bit_identical(L):
     rng = np.random.default_rng(5)
     shuffled = idx[rng.permutation(n)][: n - 77]
     knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1)), (b"gather_ept", (2, 4, 8, 16)), (b"gather_lds", (0, 1)),
-             (b"gemm_shape", (0, 3, 4, 5))]
+             (b"gemm_shape", (0, 3, 4, 5)), (b"mode1_half", (0, 1))]
     saved = {k: L.gli_get_tuning(k) for k, _ in knobs}
     try:
         ref, ref_s = {}, {}
         for to, go, ept, lds, shape in itertools.product((0, 1), (0, 1), (2, 4, 8, 16), (0, 1), (0, 3, 4, 5)):
             if (to + go + ept + lds + shape) % 2 and ept not in (4, 16):
                 continue   # a spread of the combinations keeps the test short; every value of every knob is used
-            for k, v in zip((b"tile_order", b"gather_order", b"gather_ept", b"gather_lds", b"gemm_shape"),
-                            (to, go, ept, lds, shape)):
+            half = (to + lds + shape + ept // 4) % 2   # the half-stored first mode: both values with every shape
+            for k, v in zip((b"tile_order", b"gather_order", b"gather_ept", b"gather_lds", b"gemm_shape", b"mode1_half"),
+                            (to, go, ept, lds, shape, half)):
                 assert L.gli_set_tuning(k, v) == 0
             for method in (0, 1):
                 got = eri_values_device(c, sym(7), idx, 1e-16, method)
                 got_s = eri_values_device(c, sym(7), shuffled, 1e-16, method)
                 if method not in ref:
                     ref[method], ref_s[method] = got, got_s
-                assert np.array_equal(got, ref[method]), (to, go, ept, lds, shape, method)
-                assert np.array_equal(got_s, ref_s[method]), (to, go, ept, lds, shape, method)
+                assert np.array_equal(got, ref[method]), (to, go, ept, lds, shape, half, method)
+                assert np.array_equal(got_s, ref_s[method]), (to, go, ept, lds, shape, half, method)
         assert np.array_equal(ref_s[0], ref[0][np.searchsorted(idx, shuffled)])
         assert L.gli_set_tuning(b"gather_ept", 3) == -1
         assert L.gli_set_tuning(b"gather_lds", 2) == -1
+        assert L.gli_set_tuning(b"mode1_half", 2) == -1
         assert L.gli_set_tuning(b"no_such_knob", 1) == -1
         assert L.gli_get_tuning(b"no_such_knob") == -1
     finally:
