@@ -686,6 +686,60 @@ def test_concurrent_host_threads(L):
         assert np.array_equal(a, b)
 
 
+def test_empty_and_ragged_inputs(L, orc):
+    """empty index lists through the reference entry points (every ERI method) and the device calls,
+    a batch whose first grid has no elements, and list lengths around the 64-lane and 512-element
+    boundaries against the oracle"""
+    say("empty and ragged")
+    c = golden("coeffs_K29_s0.npy")
+    gd = _abi.GaussletData()
+    gd.coefficients = c.ctypes.data_as(_abi.P_double)
+    gd.indices.istart = -(len(c) - 1) // 2
+    gd.indices.num = len(c)
+    g5 = _abi.Grid.of(sym(5))
+    empty = np.zeros(1, dtype=np.int64)
+    for method in range(6):
+        assert L.gli_set_eri_method(method) == 0
+        res = _abi.SparseEri()
+        L.compute_sparse_eri_gausslet_integrals(C.byref(gd), C.byref(g5), empty.ctypes.data_as(_abi.P_glong), 0, 1e-12,
+                                                C.byref(res))
+        assert res.num_entries == 0
+        L.delete_sparse_eri_gausslet_integrals(C.byref(res))
+    assert L.gli_set_eri_method(0) == 0
+    resd = _abi.SparseErida()
+    L.compute_sparse_erida_gausslet_integrals(C.byref(gd), C.byref(g5), empty.ctypes.data_as(_abi.P_glong), 0, 1e-12,
+                                              C.byref(resd))
+    assert resd.num_entries == 0
+    L.delete_sparse_erida_gausslet_integrals(C.byref(resd))
+    P = c.ctypes.data_as(_abi.P_double)
+    g = np.array([v for ax in sym(5) for v in ax], dtype=np.int64)
+    dv = DevArray(8)
+    for method in (0, 1, 2):
+        assert L.gli_eri_values_device(P, len(c), g.ctypes.data_as(_abi.P_glong), None, 0, 1e-12, method, dv.ptr) == 0
+    # ragged lengths
+    full = golden("golden_enum_n5.npz")["eri_indices"]
+    for m in (1, 63, 64, 65, 511, 513, 4097):
+        idx = full[:: max(1, len(full) // m)][:m]
+        ref = orc.eri_sparse(c, sym(5), idx, 1e-12)
+        for method in (0, 2):
+            got = eri_values_device(c, sym(5), idx, 1e-12, method)
+            assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.abs(ref).max()), (m, method)
+    # a batch whose first grid is empty
+    d3, n3 = eri_enumerate_device(3)
+    out3 = DevArray(8 * n3)
+    outE = DevArray(8)
+    cptr = (_abi.P_double * 2)(P, P)
+    Ks = np.array([len(c), len(c)], dtype=np.int64)
+    grids = np.array([v for ax in sym(5) for v in ax] + [v for ax in sym(3) for v in ax], dtype=np.int64)
+    dptr = (C.c_void_p * 2)(d3.ptr, d3.ptr)
+    nums = np.array([0, n3], dtype=np.int64)
+    vptr = (C.c_void_p * 2)(outE.ptr, out3.ptr)
+    _lib.check(L.gli_eri_values_batch_device(2, cptr, Ks.ctypes.data_as(_abi.P_glong), grids.ctypes.data_as(_abi.P_glong),
+                                             dptr, nums.ctypes.data_as(_abi.P_glong), 1e-12, 0, vptr), "batch")
+    want = eri_values_device(c, sym(3), d3.to_numpy(np.int64, n3), 1e-12, 0)
+    assert np.array_equal(out3.to_numpy(np.float64, n3), want)
+
+
 def test_additive_api_errors(L):
     """bad arguments of the additive entry points return -1 with a message and leave the library usable"""
     say("api errors")
