@@ -168,6 +168,9 @@ def batch_bench(args):
     element against the single-grid FP64 path (times 1/a)."""
     import torch
     L = _lib.require_gpu()
+    for kv in args.tune:
+        knob, value = kv.split("=", 1)
+        _lib.check(L.gli_set_tuning(knob.encode(), int(value)), "gli_set_tuning " + kv)
     stream = torch.cuda.ExternalStream(L.gli_stream())
     method = METHODS[args.method]
     n_g = len(BATCH)

@@ -264,6 +264,10 @@ int gli_set_tuning(const char* knob, int value)
 			GLI_REQUIRE(value == 2 || value == 4 || value == 8 || value == 16, "gather_ept must be 2, 4, 8 or 16");
 			c.gather_ept = value;
 		}
+		else if (k == "batch_branches") {
+			GLI_REQUIRE(value >= 0 && value <= 2, "batch_branches must be 0, 1 or 2");
+			c.batch_branches = value;
+		}
 		else if (k == "mode1_half") {
 			GLI_REQUIRE(value == 0 || value == 1, "mode1_half must be 0 or 1");
 			c.mode1_half = value;
@@ -300,6 +304,7 @@ int gli_get_tuning(const char* knob)
 		if (k == "gather_order") { return c.gather_xcd; }
 		if (k == "gather_lds") { return c.gather_lds; }
 		if (k == "mode1_half") { return c.mode1_half; }
+		if (k == "batch_branches") { return c.batch_branches; }
 		if (k == "gemm_shape") { return c.gemm_shape; }
 		GLI_REQUIRE(false, "unknown tuning knob");
 	}

@@ -1358,6 +1358,15 @@ struct BatchPlan {
 	int64_t total_pieces = 0;
 	int lds_bytes = 0;   // LDS for the largest key table of the batch
 	bool half = false;   // half-stored first mode (FP64 plans)
+	// two-branch graph: the grid with the most elements on the library stream, the others on a
+	// second stream (default priority) forked after the tables and joined before the end
+	bool branches = false;
+	int lead = -1;
+	hipStream_t side = nullptr;
+	hipEvent_t fork = nullptr, join = nullptr;
+	DevBuf<GatherDesc> d_gdesc_lead, d_gdesc_rest;
+	int64_t pieces_lead = 0, pieces_rest = 0;
+	int gdesc2_ept = -1;
 	SplitWorkspace* split_ws = nullptr;
 	BatchChoice choice;
 	std::string report;
@@ -1372,6 +1381,9 @@ struct BatchPlan {
 		if (exec) { (void)hipGraphExecDestroy(exec); }
 		if (graph) { (void)hipGraphDestroy(graph); }
 		if (done) { (void)hipEventDestroy(done); }
+		if (fork) { (void)hipEventDestroy(fork); }
+		if (join) { (void)hipEventDestroy(join); }
+		if (side) { (void)hipStreamDestroy(side); }
 		if (h_params) { (void)hipHostFree(h_params); }
 		if (split_ws) { split_workspace_delete(split_ws); }
 	}
@@ -1427,6 +1439,74 @@ static void batch_mode_split(BatchPlan& plan, int g, int mode, const BatchChoice
 	const ModeArgs a = batch_mode_args(*plan.grids[g], mode, plan.half);
 	mode_product_split(a.X, a.ldx, a.P, a.C, a.Kd, a.Y, a.ldy, a.Cp, a.tri_cols, a.tri_rows, a.p_base, a.tet_C1, false,
 	                   plan.split_ws, ch.gemm_order[mode]);
+}
+
+// gather descriptors of the two branches (lead grid alone, the others together)
+static void batch_gather_descs_branches(BatchPlan& plan, int ept)
+{
+	if (plan.gdesc2_ept == ept) { return; }
+	std::vector<GatherDesc> lead, rest;
+	int64_t pl = 0, pr = 0;
+	for (size_t gi = 0; gi < plan.grids.size(); ++gi) {
+		const BatchGrid& g = *plan.grids[gi];
+		GatherDesc d = gather_desc(g.s, g.d_idx, g.num, g.V.p, g.d_vals, plan.d_params.p + g.scale_off);
+		const bool is_lead = (int)gi == plan.lead;
+		int64_t& pieces = is_lead ? pl : pr;
+		d.piece_begin = pieces;
+		pieces += gather_pieces(g.num, ept);
+		(is_lead ? lead : rest).push_back(d);
+	}
+	plan.d_gdesc_lead.reserve(lead.size());
+	plan.d_gdesc_rest.reserve(std::max<size_t>(rest.size(), 1));
+	GLI_HIP(hipMemcpyAsync(plan.d_gdesc_lead.p, lead.data(), lead.size() * sizeof(GatherDesc), hipMemcpyHostToDevice, ctx().stream));
+	if (!rest.empty()) {
+		GLI_HIP(hipMemcpyAsync(plan.d_gdesc_rest.p, rest.data(), rest.size() * sizeof(GatherDesc), hipMemcpyHostToDevice, ctx().stream));
+	}
+	GLI_HIP(hipStreamSynchronize(ctx().stream));
+	plan.gdesc2_ept = ept;
+	plan.pieces_lead = pl;
+	plan.pieces_rest = pr;
+}
+
+// grouped mode product over the lead grid (lead = true) or over all other grids
+static void batch_mode_subset(BatchPlan& plan, int mode, const BatchChoice& ch, bool lead)
+{
+	ModeArgs a[kMaxGroup];
+	int n = 0;
+	for (size_t g = 0; g < plan.grids.size(); ++g) {
+		if (((int)g == plan.lead) == lead) { a[n++] = batch_mode_args(*plan.grids[g], mode, plan.half); }
+	}
+	if (n > 0) { mode_product_grouped(a, n, ch.gemm_shape[mode], ch.gemm_order[mode]); }
+}
+
+// launches of the library's helpers go to another stream for the lifetime of this object (the
+// caller holds the context lock)
+struct StreamScope {
+	hipStream_t saved;
+	explicit StreamScope(hipStream_t s) : saved(ctx().stream) { ctx().stream = s; }
+	~StreamScope() { ctx().stream = saved; }
+};
+
+// the FP64 evaluation as two branches: the other grids' products and gather on the side stream
+// overlap the lead grid's products and gather on the library stream
+static void batch_compute_branches(BatchPlan& plan, const BatchChoice& ch)
+{
+	hipStream_t lib_stream = ctx().stream;
+	for (auto& g : plan.grids) { launch_tables(g->s, plan.d_params.p + g->coeff_off, plan.tol, g->f.p, g->P.p, g->B.p); }
+	batch_gather_descs_branches(plan, ch.gather_ept);
+	GLI_HIP(hipEventRecord(plan.fork, lib_stream));
+	GLI_HIP(hipStreamWaitEvent(plan.side, plan.fork, 0));
+	{
+		StreamScope on_side(plan.side);
+		for (int m = 0; m < 3; ++m) { batch_mode_subset(plan, m, ch, false); }
+		launch_gather_grouped(plan.d_gdesc_rest.p, (int)plan.grids.size() - 1, plan.pieces_rest, ch.gather_ept,
+		                      ch.gather_order, ch.gather_lds, plan.lds_bytes);
+	}
+	GLI_HIP(hipEventRecord(plan.join, plan.side));
+	for (int m = 0; m < 3; ++m) { batch_mode_subset(plan, m, ch, true); }
+	launch_gather_grouped(plan.d_gdesc_lead.p, 1, plan.pieces_lead, ch.gather_ept, ch.gather_order, ch.gather_lds,
+	                      plan.lds_bytes);
+	GLI_HIP(hipStreamWaitEvent(lib_stream, plan.join, 0));
 }
 
 // everything after the parameter upload
@@ -1669,19 +1749,67 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 	// one eager pass sizes every workspace, then the same sequence is captured
 	batch_compute(*plan, ch);
 	GLI_HIP(hipStreamSynchronize(c.stream));
-	GLI_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-	try {
-		GLI_HIP(hipMemcpyAsync(plan->d_params.p, plan->h_params, params * sizeof(double), hipMemcpyHostToDevice, c.stream));
-		batch_compute(*plan, ch);
+	BatchPlan& pl = *plan;
+	auto capture = [&](bool branches, hipGraph_t* graph, hipGraphExec_t* exec) {
+		GLI_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+		try {
+			GLI_HIP(hipMemcpyAsync(pl.d_params.p, pl.h_params, params * sizeof(double), hipMemcpyHostToDevice, c.stream));
+			if (branches) { batch_compute_branches(pl, ch); }
+			else { batch_compute(pl, ch); }
+		}
+		catch (...) {
+			hipGraph_t broken = nullptr;
+			(void)hipStreamEndCapture(c.stream, &broken);
+			if (broken) { (void)hipGraphDestroy(broken); }
+			throw;
+		}
+		GLI_HIP(hipStreamEndCapture(c.stream, graph));
+		GLI_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+	};
+	capture(false, &plan->graph, &plan->exec);
+	// two-branch variant: FP64 plans of two or more grids, timed against the single branch when
+	// autotuning (knob batch_branches: 0 never, 1 autotuner decides, 2 always)
+	const bool eligible = n >= 2 && !(ch.split[0] || ch.split[1] || ch.split[2]);
+	if (eligible && c.batch_branches > 0 && (autotune || c.batch_branches == 2)) {
+		int64_t most = -1;
+		for (int g = 0; g < n; ++g) {
+			if (plan->grids[g]->num > most) { most = plan->grids[g]->num; plan->lead = g; }
+		}
+		GLI_HIP(hipStreamCreateWithFlags(&plan->side, hipStreamNonBlocking));
+		GLI_HIP(hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming));
+		GLI_HIP(hipEventCreateWithFlags(&plan->join, hipEventDisableTiming));
+		batch_compute_branches(*plan, ch);   // eager pass: descriptors and a check that it runs
+		GLI_HIP(hipStreamSynchronize(plan->side));
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		hipGraph_t g2 = nullptr;
+		hipGraphExec_t e2 = nullptr;
+		capture(true, &g2, &e2);
+		bool use = c.batch_branches == 2;
+		std::string note = "\"forced\"";
+		if (!use) {
+			const double t1 = time_ms([&] { GLI_HIP(hipGraphLaunch(plan->exec, c.stream)); });
+			const double t2 = time_ms([&] { GLI_HIP(hipGraphLaunch(e2, c.stream)); });
+			use = t2 < t1;
+			note = "{\"one_branch_ms\": " + fmt("%.4f", t1) + ", \"two_branch_ms\": " + fmt("%.4f", t2) + "}";
+		}
+		if (use) {
+			GLI_HIP(hipGraphExecDestroy(plan->exec));
+			GLI_HIP(hipGraphDestroy(plan->graph));
+			plan->exec = e2;
+			plan->graph = g2;
+			plan->branches = true;
+		}
+		else {
+			GLI_HIP(hipGraphExecDestroy(e2));
+			GLI_HIP(hipGraphDestroy(g2));
+		}
+		std::string& r = plan->report;
+		if (!r.empty() && r.back() == '}') {
+			r.pop_back();
+			r += std::string(r.size() > 1 ? ", " : "") + "\"branches\": {\"timing\": " + note + ", \"chosen\": " +
+			     (use ? "\"two\"" : "\"one\"") + ", \"lead_grid\": " + std::to_string(plan->lead) + "}}";
+		}
 	}
-	catch (...) {
-		hipGraph_t broken = nullptr;
-		(void)hipStreamEndCapture(c.stream, &broken);
-		if (broken) { (void)hipGraphDestroy(broken); }
-		throw;
-	}
-	GLI_HIP(hipStreamEndCapture(c.stream, &plan->graph));
-	GLI_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
 	GLI_HIP(hipEventCreateWithFlags(&plan->done, hipEventDisableTiming));
 	return plan.release();
 }

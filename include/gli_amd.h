@@ -48,6 +48,9 @@ int gli_synchronize(void);
  *   "gather_lds"  1 (default): each gather workgroup copies the key tables into LDS first; 0: global reads
  *   "mode1_half"  1 (default): the FP64 paths compute the first mode product's (m2, m3)-symmetric output
  *                 once (m3 >= m2) and the second reads it mirrored; 0: full output
+ *   "batch_branches" batch plans of two or more grids with FP64 products: 1 (default) the autotuner also
+ *                 times a graph whose largest grid runs beside the others on a second stream and keeps
+ *                 the faster; 0 never; 2 always
  *   "gather_order" 0: workgroups take consecutive pieces of the list; 1 (default): each XCD takes one
  *                 contiguous chunk of the list (its L2 then holds the key-tensor entries it reuses)
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
