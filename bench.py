@@ -401,6 +401,8 @@ def main():
                     help="comma-separated methods timed beside the main one, each with its error against the main "
                          "FP64 values over every element (e.g. mixed,bf16x2,fp32,fp32x2)")
     ap.add_argument("--no-compare", action="store_true", help="time no other method")
+    ap.add_argument("--no-plan", action="store_true",
+                    help="skip the one-grid batch-plan replay reported beside the single-call rate")
     ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
                     help="set a kernel tuning knob (gli_set_tuning) before timing; repeatable")
     ap.add_argument("--batch", action="store_true",
@@ -543,6 +545,48 @@ def main():
         log("%s: %.3f ms/step, max abs err vs %s over all elements %.3g"
             % (name, oth_ms, args.method, compare[-1]["max_abs_err_vs_%s_all_elements" % args.method]))
 
+    # the same workload as a one-grid batch plan (repeated evaluation): every element's key-tensor
+    # position is computed once at plan creation, then each replay of the captured graph runs the
+    # tables, the three products and a gather from those positions; reported beside `value`
+    plan_line = None
+    if not args.no_plan:
+        plan = C.c_void_p()
+        cptr = (_abi.P_double * 1)(c.ctypes.data_as(_abi.P_double))
+        Ks = np.array([len(c)], dtype=np.int64)
+        dptr = (C.c_void_p * 1)(d_idx.value)
+        nn = np.array([num], dtype=np.int64)
+        d_pl = Dev(L, 8 * num)
+        vptr = (C.c_void_p * 1)(d_pl.ptr.value)
+        t0 = time.perf_counter()
+        _lib.check(L.gli_eri_batch_create(1, cptr, Ks.ctypes.data_as(_abi.P_glong), g.ctypes.data_as(_abi.P_glong), dptr,
+                                          nn.ctypes.data_as(_abi.P_glong), args.tol, method if method in (0, 1) else 0,
+                                          vptr, 1, C.byref(plan)), "gli_eri_batch_create")
+        create_ms = (time.perf_counter() - t0) * 1e3
+        buf = C.create_string_buffer(1 << 16)
+        L.gli_eri_batch_describe(plan, buf, len(buf))
+        chosen = json.loads(buf.value.decode()).get("gather_chosen")
+        for _ in range(args.warmup):
+            _lib.check(L.gli_eri_batch_run(plan, cptr, None), "plan run")
+        L.gli_synchronize()
+        torch.cuda.synchronize()
+        p0 = torch.cuda.Event(enable_timing=True)
+        p1 = torch.cuda.Event(enable_timing=True)
+        p0.record(stream)
+        for _ in range(args.steps):
+            _lib.check(L.gli_eri_batch_run(plan, cptr, None), "plan run")
+        p1.record(stream)
+        L.gli_synchronize()
+        torch.cuda.synchronize()
+        plan_ms = p0.elapsed_time(p1) / args.steps
+        pv = np.empty(num)
+        _lib.check(L.gli_memcpy(pv.ctypes.data, d_pl.ptr, 8 * num, 1), "copy plan values")
+        L.gli_eri_batch_destroy(plan)
+        del d_pl
+        plan_line = {"ms_per_step": plan_ms, "value": num / (plan_ms / 1e3), "plan_create_ms": create_ms,
+                     "gather": chosen, "max_abs_err_vs_single_call": float(np.max(np.abs(pv - vals)))}
+        del pv
+        log("one-grid plan replay: %.3f ms/step (plan creation %.0f ms), gather %s" % (plan_ms, create_ms, chosen))
+
     # roofline of the dominant kernel
     workload = "eri_%dx%dx%d_K%d_seed%d%s" % (args.n, args.n, args.n, args.K, args.seed, "_asym" if args.asymmetric else "")
     dom = max(phases.items(), key=lambda kv: kv[1]) if phases else ("step", step_ms)
@@ -606,6 +650,7 @@ def main():
         "speedup_vs_fp64_direct": value / direct_rate,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "compare": compare,
+        "plan_replay": plan_line,
     }
     L.gli_device_free(d_idx)
     print(json.dumps(line), flush=True)

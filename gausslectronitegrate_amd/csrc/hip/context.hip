@@ -264,6 +264,10 @@ int gli_set_tuning(const char* knob, int value)
 			GLI_REQUIRE(value == 2 || value == 4 || value == 8 || value == 16, "gather_ept must be 2, 4, 8 or 16");
 			c.gather_ept = value;
 		}
+		else if (k == "batch_positions") {
+			GLI_REQUIRE(value == 0 || value == 1, "batch_positions must be 0 or 1");
+			c.batch_positions = value;
+		}
 		else if (k == "batch_branches") {
 			GLI_REQUIRE(value >= 0 && value <= 2, "batch_branches must be 0, 1 or 2");
 			c.batch_branches = value;
@@ -305,6 +309,7 @@ int gli_get_tuning(const char* knob)
 		if (k == "gather_lds") { return c.gather_lds; }
 		if (k == "mode1_half") { return c.mode1_half; }
 		if (k == "batch_branches") { return c.batch_branches; }
+		if (k == "batch_positions") { return c.batch_positions; }
 		if (k == "gemm_shape") { return c.gemm_shape; }
 		GLI_REQUIRE(false, "unknown tuning knob");
 	}

@@ -538,7 +538,10 @@ struct GatherDesc {
 	const double* scale;
 	int64_t piece_begin;   // first workgroup-sized piece of this problem in a grouped launch
 	int nlut, C1;          // entries of the key table and of the tetrahedral offsets
+	uint32_t* pos;         // key-tensor position of every element (written by k_positions, read by
+	                       // k_gather_cached; kNoPosition for indices outside the grid), or nullptr
 };
+constexpr uint32_t kNoPosition = 0xffffffffu;
 
 // LDS bytes a workgroup needs to hold the key table and the tetrahedral offsets
 __host__ __device__ inline int gather_lds_bytes(int nlut, int C1)
@@ -548,7 +551,8 @@ __host__ __device__ inline int gather_lds_bytes(int nlut, int C1)
 constexpr int kGatherLdsMax = 48 * 1024;
 
 // lut and tet3 point to the descriptor's tables in global memory, or to their copies in LDS
-template <int GATHER_EPT>
+// kPos: write each element's key-tensor position to d.pos instead of reading V (the same decode)
+template <int GATHER_EPT, bool kPos = false>
 __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const short* __restrict__ lut,
                                    const uint32_t* __restrict__ tet3)
 {
@@ -618,6 +622,14 @@ __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const
 		}
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] += tet3[k3[u]]; }
+		if (kPos) {
+#pragma unroll
+			for (int u = 0; u < GATHER_EPT; ++u) {
+				const int64_t e = wave_base + u * 64 + lane;
+				if (e < d.num) { d.pos[e] = vi[u]; }
+			}
+			return;
+		}
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = V[vi[u]]; }
 	}
@@ -625,6 +637,14 @@ __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const
 		int64_t vi[GATHER_EPT];
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] = (t[u] >= 0 && t[u] < g.N4) ? gather_index(g, dc, lut, t[u]) : -1; }
+		if (kPos) {
+#pragma unroll
+			for (int u = 0; u < GATHER_EPT; ++u) {
+				const int64_t e = wave_base + u * 64 + lane;
+				if (e < d.num) { d.pos[e] = (vi[u] >= 0) ? (uint32_t)vi[u] : kNoPosition; }
+			}
+			return;
+		}
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
 	}
@@ -652,7 +672,7 @@ __device__ inline int64_t gather_piece(int64_t xcd_chunks)
 // With kLds the workgroup first copies the key table and the tetrahedral offsets into LDS: their
 // reads then no longer compete with the V reads for the vector L1's miss path (the bound of this
 // kernel); the copy pays off from 2048 elements per workgroup on.
-template <int GATHER_EPT, bool kLds>
+template <int GATHER_EPT, bool kLds, bool kPos = false>
 __device__ inline void gather_block(const GatherDesc& d, int64_t wave_base)
 {
 	extern __shared__ uint32_t gather_lds[];
@@ -668,7 +688,7 @@ __device__ inline void gather_block(const GatherDesc& d, int64_t wave_base)
 		tet3 = st;
 	}
 	if (wave_base >= d.num) { return; }
-	gather_wave<GATHER_EPT>(d, wave_base, lut, tet3);
+	gather_wave<GATHER_EPT, kPos>(d, wave_base, lut, tet3);
 }
 
 template <int GATHER_EPT, bool kLds>
@@ -682,6 +702,48 @@ __global__ __launch_bounds__(256) void k_gather(GatherDesc d, int64_t xcd_chunks
 }
 
 // all grids of a batch in one launch: descriptors in device memory, pieces concatenated
+// key-tensor positions of one list (the gather's decode, no V reads)
+__global__ __launch_bounds__(256) void k_positions(GatherDesc d)
+{
+	const int64_t wave_base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * 8);
+	if ((int64_t)blockIdx.x * (blockDim.x >> 6) * (64 * 8) >= d.num) { return; }   // whole workgroup idle
+	gather_block<8, true, true>(d, wave_base);
+}
+
+// values from cached positions: per element a 4-byte position read, the V read, the scale and the
+// 8-byte store; all grids of a batch in one launch, pieces as in k_gather_grouped
+template <int GATHER_EPT>
+__global__ __launch_bounds__(256) void k_gather_cached_grouped(const GatherDesc* __restrict__ descs, int n, int64_t xcd_chunks)
+{
+	const int64_t piece = gather_piece(xcd_chunks);
+	int gi = 0;
+	while (gi + 1 < n && piece >= descs[gi + 1].piece_begin) { ++gi; }
+	const GatherDesc d = descs[gi];
+	const int64_t first = (piece - d.piece_begin) * (blockDim.x >> 6) * (64 * GATHER_EPT);
+	if (piece < d.piece_begin || first >= d.num) { return; }
+	const int lane = threadIdx.x & 63;
+	const int64_t wave_base = first + (threadIdx.x >> 6) * (64 * GATHER_EPT);
+	uint32_t q[GATHER_EPT];
+#pragma unroll
+	for (int u = 0; u < GATHER_EPT; ++u) {
+		const int64_t e = wave_base + u * 64 + lane;
+		q[u] = (e < d.num) ? d.pos[e] : kNoPosition;
+	}
+	double v[GATHER_EPT];
+#pragma unroll
+	for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (q[u] != kNoPosition) ? d.V[q[u]] : __builtin_nan(""); }
+	if (d.scale != nullptr) {
+		const double sc = *d.scale;
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) { v[u] *= sc; }
+	}
+#pragma unroll
+	for (int u = 0; u < GATHER_EPT; ++u) {
+		const int64_t e = wave_base + u * 64 + lane;
+		if (e < d.num) { d.out[e] = v[u]; }
+	}
+}
+
 template <int GATHER_EPT, bool kLds>
 __global__ __launch_bounds__(256) void k_gather_grouped(const GatherDesc* __restrict__ descs, int n, int64_t xcd_chunks)
 {
@@ -1042,6 +1104,7 @@ static GatherDesc gather_desc(const EriSetup& s, const int64_t* d_idx, int64_t n
 	d.piece_begin = 0;
 	d.nlut = s.R * s.R * (4 * s.R - 3);
 	d.C1 = s.C1;
+	d.pos = nullptr;
 	return d;
 }
 
@@ -1099,6 +1162,34 @@ static void launch_gather_grouped(const GatherDesc* d_descs, int n, int64_t tota
 	gather_dispatch(ept, use_lds, [&](auto E, auto Lds) {
 		k_gather_grouped<decltype(E)::value, decltype(Lds)::value><<<(unsigned)blocks, 256, shm, st>>>(d_descs, n, chunks);
 	});
+	GLI_HIP(hipGetLastError());
+}
+
+// key-tensor positions of d's list into d.pos (needs tetrahedral indices below 2^32 and the key
+// tables in LDS)
+static void launch_positions(const GatherDesc& d)
+{
+	const int64_t blocks = ceil_div(d.num, 256 * 8);
+	if (blocks == 0) { return; }
+	GLI_REQUIRE(blocks < 2147483647, "too many elements for one launch");
+	const int lds = gather_lds_bytes(d.nlut, d.C1);
+	GLI_REQUIRE(lds <= kGatherLdsMax, "key tables too large for the position pass");
+	k_positions<<<(unsigned)blocks, 256, (size_t)lds, ctx().stream>>>(d);
+	GLI_HIP(hipGetLastError());
+}
+
+static void launch_gather_cached_grouped(const GatherDesc* d_descs, int n, int64_t total_pieces, int ept, int xcd)
+{
+	if (total_pieces == 0) { return; }
+	const int64_t chunks = xcd ? ceil_div(total_pieces, 8) : 0;
+	const int64_t blocks = xcd ? 8 * chunks : total_pieces;
+	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
+	hipStream_t st = ctx().stream;
+	switch (ept) {
+		case 16: k_gather_cached_grouped<16><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); break;
+		case 8: k_gather_cached_grouped<8><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); break;
+		default: k_gather_cached_grouped<4><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); break;
+	}
 	GLI_HIP(hipGetLastError());
 }
 
@@ -1332,6 +1423,7 @@ struct BatchGrid {
 	int64_t num = 0;
 	double* d_vals = nullptr;
 	DevBuf<double> f, P, B, Y1, Y2, V;
+	DevBuf<uint32_t> pos;   // key-tensor position of every element (plans with cached positions)
 	const double* T3 = nullptr;
 	size_t coeff_off = 0, scale_off = 0;
 };
@@ -1358,6 +1450,7 @@ struct BatchPlan {
 	int64_t total_pieces = 0;
 	int lds_bytes = 0;   // LDS for the largest key table of the batch
 	bool half = false;   // half-stored first mode (FP64 plans)
+	bool cached = false; // the gather reads positions computed once at creation instead of decoding
 	// two-branch graph: the grid with the most elements on the library stream, the others on a
 	// second stream (default priority) forked after the tables and joined before the end
 	bool branches = false;
@@ -1407,6 +1500,14 @@ static ModeArgs batch_mode_args(const BatchGrid& bg, int mode, bool half)
 	return ModeArgs{bg.Y2.p, Cp * Cp, Cp * Cp, bg.B.p, Cp, s.Cp, s.Mp, bg.V.p, 0, s.Cp, 1, 1, 0, s.C1};
 }
 
+// the gather of a batch plan: from the cached positions when the plan holds them
+static void batch_launch_gather(const BatchPlan& plan, const GatherDesc* descs, int n, int64_t pieces, int ept, int order,
+                                int lds)
+{
+	if (plan.cached) { launch_gather_cached_grouped(descs, n, pieces, ept, order); }
+	else { launch_gather_grouped(descs, n, pieces, ept, order, lds, plan.lds_bytes); }
+}
+
 // gather descriptors for one elements-per-lane choice (piece offsets depend on it)
 static void batch_gather_descs(BatchPlan& plan, int ept)
 {
@@ -1415,6 +1516,7 @@ static void batch_gather_descs(BatchPlan& plan, int ept)
 	int64_t pieces = 0;
 	for (auto& g : plan.grids) {
 		GatherDesc d = gather_desc(g->s, g->d_idx, g->num, g->V.p, g->d_vals, plan.d_params.p + g->scale_off);
+		d.pos = plan.cached ? g->pos.p : nullptr;
 		d.piece_begin = pieces;
 		pieces += gather_pieces(g->num, ept);
 		h.push_back(d);
@@ -1450,6 +1552,7 @@ static void batch_gather_descs_branches(BatchPlan& plan, int ept)
 	for (size_t gi = 0; gi < plan.grids.size(); ++gi) {
 		const BatchGrid& g = *plan.grids[gi];
 		GatherDesc d = gather_desc(g.s, g.d_idx, g.num, g.V.p, g.d_vals, plan.d_params.p + g.scale_off);
+		d.pos = plan.cached ? g.pos.p : nullptr;
 		const bool is_lead = (int)gi == plan.lead;
 		int64_t& pieces = is_lead ? pl : pr;
 		d.piece_begin = pieces;
@@ -1499,13 +1602,12 @@ static void batch_compute_branches(BatchPlan& plan, const BatchChoice& ch)
 	{
 		StreamScope on_side(plan.side);
 		for (int m = 0; m < 3; ++m) { batch_mode_subset(plan, m, ch, false); }
-		launch_gather_grouped(plan.d_gdesc_rest.p, (int)plan.grids.size() - 1, plan.pieces_rest, ch.gather_ept,
-		                      ch.gather_order, ch.gather_lds, plan.lds_bytes);
+		batch_launch_gather(plan, plan.d_gdesc_rest.p, (int)plan.grids.size() - 1, plan.pieces_rest, ch.gather_ept,
+		                    ch.gather_order, ch.gather_lds);
 	}
 	GLI_HIP(hipEventRecord(plan.join, plan.side));
 	for (int m = 0; m < 3; ++m) { batch_mode_subset(plan, m, ch, true); }
-	launch_gather_grouped(plan.d_gdesc_lead.p, 1, plan.pieces_lead, ch.gather_ept, ch.gather_order, ch.gather_lds,
-	                      plan.lds_bytes);
+	batch_launch_gather(plan, plan.d_gdesc_lead.p, 1, plan.pieces_lead, ch.gather_ept, ch.gather_order, ch.gather_lds);
 	GLI_HIP(hipStreamWaitEvent(lib_stream, plan.join, 0));
 }
 
@@ -1529,8 +1631,8 @@ static void batch_compute(BatchPlan& plan, const BatchChoice& ch)
 		}
 	}
 	batch_gather_descs(plan, ch.gather_ept);
-	launch_gather_grouped(plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ch.gather_ept, ch.gather_order,
-	                      ch.gather_lds, plan.lds_bytes);
+	batch_launch_gather(plan, plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ch.gather_ept, ch.gather_order,
+	                    ch.gather_lds);
 }
 
 // median time (ms) of a few repetitions of f on the library stream
@@ -1639,12 +1741,13 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 	double best = 1e30;
 	bool first = true;
 	for (int lds = 0; lds < 2; ++lds) {
-		if (lds && plan.lds_bytes > kGatherLdsMax) { continue; }
+		if (lds && (plan.cached || plan.lds_bytes > kGatherLdsMax)) { continue; }   // no key tables with cached positions
 		for (int ept : { 2, 4, 8, 16 }) {
+			if (plan.cached && ept == 2) { continue; }
 			for (int order = 0; order < 2; ++order) {
 				batch_gather_descs(plan, ept);
 				const double t = time_ms([&] {
-					launch_gather_grouped(plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ept, order, lds, plan.lds_bytes);
+					batch_launch_gather(plan, plan.d_gdesc.p, (int)plan.grids.size(), plan.total_pieces, ept, order, lds);
 				});
 				rep += std::string(first ? "" : ", ") + "{\"ept\": " + std::to_string(ept) + ", \"order\": " + std::to_string(order) +
 				       ", \"lds\": " + std::to_string(lds) + ", \"ms\": " + fmt("%.4f", t) + "}";
@@ -1659,7 +1762,7 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 		}
 	}
 	rep += "], \"gather_chosen\": {\"ept\": " + std::to_string(ch.gather_ept) + ", \"order\": " + std::to_string(ch.gather_order) +
-	       ", \"lds\": " + std::to_string(ch.gather_lds) + "}}";
+	       ", \"lds\": " + std::to_string(ch.gather_lds) + ", \"cached_positions\": " + (plan.cached ? "true" : "false") + "}}";
 	plan.report = rep;
 	return ch;
 }
@@ -1727,6 +1830,21 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 	}
 	if (method == GLI_ERI_TENSOR_MIXED) { plan->split_ws = split_workspace_new(); }
 	GLI_HIP(hipMemcpyAsync(plan->d_params.p, plan->h_params, params * sizeof(double), hipMemcpyHostToDevice, c.stream));
+
+	// every element's position in its grid's key tensor, once per plan: the index lists are fixed for
+	// the plan's lifetime, so each replay reads 4-byte positions instead of decoding 8-byte four-indices
+	bool cache_ok = c.batch_positions != 0 && plan->lds_bytes <= kGatherLdsMax;
+	for (auto& g : plan->grids) { cache_ok = cache_ok && g->s.ks->tet32; }
+	if (cache_ok) {
+		for (auto& g : plan->grids) {
+			g->pos.alloc((size_t)std::max<int64_t>(g->num, 1));
+			GatherDesc d = gather_desc(g->s, g->d_idx, g->num, nullptr, nullptr, nullptr);
+			d.pos = g->pos.p;
+			launch_positions(d);
+		}
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		plan->cached = true;
+	}
 
 	BatchChoice ch;
 	if (autotune) { ch = batch_autotune(*plan); }

@@ -51,6 +51,9 @@ int gli_synchronize(void);
  *   "batch_branches" batch plans of two or more grids with FP64 products: 1 (default) the autotuner also
  *                 times a graph whose largest grid runs beside the others on a second stream and keeps
  *                 the faster; 0 never; 2 always
+ *   "batch_positions" 1 (default): a batch plan computes every element's position in its key tensor once,
+ *                 at creation, and each replay reads those 4-byte positions instead of decoding the
+ *                 8-byte four-indices; 0: every replay decodes
  *   "gather_order" 0: workgroups take consecutive pieces of the list; 1 (default): each XCD takes one
  *                 contiguous chunk of the list (its L2 then holds the key-tensor entries it reuses)
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
@@ -118,7 +121,7 @@ int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, cons
  * keep each grid's mirror symmetry) and per-grid output scales (NULL: 1; e.g. 1/a for a grid
  * spacing a); it is asynchronous on gli_stream().  gli_eri_batch_describe writes the autotuner's
  * timings and choices as JSON (returns the full length).  The index and value arrays must stay
- * allocated until the plan is destroyed. */
+ * allocated, and the index lists unchanged, until the plan is destroyed. */
 typedef struct gli_eri_batch gli_eri_batch;
 int gli_eri_batch_create(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,
                          const int64_t* const* d_indices, const int64_t* num, double tol, int method,

@@ -442,16 +442,18 @@ def _batch_arrays(entries, coeffs=None):
 
 
 @pytest.mark.parametrize("method", [0, 1])
-@pytest.mark.parametrize("branches", [1, 2])
-def test_batch_plan_matches_single_grids(L, method, branches):
+@pytest.mark.parametrize("branches,positions", [(1, 1), (2, 0)])
+def test_batch_plan_matches_single_grids(L, method, branches, positions):
     """a 4-grid plan (uneven K, one non-cubic grid with an unsorted sample): every replay equals the
     single-grid path bit for bit (FP64; the same kernels in the same summation order) or within
     1e-10 of FP64 (mixed, whose autotuner may keep FP64 products where they are faster); output
     scales, new coefficients and a symmetry change behave as documented.  branches = 2 forces the
-    two-branch graph (largest grid beside the others on a second stream) wherever it applies."""
-    say("batch plan method", method, "branches", branches)
-    saved = L.gli_get_tuning(b"batch_branches")
+    two-branch graph (largest grid beside the others on a second stream) wherever it applies;
+    positions = 1 caches every element's key-tensor position at plan creation, 0 decodes per replay."""
+    say("batch plan method", method, "branches", branches, "positions", positions)
+    saved = L.gli_get_tuning(b"batch_branches"), L.gli_get_tuning(b"batch_positions")
     assert L.gli_set_tuning(b"batch_branches", branches) == 0
+    assert L.gli_set_tuning(b"batch_positions", positions) == 0
     rng = np.random.default_rng(11)
     ragged = rng.integers(0, (5 * 3 * 4) ** 4, size=5000)
     specs = [(sym(3), golden("coeffs_K29_s0.npy"), None),
@@ -466,13 +468,15 @@ def test_batch_plan_matches_single_grids(L, method, branches):
                                           grids.ctypes.data_as(_abi.P_glong), dptr, nums.ctypes.data_as(_abi.P_glong), 1e-16,
                                           method, vptr, 1, C.byref(plan)), "gli_eri_batch_create")
     finally:
-        L.gli_set_tuning(b"batch_branches", saved)
+        L.gli_set_tuning(b"batch_branches", saved[0])
+        L.gli_set_tuning(b"batch_positions", saved[1])
     try:
         import json
         buf = C.create_string_buffer(1 << 16)
         assert L.gli_eri_batch_describe(plan, buf, len(buf)) > 0
         report = json.loads(buf.value.decode())
         assert len(report["modes"]) == 3 and report["gather"]
+        assert report["gather_chosen"]["cached_positions"] == bool(positions)
         split_any = any(m["chosen"] != "fp64" for m in report["modes"])
         if not split_any:
             assert report["branches"]["chosen"] == ("two" if branches == 2 else report["branches"]["chosen"])
