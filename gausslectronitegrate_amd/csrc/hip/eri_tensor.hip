@@ -221,6 +221,19 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 		__syncthreads();
 	}
 
+	// per-tile constants, so that no output element needs a division: the rows of a tile share k1
+	// (tetrahedral store; Cp and p_base are multiples of BM) and start at (m2, m3) = (hm2, hm3)
+	// (half-stored first mode)
+	int64_t tk1 = 0, tk2 = 0, hm2 = 0, hm3 = 0;
+	if (a.tet_C1 > 0) {
+		const int64_t gp0 = a.p_base + p0;
+		tk1 = gp0 / a.Cp;
+		tk2 = gp0 - tk1 * a.Cp;
+	}
+	if (a.half_n > 0) {
+		hm2 = p0 / a.half_n;
+		hm3 = p0 - hm2 * a.half_n;
+	}
 	// accumulator layout of the 16x16 FP64 block (v_mfma_f64_16x16x4f64): lane holds
 	// column lane%16 and rows lane/16 + 4 v, v = 0..3
 #pragma unroll
@@ -228,15 +241,24 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 #pragma unroll
 		for (int j = 0; j < TN; ++j) {
 			const int col = k0 + wn * (BN / 2) + j * 16 + lane % 16;
+			// tet_index(k1, k2, col) = tet_index(k1, k1, col) + (k2 - k1)
+			const int64_t tbase = (a.tet_C1 > 0) ? tet_index((int)tk1, (int)tk1, col) - tk1 : 0;
 #pragma unroll
 			for (int v = 0; v < 4; ++v) {
-				const int64_t row = p0 + wm * (BM / 2) + i * 16 + (lane / 16) + 4 * v;
+				const int d = wm * (BM / 2) + i * 16 + (lane / 16) + 4 * v;   // row - p0
+				const int64_t row = p0 + d;
 				if (row >= a.P || col >= a.C) { continue; }
-				if (a.half_n > 0 && row % a.half_n < row / a.half_n) { continue; }   // m3 < m2: never read
+				if (a.half_n > 0) {
+					int64_t m3 = hm3 + d, m2 = hm2;
+					while (m3 >= a.half_n) {
+						m3 -= a.half_n;
+						++m2;
+					}
+					if (m3 < m2) { continue; }   // never read
+				}
 				if (a.tet_C1 > 0) {
-					const int64_t gp = a.p_base + row;
-					const int k1 = (int)(gp / a.Cp), k2 = (int)(gp % a.Cp);
-					if (k1 <= k2 && k2 <= col && col < a.tet_C1) { a.Y[tet_index(k1, k2, col)] = acc[i][j][v]; }
+					const int64_t k2 = tk2 + d;
+					if (tk1 <= k2 && k2 <= col && col < a.tet_C1) { a.Y[tbase + k2] = acc[i][j][v]; }
 				}
 				else {
 					a.Y[row * a.ldy + col] = acc[i][j][v];
@@ -325,17 +347,23 @@ __global__ __launch_bounds__(256) void k_mode_product_f32(ModeArgs a, TileOrder 
 		__syncthreads();
 	}
 	// FP32 16x16 accumulator layout: lane holds column lane%16 and rows 4*(lane/16) + v
+	int64_t tk1 = 0, tk2 = 0;   // the tile's rows share k1 (tetrahedral store)
+	if (a.tet_C1 > 0) {
+		tk1 = (a.p_base + p0) / a.Cp;
+		tk2 = a.p_base + p0 - tk1 * a.Cp;
+	}
 	for (int i = 0; i < 2; ++i) {
 		for (int j = 0; j < 2; ++j) {
 			const int col = k0 + wn * 32 + j * 16 + lane % 16;
+			const int64_t tbase = (a.tet_C1 > 0) ? tet_index((int)tk1, (int)tk1, col) - tk1 : 0;
 			for (int v = 0; v < 4; ++v) {
-				const int64_t row = p0 + wm * 32 + i * 16 + 4 * (lane / 16) + v;
+				const int d = wm * 32 + i * 16 + 4 * (lane / 16) + v;
+				const int64_t row = p0 + d;
 				if (row >= a.P || col >= a.C) { continue; }
 				const double val = (double)acc[i][j][v];
 				if (a.tet_C1 > 0) {
-					const int64_t gp = a.p_base + row;
-					const int k1 = (int)(gp / a.Cp), k2 = (int)(gp % a.Cp);
-					if (k1 <= k2 && k2 <= col && col < a.tet_C1) { a.Y[tet_index(k1, k2, col)] = val; }
+					const int64_t k2 = tk2 + d;
+					if (tk1 <= k2 && k2 <= col && col < a.tet_C1) { a.Y[tbase + k2] = val; }
 				}
 				else {
 					a.Y[row * a.ldy + col] = val;

@@ -180,11 +180,18 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 	}
 
 	// accumulator layout of the 16x16 FP32 block: lane holds column lane%16, rows 4*(lane/16)+v
+	int64_t tk1 = 0, tk2 = 0;   // the tile's rows share k1 (tetrahedral store)
+	if (tet_C1 > 0) {
+		tk1 = (p_base + p0) / Cp;
+		tk2 = p_base + p0 - tk1 * Cp;
+	}
 	for (int i = 0; i < 2; ++i) {
 		for (int j = 0; j < 2; ++j) {
 			const int col = k0 + wn * 32 + j * 16 + lane % 16;
+			const int64_t tbase = (tet_C1 > 0) ? tet_index((int)tk1, (int)tk1, col) - tk1 : 0;
 			for (int v = 0; v < 4; ++v) {
-				const int64_t row = p0 + wm * 32 + i * 16 + 4 * (lane / 16) + v;
+				const int dr = wm * 32 + i * 16 + 4 * (lane / 16) + v;
+				const int64_t row = p0 + dr;
 				if (row >= P || col >= C) { continue; }
 				double acc;
 				if (kOneChunk) {
@@ -199,11 +206,8 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 				}
 				const double val = ldexp(acc, ex[row] + eb[col]);
 				if (tet_C1 > 0) {
-					const int64_t gp = p_base + row;
-					const int k1 = (int)(gp / Cp), k2 = (int)(gp % Cp);
-					if (k1 <= k2 && k2 <= col && col < tet_C1) {
-						Y[tet_index(k1, k2, col)] = val;
-					}
+					const int64_t k2 = tk2 + dr;
+					if (tk1 <= k2 && k2 <= col && col < tet_C1) { Y[tbase + k2] = val; }
 				}
 				else {
 					Y[row * ldy + col] = val;
@@ -272,6 +276,8 @@ void mode_product_split(const double* X, int64_t ldx, int64_t P, int C, int Kd, 
 	SplitWorkspace& w = ws ? *ws : split_ws();
 	const int ns = w.b_ns;
 	GLI_REQUIRE(ns == 2 || ns == 5, "split_factor must run before mode_product_split");
+	GLI_REQUIRE(!(tri_cols || tri_rows || tet_C1 > 0) || (Cp % SBM == 0 && p_base % SBM == 0),
+	            "mode-product tile does not fit the key padding");
 	const int Kp = (int)round_up(Kd, SBK);
 	const int64_t Cpad = round_up(C, SBN);
 	const uint16_t* xd;
