@@ -25,8 +25,13 @@ INCLUDE = os.path.join(REPO, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
+# -amdgpu-mfma-vgpr-form: MFMA accumulators live in VGPRs.  Without it the compiler keeps the
+# loop-carried accumulators of the mode-product tiles in VGPRs anyway and copies them to and from
+# AGPRs around every slice (64 extra moves and a wait for the last MFMA per slice), and the
+# combined register count limits the tiles to 3 waves per SIMD instead of 5.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-amdgpu-mfma-vgpr-form",
              "-Wno-unused-result"]
 C_FLAGS = ["-O3", "-fPIC", "-std=gnu11", "-fopenmp", "-Wall", "-Wno-unused-function"]
 

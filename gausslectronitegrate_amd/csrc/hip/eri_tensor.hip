@@ -173,20 +173,31 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 	// (already in LDS) feeds the MFMAs
 	constexpr int PA = (GBK * BM) / 256, PB = (GBK * BN) / 256;
 	double ra[PA], rb[PB];
+	// branch-free loads with 32-bit element offsets (the host keeps both operands below 2^31
+	// elements); an out-of-range element reads offset 0 and is replaced by 0.  Divergent branches
+	// here make the compiler move the accumulators between AGPRs and VGPRs on every slice.
+	// The in-range masks are applied when the values go to LDS (the next slice), so that the loads
+	// stay in flight during this slice's MFMAs.
+	const int ldx = (int)a.ldx, ldb = (int)a.ldb, p0i = (int)p0, Pi = (int)a.P;
+	const int m3i = (int)sym_m3, sym_base = (int)(sym_m3 * a.ldx + sym_k);
+	bool oka[PA], okb[PB];
 	auto gload = [&](int mu0) {
 #pragma unroll
 		for (int r = 0; r < PA; ++r) {
 			const int id = tid + 256 * r;
 			const int mu = id / BM, p = id % BM;
-			const int64_t m = mu0 + mu;
-			const int64_t off = (a.sym_x && m > sym_m3) ? sym_m3 * a.ldx + m * a.Cp + sym_k + p : m * a.ldx + p0 + p;
-			ra[r] = (m < a.Kd && p0 + p < a.P) ? a.X[off] : 0.0;
+			const int m = mu0 + mu;
+			oka[r] = m < a.Kd && p0i + p < Pi;
+			const bool mirror = a.sym_x && m > m3i;
+			const int off = (mirror ? sym_base + m * a.Cp : m * ldx + p0i) + p;
+			ra[r] = a.X[oka[r] ? (unsigned)off : 0u];
 		}
 #pragma unroll
 		for (int r = 0; r < PB; ++r) {
 			const int id = tid + 256 * r;
 			const int mu = id / BN, k = id % BN;
-			rb[r] = (mu0 + mu < a.Kd && k0 + k < a.C) ? a.B[(int64_t)(mu0 + mu) * a.ldb + k0 + k] : 0.0;
+			okb[r] = mu0 + mu < a.Kd && k0 + k < a.C;
+			rb[r] = a.B[okb[r] ? (unsigned)((mu0 + mu) * ldb + k0 + k) : 0u];
 		}
 	};
 	gload(0);
@@ -194,15 +205,15 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 #pragma unroll
 		for (int r = 0; r < PA; ++r) {
 			const int id = tid + 256 * r;
-			As[id / BM][id % BM] = ra[r];
+			As[id / BM][id % BM] = oka[r] ? ra[r] : 0.0;
 		}
 #pragma unroll
 		for (int r = 0; r < PB; ++r) {
 			const int id = tid + 256 * r;
-			Bs[id / BN][id % BN] = rb[r];
+			Bs[id / BN][id % BN] = okb[r] ? rb[r] : 0.0;
 		}
 		__syncthreads();
-		if (mu0 + GBK < a.Kd) { gload(mu0 + GBK); }
+		gload(mu0 + GBK);   // past the last slice every element is masked off (reads offset 0)
 #pragma unroll
 		for (int kk = 0; kk < GBK; kk += 4) {
 			double av[TM], bv[TN];
@@ -1238,9 +1249,17 @@ static void launch_mode(const ModeArgs& a, const TileOrder& o, hipStream_t st)
 	k_mode_product<BM, BN, BK><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o);
 }
 
+// the tile kernels address both operands with 32-bit element offsets
+static void check_mode_extent(const ModeArgs& a)
+{
+	GLI_REQUIRE((int64_t)a.Kd * a.ldx <= INT32_MAX && (int64_t)a.Kd * a.ldb <= INT32_MAX && a.P <= INT32_MAX,
+	            "mode-product operands above 2^31 elements");
+}
+
 void mode_product_shaped(const ModeArgs& a, int shape, int grouped)
 {
 	if (a.P <= 0 || a.C <= 0) { return; }
+	check_mode_extent(a);
 	GLI_REQUIRE(gemm_shape_fits(shape, a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0, a.p_base),
 	            "mode-product tile shape does not fit the key padding");
 	GLI_REQUIRE(ceil_div(a.P, 64) * ceil_div(a.C, 64) < 2147483647 - 8 * ceil_div(a.C, 64), "mode product too large");
@@ -1285,6 +1304,7 @@ static void mode_product_f32(const ModeArgs& a, bool split)
 void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped)
 {
 	GLI_REQUIRE(n >= 1 && n <= kMaxGroup, "too many problems in one grouped mode product");
+	for (int g = 0; g < n; ++g) { check_mode_extent(a[g]); }
 	ModeGroup grp;
 	grp.n = n;
 	int blocks = 0;
