@@ -87,6 +87,8 @@ __device__ inline void combine_groups(const float* g, double& hi, double& lo)
 	}
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));   // 8 digits; a native vector stays in registers
+
 // Y[p][k] from the digits of X (Ppad columns) and B (Cpad columns)
 constexpr int KC = 192;  // exactness bound of an FP32 group accumulator: NS * 2^14 * KC < 2^24 for NS <= 5
 
@@ -118,45 +120,64 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 	f4 grp[NS][2][2];                   // FP32 group accumulators, one per digit sum
 	double y[2][2][kOneChunk ? 1 : 4];  // compensated FP64 accumulators (hi, lo) across chunks
 	double ylo[2][2][kOneChunk ? 1 : 4];
+#pragma unroll
 	for (int i = 0; i < 2; ++i) {
+#pragma unroll
 		for (int j = 0; j < 2; ++j) {
+#pragma unroll
 			for (int v = 0; v < (kOneChunk ? 1 : 4); ++v) { y[i][j][v] = 0; ylo[i][j][v] = 0; }
+#pragma unroll
 			for (int d = 0; d < NS; ++d) { grp[d][i][j] = f4{0, 0, 0, 0}; }
 		}
 	}
 
-	// register prefetch of the next chunk: NS * 64 rows * 4 runs of 8 digits per operand
+	// register prefetch of the next chunk: NS * 64 rows * 4 runs of 8 digits per operand (written
+	// out inline: a lambda capturing the arrays sends them to scratch memory)
 	constexpr int PER = NS * SBM * (SBK / 8) / 256;
-	uint4 ra[PER], rb[PER];
-	auto gload = [&](int mu0) {
+	u32x4 ra[PER], rb[PER];
 #pragma unroll
-		for (int r = 0; r < PER; ++r) {
-			const int id = tid + 256 * r;
-			const int sl = id / (SBM * 4), row = (id / 4) % SBM, c8 = (id % 4) * 8;
-			ra[r] = *(const uint4*)&Xd[((int64_t)sl * Ppad + p0 + row) * Kp + mu0 + c8];
-			rb[r] = *(const uint4*)&Bd[((int64_t)sl * Cpad + k0 + row) * Kp + mu0 + c8];
-		}
-	};
-	gload(0);
+	for (int r = 0; r < PER; ++r) {
+		const int id = tid + 256 * r;
+		const int sl = id / (SBM * 4), row = (id / 4) % SBM, c8 = (id % 4) * 8;
+		ra[r] = *(const u32x4*)&Xd[((int64_t)sl * Ppad + p0 + row) * Kp + c8];
+		rb[r] = *(const u32x4*)&Bd[((int64_t)sl * Cpad + k0 + row) * Kp + c8];
+	}
 	for (int mu0 = 0; mu0 < Kp; mu0 += SBK) {
 #pragma unroll
 		for (int r = 0; r < PER; ++r) {
 			const int id = tid + 256 * r;
 			const int sl = id / (SBM * 4), row = (id / 4) % SBM, c8 = (id % 4) * 8;
-			*(uint4*)&As[sl][row][c8] = ra[r];
-			*(uint4*)&Bs[sl][row][c8] = rb[r];
+			*(u32x4*)&As[sl][row][c8] = ra[r];
+			*(u32x4*)&Bs[sl][row][c8] = rb[r];
 		}
 		__syncthreads();
-		if (mu0 + SBK < Kp) { gload(mu0 + SBK); }
+		{
+			// unconditional (the last one re-reads chunk 0 and is unused): one basic block
+			const int mn = (mu0 + SBK < Kp) ? mu0 + SBK : 0;
+#pragma unroll
+			for (int r = 0; r < PER; ++r) {
+				const int id = tid + 256 * r;
+				const int sl = id / (SBM * 4), row = (id / 4) % SBM, c8 = (id % 4) * 8;
+				ra[r] = *(const u32x4*)&Xd[((int64_t)sl * Ppad + p0 + row) * Kp + mn + c8];
+				rb[r] = *(const u32x4*)&Bd[((int64_t)sl * Cpad + k0 + row) * Kp + mn + c8];
+			}
+		}
 		bf16x8 b[NS][2];
+#pragma unroll
 		for (int t = 0; t < NS; ++t) {
+#pragma unroll
 			for (int j = 0; j < 2; ++j) { b[t][j] = *(const bf16x8*)&Bs[t][wn * 32 + j * 16 + lane % 16][8 * (lane / 16)]; }
 		}
+#pragma unroll
 		for (int sl = 0; sl < NS; ++sl) {
 			bf16x8 a[2];
+#pragma unroll
 			for (int i = 0; i < 2; ++i) { a[i] = *(const bf16x8*)&As[sl][wm * 32 + i * 16 + lane % 16][8 * (lane / 16)]; }
+#pragma unroll
 			for (int t = 0; t < NS - sl; ++t) {
+#pragma unroll
 				for (int i = 0; i < 2; ++i) {
+#pragma unroll
 					for (int j = 0; j < 2; ++j) {
 						grp[sl + t][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[t][j], grp[sl + t][i][j], 0, 0, 0);
 					}
@@ -166,13 +187,18 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 		__syncthreads();
 		// flush the exact group sums into FP64 at the end of every KC chunk and at the end
 		if (!kOneChunk && (((mu0 + SBK) % KC) == 0 || mu0 + SBK >= Kp)) {
+#pragma unroll
 			for (int i = 0; i < 2; ++i) {
+#pragma unroll
 				for (int j = 0; j < 2; ++j) {
+#pragma unroll
 					for (int v = 0; v < 4; ++v) {
 						float g[NS];
+#pragma unroll
 						for (int d = 0; d < NS; ++d) { g[d] = grp[d][i][j][v]; }
 						combine_groups<NS>(g, y[i][j][kOneChunk ? 0 : v], ylo[i][j][kOneChunk ? 0 : v]);
 					}
+#pragma unroll
 					for (int d = 0; d < NS; ++d) { grp[d][i][j] = f4{0, 0, 0, 0}; }
 				}
 			}
@@ -185,10 +211,13 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 		tk1 = (p_base + p0) / Cp;
 		tk2 = p_base + p0 - tk1 * Cp;
 	}
+#pragma unroll
 	for (int i = 0; i < 2; ++i) {
+#pragma unroll
 		for (int j = 0; j < 2; ++j) {
 			const int col = k0 + wn * 32 + j * 16 + lane % 16;
 			const int64_t tbase = (tet_C1 > 0) ? tet_index((int)tk1, (int)tk1, col) - tk1 : 0;
+#pragma unroll
 			for (int v = 0; v < 4; ++v) {
 				const int dr = wm * 32 + i * 16 + 4 * (lane / 16) + v;
 				const int64_t row = p0 + dr;
@@ -196,6 +225,7 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 				double acc;
 				if (kOneChunk) {
 					float g[NS];
+#pragma unroll
 					for (int d = 0; d < NS; ++d) { g[d] = grp[d][i][j][v]; }
 					double hi = 0, lo = 0;
 					combine_groups<NS>(g, hi, lo);
