@@ -264,6 +264,46 @@ def test_erida_golden_7(L):
     assert np.max(np.abs(e.integral_values - g["erida_values"])) <= 1e-13
 
 
+def _reference_test_basis(num_points, num_states):
+    """The deterministic pseudo-random basis of the reference's projection tests
+    (test/test_erida_integrals.c:164-168, test/test_eri_integrals.c)."""
+    i = np.arange(num_points * num_states, dtype=np.int64)
+    b = -0.12 * np.cos(math.sqrt(7) * math.pi * ((409 * i) % 349) - 6) / (1.7 + np.sin(math.sqrt(3) * math.pi * ((557 * i) % 281)))
+    return b.reshape(num_points, num_states)
+
+
+def test_erida_project_5(L):
+    """test_project_erida_integrals (test/test_erida_integrals.c:112-228): 5^3 grid, tol 1e-5, five states;
+    the projection equals sum_ij b_ip b_iq b_jr b_js M_ij over the dense matrix M, to 1e-12."""
+    say("erida project 5")
+    c = golden("ref_erida.npz")["coeffs"]
+    e = gli.compute_sparse_erida_gausslet_integrals(c, sym(5), 1e-5)
+    basis = _reference_test_basis(125, 5)
+    proj = e.project(basis)
+    M = e.fill_matrix(sym(5))
+    pq = np.einsum("ip,iq->ipq", basis, basis).reshape(125, 25)
+    ref = (pq.T @ M @ pq).reshape(5, 5, 5, 5)
+    assert np.max(np.abs(proj - ref)) <= 1e-12
+    assert np.max(np.abs(proj - proj.transpose(2, 3, 0, 1))) <= 1e-14
+
+
+def test_erida_sparse_completeness_7(L, orc):
+    """test_erida_integral_sparse_completeness (test/test_erida_integrals.c:230-299): 7^3 grid, tol 1e-6;
+    the dense matrix filled from the symmetry-reduced set has no zero entry.  A seeded sample of entries
+    is also checked against the oracle's single-pair sum (erida_integrals.c:119-201)."""
+    say("erida completeness 7")
+    c = golden("ref_erida.npz")["coeffs"]
+    e = gli.compute_sparse_erida_gausslet_integrals(c, sym(7), 1e-6)
+    M = e.fill_matrix(sym(7))
+    assert M.shape == (343, 343)
+    assert np.count_nonzero(M == 0) == 0
+    assert np.array_equal(M, M.T)
+    pts = np.array(list(itertools.product(range(-3, 4), repeat=3)))
+    rng = np.random.default_rng(31)
+    for i, j in rng.integers(0, 343, size=(40, 2)):
+        assert abs(M[i, j] - orc.erida_single(c, (pts[i], pts[j]), 1e-6)) <= 1e-12
+
+
 # ---------------------------------------------------------------------------
 # one-body
 
