@@ -12,10 +12,14 @@
 // point p of the tuple has min_g lin(g p) >= i.
 //
 // Work decomposition: a unit is a pair (i, j) with i an orbit-minimal point;
-// a row is a unit plus one k; one thread per row walks the l's that centre
-// the box (per-axis bit masks, ascending l) and applies the full test.  Rows
-// are numbered in lexicographic (i, j, k) order, so a count pass, an exclusive
-// scan and a write pass emit the list already sorted.
+// a row is a unit plus one k; a work item (one workgroup) is a block of
+// consecutive rows of one unit.  The l's of a row that centre the box form a
+// product of three intervals, so an item's candidates (row by row, ascending
+// l) are numbered in closed form and the lanes of a wave test 64 consecutive
+// candidates: every lane does the same kind of work, whatever the rows'
+// candidate counts.  Each candidate is tested once; the results go to an
+// acceptance bitmap, and an expand pass writes the accepted four-indices at
+// offsets from a scan of the items' counts, already sorted.
 #include "common.hpp"
 #include "../gli_device.h"
 
@@ -49,9 +53,16 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 // minimg[q] == i and g belongs to canon[q], the elements taking q to its orbit minimum.
 // A placement containing a point with orbit minimum < i always has an image below t.
 // This is the same decision as testing all 48 elements (eri_integrals.c:300-352, 463-487).
+//
+// The elements to test are the union of canon[q] over the placement's points in i's orbit;
+// the identity image of the unshifted tuple is t itself and is skipped (element 0 is the
+// identity).  W[g] holds the signed weights of the original axes in the image's linear
+// index, lin(g q) = W[g] . q + H, so an image point costs three 24-bit multiply-adds.
 __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], int i, int IJ, int KL,
-                                 const int* __restrict__ minimg, const unsigned long long* __restrict__ canon)
+                                 const int* __restrict__ minimg, const unsigned long long* __restrict__ canon,
+                                 const int4* __restrict__ W)
 {
+	const int n2 = g.n * g.n, H = g.h * (n2 + g.n + 1);
 	int c[3];
 	for (int a = 0; a < 3; ++a) {
 		const int mn = min(min(P[0][a], P[1][a]), min(P[2][a], P[3][a]));
@@ -61,28 +72,27 @@ __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], int i, in
 	for (int bx = 0; bx <= abs(c[0]); ++bx) {
 		for (int by = 0; by <= abs(c[1]); ++by) {
 			for (int bz = 0; bz <= abs(c[2]); ++bz) {
-				int Q[4][3], lq[4];
+				int Q[4][3];
+				unsigned long long emask = 0;
 				for (int p = 0; p < 4; ++p) {
 					Q[p][0] = P[p][0] - bx * c[0];
 					Q[p][1] = P[p][1] - by * c[1];
 					Q[p][2] = P[p][2] - bz * c[2];
-					lq[p] = lin_c(g, Q[p][0], Q[p][1], Q[p][2]);
-					if (minimg[lq[p]] < i) { return false; }
+					const int lq = __mul24(Q[p][0], n2) + __mul24(Q[p][1], g.n) + Q[p][2] + H;
+					const int mi = minimg[lq];
+					if (mi < i) { return false; }
+					if (mi == i) { emask |= canon[lq]; }
 				}
-				for (int p = 0; p < 4; ++p) {
-					if (minimg[lq[p]] != i) { continue; }
-					for (unsigned long long m = canon[lq[p]]; m; m &= m - 1) {
-						const int e = __builtin_ctzll(m);
-						const int c0 = c_oh_col[e][0], c1 = c_oh_col[e][1], c2 = c_oh_col[e][2];
-						const int s0 = c_oh_sgn[e][0], s1 = c_oh_sgn[e][1], s2 = c_oh_sgn[e][2];
-						int L[4];
-						for (int r = 0; r < 4; ++r) { L[r] = lin_c(g, s0 * Q[r][c0], s1 * Q[r][c1], s2 * Q[r][c2]); }
-						const int a = min(L[0], L[1]), b = max(L[0], L[1]);
-						const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
-						int ab = a * g.N + b, cd = cc * g.N + d;
-						if (ab > cd) { const int t = ab; ab = cd; cd = t; }
-						if (ab < IJ || (ab == IJ && cd < KL)) { return false; }
-					}
+				if ((bx | by | bz) == 0) { emask &= ~1ull; }
+				for (; emask; emask &= emask - 1) {
+					const int4 w = W[__builtin_ctzll(emask)];
+					int L[4];
+					for (int r = 0; r < 4; ++r) { L[r] = __mul24(w.x, Q[r][0]) + __mul24(w.y, Q[r][1]) + __mul24(w.z, Q[r][2]) + H; }
+					const int a = min(L[0], L[1]), b = max(L[0], L[1]);
+					const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
+					int ab = __mul24(a, g.N) + b, cd = __mul24(cc, g.N) + d;
+					if (ab > cd) { const int t = ab; ab = cd; cd = t; }
+					if (ab < IJ || (ab == IJ && cd < KL)) { return false; }
 				}
 			}
 		}
@@ -90,71 +100,250 @@ __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], int i, in
 	return true;
 }
 
-// bit mask over l_a in [-h, h] of the l coordinates that centre the box on one axis
-__device__ inline unsigned centre_mask(const EnumGrid& g, int pi, int pj, int pk)
+// ---------------------------------------------------------------------------
+// Candidates.  For a row (unit (i, j) plus k) the l coordinates that centre the box on axis a form
+// one interval [A_a, A_a + L_a): with lo, hi the extent of (i, j, k) on that axis and s = lo + hi,
+//   s >  1:  l in [-1 - hi, 1 - hi]              (l becomes the new minimum)
+//   s < -1:  l in [-1 - lo, 1 - lo]              (l becomes the new maximum)
+//   else:    l in [min(lo, -1 - hi), max(hi, 1 - lo)]
+// clipped to [-h, h].  The row's candidates are the product of the three intervals in ascending l
+// (lexicographic (lx, ly, lz)), minus the prefix below lmin (k, or j when k == i), which the
+// identity image rules out.  Candidates are numbered row by row, so testing them in that order and
+// keeping the accepted ones yields the sorted list.
+//
+// A work item is a block of ENUM_ROWS consecutive rows of one unit (units differ in size by a
+// factor of 16 and more, so whole units would leave a long tail).  Each item's candidates occupy
+// a whole number of 64-bit words of the acceptance bitmap, one bit per candidate.
+
+constexpr int ENUM_T = 256, ENUM_ROWS = 64;
+
+__device__ inline void axis_interval(int h, int pi, int pj, int pk, int& start, int& len)
 {
 	const int lo = min(min(pi, pj), pk), hi = max(max(pi, pj), pk);
-	unsigned m = 0;
-	for (int l = -g.h; l <= g.h; ++l) {
-		const int s = min(lo, l) + max(hi, l);
-		if (s >= -1 && s <= 1) { m |= 1u << (l + g.h); }
-	}
-	return m;
+	const int s = lo + hi;
+	int a, b;
+	if (s > 1) { a = -1 - hi; b = 1 - hi; }
+	else if (s < -1) { a = -1 - lo; b = 1 - lo; }
+	else { a = min(lo, -1 - hi); b = max(hi, 1 - lo); }
+	a = max(a, -h);
+	b = min(b, h);
+	start = a;
+	len = max(0, b - a + 1);
 }
 
-// one row = unit (i, j) and k; kind 0 counts, kind 1 writes
-template <int kWrite>
-__global__ __launch_bounds__(256) void k_eri_rows(EnumGrid g, const int2* __restrict__ units, const int64_t* __restrict__ row_base,
-	const int* __restrict__ klist, const int* __restrict__ kstart, const int* __restrict__ klen,
-	const int* __restrict__ unit_icand, const int* __restrict__ minimg, const unsigned long long* __restrict__ canon,
-	uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets, int64_t* __restrict__ out)
+// v / d for v < 2^16, 0 < d < 2^8: float reciprocal and one correction step
+__device__ inline uint32_t udiv_small(uint32_t v, uint32_t d)
 {
-	const int u = blockIdx.x;
-	const int2 ij = units[u];
-	const int ic = unit_icand[u];
-	const int i = ij.x, j = ij.y;
-	const int nk = klen[ic];
-	const int* ks = klist + kstart[ic];
+	uint32_t q = (uint32_t)((float)v * __builtin_amdgcn_rcpf((float)d));
+	const int r = (int)v - (int)(q * d);
+	if (r < 0) { q -= 1; }
+	else if (r >= (int)d) { q += 1; }
+	return q;
+}
+
+// the rows of one work item in LDS
+struct EnumRows {
+	uint32_t pre[ENUM_ROWS + 1];   // exclusive prefix of the rows' candidate counts
+	uint32_t iv[ENUM_ROWS];        // per axis 5 bits start + h and 5 bits length (x in bits 0-9, y 10-19, z 20-29)
+	uint32_t ks[ENUM_ROWS];        // k (bits 0-15) and the candidates skipped below lmin (bits 16-31)
+	uint32_t kc[ENUM_ROWS];        // k's coordinates + h, 5 bits per axis
+	uint32_t wsum[ENUM_T / 64];
+};
+
+struct EnumItem {
+	int i, j, IJ, r0, rows;
 	int pi[3], pj[3];
-	pi[0] = i / (g.n * g.n) - g.h; pi[1] = (i / g.n) % g.n - g.h; pi[2] = i % g.n - g.h;
-	pj[0] = j / (g.n * g.n) - g.h; pj[1] = (j / g.n) % g.n - g.h; pj[2] = j % g.n - g.h;
-	const int IJ = i * g.N + j;
-	for (int kp = threadIdx.x; kp < nk; kp += blockDim.x) {
-		const int k = ks[kp];
-		const int64_t row = row_base[u] + kp;
-		int pk[3];
-		pk[0] = k / (g.n * g.n) - g.h; pk[1] = (k / g.n) % g.n - g.h; pk[2] = k % g.n - g.h;
-		const unsigned mx = centre_mask(g, pi[0], pj[0], pk[0]);
-		const unsigned my = centre_mask(g, pi[1], pj[1], pk[1]);
-		const unsigned mz = centre_mask(g, pi[2], pj[2], pk[2]);
-		const int lmin = (k == i) ? j : k;
-		uint32_t cnt = 0;
-		int64_t pos = kWrite ? offsets[row] : 0;
-		if (mx && my && mz) {
-			int P[4][3];
-			for (int a = 0; a < 3; ++a) { P[0][a] = pi[a]; P[1][a] = pj[a]; P[2][a] = pk[a]; }
-			for (unsigned bxm = mx; bxm; bxm &= bxm - 1) {
-				const int lx = __builtin_ctz(bxm) - g.h;
-				for (unsigned bym = my; bym; bym &= bym - 1) {
-					const int ly = __builtin_ctz(bym) - g.h;
-					for (unsigned bzm = mz; bzm; bzm &= bzm - 1) {
-						const int lz = __builtin_ctz(bzm) - g.h;
-						const int l = lin_c(g, lx, ly, lz);
-						if (l < lmin || minimg[l] < i) { continue; }
-						P[3][0] = lx; P[3][1] = ly; P[3][2] = lz;
-						if (eri_is_canonical(g, P, i, IJ, k * g.N + l, minimg, canon)) {
-							if (kWrite) {
-								out[pos++] = (((int64_t)i * g.N + j) * g.N + k) * g.N + l;
-							}
-							else {
-								cnt++;
-							}
-						}
-					}
-				}
+	const int* ks;
+};
+
+__device__ inline EnumItem enum_item(const EnumGrid& g, const int2* __restrict__ items, const int2* __restrict__ units,
+                                     const int* __restrict__ unit_icand, const int* __restrict__ klist,
+                                     const int* __restrict__ kstart, const int* __restrict__ klen)
+{
+	EnumItem it;
+	const int2 ur = items[blockIdx.x];
+	const int2 ij = units[ur.x];
+	const int ic = unit_icand[ur.x];
+	it.i = ij.x;
+	it.j = ij.y;
+	it.IJ = it.i * g.N + it.j;
+	it.r0 = ur.y;
+	it.rows = min(ENUM_ROWS, klen[ic] - ur.y);
+	it.ks = klist + kstart[ic];
+	it.pi[0] = it.i / (g.n * g.n) - g.h; it.pi[1] = (it.i / g.n) % g.n - g.h; it.pi[2] = it.i % g.n - g.h;
+	it.pj[0] = it.j / (g.n * g.n) - g.h; it.pj[1] = (it.j / g.n) % g.n - g.h; it.pj[2] = it.j % g.n - g.h;
+	return it;
+}
+
+// exclusive prefix of v over the workgroup; total receives the sum
+__device__ inline uint32_t enum_block_scan(uint32_t v, uint32_t* wsum, uint32_t& total)
+{
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	uint32_t x = v;
+	for (int off = 1; off < 64; off <<= 1) {
+		const uint32_t y = __shfl_up(x, off, 64);
+		if (lane >= off) { x += y; }
+	}
+	if (lane == 63) { wsum[w] = x; }
+	__syncthreads();
+	uint32_t before = 0;
+	total = 0;
+	for (int q = 0; q < ENUM_T / 64; ++q) {
+		const uint32_t t = wsum[q];
+		before += (q < w) ? t : 0u;
+		total += t;
+	}
+	__syncthreads();
+	return before + x - v;
+}
+
+// the item's rows into LDS (thread r < rows takes row r); returns the item's candidate count
+__device__ uint32_t enum_load_rows(const EnumGrid& g, const EnumItem& it, EnumRows& s)
+{
+	const int r = threadIdx.x;
+	uint32_t c = 0;
+	if (r < it.rows) {
+		const int k = it.ks[it.r0 + r];
+		const int pk[3] = { k / (g.n * g.n) - g.h, (k / g.n) % g.n - g.h, k % g.n - g.h };
+		int st[3], len[3];
+		for (int a = 0; a < 3; ++a) { axis_interval(g.h, it.pi[a], it.pj[a], pk[a], st[a], len[a]); }
+		// candidates below lmin: lexicographically smaller (lx, ly, lz) than lmin's coordinates
+		const int lmin = (k == it.i) ? it.j : k;
+		const int m[3] = { lmin / (g.n * g.n) - g.h, (lmin / g.n) % g.n - g.h, lmin % g.n - g.h };
+		int lt[3];
+		bool in[3];
+		for (int a = 0; a < 3; ++a) {
+			lt[a] = min(max(m[a] - st[a], 0), len[a]);
+			in[a] = m[a] >= st[a] && m[a] < st[a] + len[a];
+		}
+		const uint32_t total = (uint32_t)(len[0] * len[1] * len[2]);
+		const uint32_t below = (uint32_t)(lt[0] * len[1] * len[2] + (in[0] ? lt[1] * len[2] + (in[1] ? lt[2] : 0) : 0));
+		c = total - below;
+		uint32_t iv = 0, kc = 0;
+		for (int a = 0; a < 3; ++a) {
+			iv |= (uint32_t)((st[a] + g.h) | (len[a] << 5)) << (10 * a);
+			kc |= (uint32_t)(pk[a] + g.h) << (5 * a);
+		}
+		s.iv[r] = iv;
+		s.ks[r] = (uint32_t)k | (below << 16);
+		s.kc[r] = kc;
+	}
+	uint32_t total;
+	const uint32_t before = enum_block_scan(c, s.wsum, total);
+	if (r < it.rows) { s.pre[r] = before; }
+	__syncthreads();
+	return total;
+}
+
+// candidate q < total of the loaded item -> k, its coordinates K and the point L (relative to the centre)
+__device__ inline void enum_candidate(const EnumGrid& g, const EnumRows& s, int rows, uint32_t q, int& k, int K[3], int L[3])
+{
+	int lo = 0, hi = rows;   // the last row r with pre[r] <= q (it has candidates)
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (s.pre[mid] <= q) { lo = mid; }
+		else { hi = mid; }
+	}
+	const uint32_t ksw = s.ks[lo], iv = s.iv[lo], kc = s.kc[lo];
+	k = (int)(ksw & 0xffffu);
+	for (int a = 0; a < 3; ++a) { K[a] = (int)((kc >> (5 * a)) & 31u) - g.h; }
+	const uint32_t v = q - s.pre[lo] + (ksw >> 16);
+	const uint32_t lz = (iv >> 25) & 31u, ly = (iv >> 15) & 31u;
+	const uint32_t vz = udiv_small(v, lz);
+	L[2] = (int)((iv >> 20) & 31u) - g.h + (int)(v - vz * lz);
+	const uint32_t vy = udiv_small(vz, ly);
+	L[1] = (int)((iv >> 10) & 31u) - g.h + (int)(vz - vy * ly);
+	L[0] = (int)(iv & 31u) - g.h + (int)vy;
+}
+
+// pass 1: bitmap words of each item
+__global__ __launch_bounds__(ENUM_T) void k_eri_item_words(EnumGrid g, const int2* __restrict__ items, const int2* __restrict__ units,
+	const int* __restrict__ unit_icand, const int* __restrict__ klist, const int* __restrict__ kstart, const int* __restrict__ klen,
+	int64_t* __restrict__ words)
+{
+	__shared__ EnumRows s;
+	const EnumItem it = enum_item(g, items, units, unit_icand, klist, kstart, klen);
+	const uint32_t tot = enum_load_rows(g, it, s);
+	if (threadIdx.x == 0) { words[blockIdx.x] = (tot + 63) / 64; }
+}
+
+// pass 2: test every candidate once; one bit per candidate, accepted count per item
+__global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __restrict__ items, const int2* __restrict__ units,
+	const int* __restrict__ unit_icand, const int* __restrict__ klist, const int* __restrict__ kstart, const int* __restrict__ klen,
+	const int* __restrict__ minimg, const unsigned long long* __restrict__ canon, const int64_t* __restrict__ word_base,
+	unsigned long long* __restrict__ bits, uint32_t* __restrict__ accepted)
+{
+	__shared__ EnumRows s;
+	__shared__ int4 W[48];
+	if (threadIdx.x < 48) {
+		const int e = threadIdx.x;
+		int w[3];
+		w[c_oh_col[e][0]] = c_oh_sgn[e][0] * g.n * g.n;
+		w[c_oh_col[e][1]] = c_oh_sgn[e][1] * g.n;
+		w[c_oh_col[e][2]] = c_oh_sgn[e][2];
+		W[e] = make_int4(w[0], w[1], w[2], 0);
+	}
+	const EnumItem it = enum_item(g, items, units, unit_icand, klist, kstart, klen);
+	const int lane = threadIdx.x & 63;
+	const int64_t word = word_base[blockIdx.x];
+	const uint32_t tot = enum_load_rows(g, it, s);   // its barriers also publish W
+	uint32_t acc = 0;
+	int P[4][3];
+	for (int a = 0; a < 3; ++a) { P[0][a] = it.pi[a]; P[1][a] = it.pj[a]; }
+	for (uint32_t q0 = 0; q0 < tot; q0 += ENUM_T) {
+		const uint32_t q = q0 + threadIdx.x;
+		bool keep = false;
+		if (q < tot) {
+			int k, L[3];
+			enum_candidate(g, s, it.rows, q, k, P[2], L);
+			const int l = lin_c(g, L[0], L[1], L[2]);
+			if (minimg[l] >= it.i) {
+				P[3][0] = L[0]; P[3][1] = L[1]; P[3][2] = L[2];
+				keep = eri_is_canonical(g, P, it.i, it.IJ, __mul24(k, g.N) + l, minimg, canon, W);
 			}
 		}
-		if (!kWrite) { counts[row] = cnt; }
+		const unsigned long long m = __ballot(keep);
+		const uint32_t wq = q0 + (threadIdx.x & ~63u);
+		if (lane == 0 && wq < tot) {
+			bits[word + wq / 64] = m;
+			acc += (uint32_t)__popcll(m);
+		}
+	}
+	if (lane == 0) { s.wsum[threadIdx.x >> 6] = acc; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t t = 0;
+		for (int w = 0; w < ENUM_T / 64; ++w) { t += s.wsum[w]; }
+		accepted[blockIdx.x] = t;
+	}
+}
+
+// pass 3: the accepted candidates of each item in candidate order, from its output offset on.  Each
+// thread takes one bitmap word per round (coalesced loads, one block scan per 256 words) and writes
+// the candidates of its set bits.
+__global__ __launch_bounds__(ENUM_T) void k_eri_expand(EnumGrid g, const int2* __restrict__ items, const int2* __restrict__ units,
+	const int* __restrict__ unit_icand, const int* __restrict__ klist, const int* __restrict__ kstart, const int* __restrict__ klen,
+	const int64_t* __restrict__ word_base, const unsigned long long* __restrict__ bits, const int64_t* __restrict__ out_base,
+	int64_t* __restrict__ out)
+{
+	__shared__ EnumRows s;
+	const EnumItem it = enum_item(g, items, units, unit_icand, klist, kstart, klen);
+	const int64_t word = word_base[blockIdx.x];
+	int64_t pos = out_base[blockIdx.x];
+	const uint32_t tot = enum_load_rows(g, it, s);
+	const uint32_t nw = (tot + 63) / 64;
+	const int64_t ij = (int64_t)it.IJ;
+	for (uint32_t w0 = 0; w0 < nw; w0 += ENUM_T) {
+		const uint32_t w = w0 + threadIdx.x;
+		unsigned long long m = (w < nw) ? bits[word + w] : 0ull;
+		uint32_t total;
+		uint32_t rank = enum_block_scan((uint32_t)__popcll(m), s.wsum, total);
+		for (; m; m &= m - 1) {
+			int k, K[3], L[3];
+			enum_candidate(g, s, it.rows, w * 64 + (uint32_t)__builtin_ctzll(m), k, K, L);
+			out[pos + rank++] = (ij * g.N + k) * g.N + lin_c(g, L[0], L[1], L[2]);
+		}
+		pos += total;
 	}
 }
 
@@ -230,15 +419,15 @@ struct EnumPlan {
 	std::vector<unsigned long long> canon;  // per point: the elements g with lin(g p) == minimg
 	std::vector<int2> units;       // (i, j)
 	std::vector<int> unit_icand;   // index of i in the orbit-minimal list
-	std::vector<int64_t> row_base; // first row of each unit
 	std::vector<int> klist, kstart, klen;
-	int64_t rows = 0;
 };
 
 static void plan_enumeration(int n, EnumPlan& p)
 {
 	static const signed char col[48][3] = GLI_OH_COL;
 	static const signed char sgn[48][3] = GLI_OH_SGN;
+	GLI_REQUIRE(col[0][0] == 0 && col[0][1] == 1 && col[0][2] == 2 && sgn[0][0] == 1 && sgn[0][1] == 1 && sgn[0][2] == 1,
+	            "O_h element 0 must be the identity");
 	p.g.n = n;
 	p.g.h = (n - 1) / 2;
 	p.g.N = n * n * n;
@@ -279,8 +468,6 @@ static void plan_enumeration(int n, EnumPlan& p)
 			// j runs over the same admissible set as k (j >= i, orbit minimum >= i)
 			p.units.push_back(int2{i, ks[q]});
 			p.unit_icand.push_back((int)c);
-			p.row_base.push_back(p.rows);
-			p.rows += p.klen[c];
 		}
 	}
 }
@@ -295,7 +482,6 @@ static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 	plan_enumeration((int)n, p);
 	DevBuf<int2> units(p.units.size());
 	DevBuf<int> unit_ic(p.unit_icand.size()), klist(p.klist.size()), kstart(p.kstart.size()), klen(p.klen.size()), minimg(p.minimg.size());
-	DevBuf<int64_t> row_base(p.row_base.size());
 	hipStream_t st = c.stream;
 	GLI_HIP(hipMemcpyAsync(units.p, p.units.data(), p.units.size() * sizeof(int2), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(unit_ic.p, p.unit_icand.data(), p.unit_icand.size() * sizeof(int), hipMemcpyHostToDevice, st));
@@ -305,17 +491,37 @@ static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 	GLI_HIP(hipMemcpyAsync(minimg.p, p.minimg.data(), p.minimg.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	DevBuf<unsigned long long> canon(p.canon.size());
 	GLI_HIP(hipMemcpyAsync(canon.p, p.canon.data(), p.canon.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
-	GLI_HIP(hipMemcpyAsync(row_base.p, p.row_base.data(), p.row_base.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-	DevBuf<uint32_t> counts(p.rows);
-	DevBuf<int64_t> offsets(p.rows);
-	const unsigned nu = (unsigned)p.units.size();
-	k_eri_rows<0><<<nu, 256, 0, st>>>(p.g, units.p, row_base.p, klist.p, kstart.p, klen.p, unit_ic.p, minimg.p, canon.p, counts.p, nullptr, nullptr);
-	GLI_HIP(hipGetLastError());
-	const int64_t total = exclusive_scan<uint32_t>(counts.p, p.rows, offsets.p);
+	// work items: blocks of ENUM_ROWS rows of each unit, in (unit, row) order
+	std::vector<int2> hitems;
+	for (size_t u = 0; u < p.units.size(); ++u) {
+		const int nk = p.klen[p.unit_icand[u]];
+		for (int r0 = 0; r0 < nk; r0 += ENUM_ROWS) { hitems.push_back(int2{(int)u, r0}); }
+	}
+	GLI_REQUIRE(hitems.size() < (size_t)INT32_MAX, "enumeration has too many work items");
+	const unsigned ni = (unsigned)hitems.size();
+	DevBuf<int2> items(std::max<size_t>(hitems.size(), 1));
+	if (ni > 0) { GLI_HIP(hipMemcpyAsync(items.p, hitems.data(), hitems.size() * sizeof(int2), hipMemcpyHostToDevice, st)); }
+	DevBuf<int64_t> words(ni), word_base(ni), out_base(ni);
+	DevBuf<uint32_t> accepted(ni);
+	int64_t total = 0;
 	int64_t* out = nullptr;
-	GLI_HIP(hipMalloc(&out, (size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));
-	k_eri_rows<1><<<nu, 256, 0, st>>>(p.g, units.p, row_base.p, klist.p, kstart.p, klen.p, unit_ic.p, minimg.p, canon.p, nullptr, offsets.p, out);
-	GLI_HIP(hipGetLastError());
+	if (ni > 0) {
+		k_eri_item_words<<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, words.p);
+		GLI_HIP(hipGetLastError());
+		const int64_t nwords = exclusive_scan<int64_t>(words.p, ni, word_base.p);
+		DevBuf<unsigned long long> bits((size_t)std::max<int64_t>(nwords, 1));
+		k_eri_test<<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, minimg.p, canon.p, word_base.p,
+		                                  bits.p, accepted.p);
+		GLI_HIP(hipGetLastError());
+		total = exclusive_scan<uint32_t>(accepted.p, ni, out_base.p);
+		GLI_HIP(hipMalloc(&out, (size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));
+		k_eri_expand<<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, word_base.p, bits.p, out_base.p, out);
+		GLI_HIP(hipGetLastError());
+		GLI_HIP(hipStreamSynchronize(st));   // before the bitmap is released
+	}
+	else {
+		GLI_HIP(hipMalloc(&out, sizeof(int64_t)));
+	}
 	GLI_HIP(hipStreamSynchronize(st));
 	*d_out = out;
 	*num = total;
