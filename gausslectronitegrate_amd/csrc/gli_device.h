@@ -8,6 +8,7 @@
  */
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -27,6 +28,14 @@ int gli_hip_erida_enumerate(int64_t n, int64_t** out, int64_t* num);
 
 int gli_hip_eri_values(const double* coeffs, int64_t K, const int64_t g[6],
                        const int64_t* indices, int64_t num, double tol, int method, double* values);
+/* the same, and a host copy of the indices into indices_copy (fused with their upload) */
+int gli_hip_eri_values_copy(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* indices, int64_t num,
+                            double tol, int method, double* values, int64_t* indices_copy);
+
+/* host arrays of any size, released with free(); large ones on 2 MB pages (hostmem.c) */
+void* gli_host_alloc(size_t bytes);
+/* dst[0, bytes) = src, and dst2 too unless NULL, split over the OpenMP threads */
+void gli_host_copy(void* dst, void* dst2, const void* src, size_t bytes);
 int gli_hip_erida_values(const double* coeffs, int64_t K, const int64_t g[6],
                          const int64_t* indices, int64_t num, double tol, double* values);
 

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -171,6 +172,17 @@ void upload_async(void* dst, const void* src, size_t bytes);
 // the grid g[6] is the smallest box starting at 0 that holds every moved tuple.  Fails when the
 // indices would reach 2^53.
 void points_to_indices(const int64_t* points, int64_t num, int np, int64_t g[6], std::vector<int64_t>& idx);
+
+// Streams num elements through the device in chunks: in_bytes per element of host_in go up,
+// work(d_in, d_out, count) runs on the library stream for each chunk, and out_bytes per element
+// of d_out come back into host_out.  in_copy (or nullptr) receives a host copy of host_in, made in
+// the same pass as the staging copy.  Returns when host_out is complete.
+void stream_chunks(const void* host_in, size_t in_bytes, void* host_out, size_t out_bytes, int64_t num, void* in_copy,
+                   const std::function<void(const void*, void*, int64_t)>& work);
+// bytes of one chunk buffer of the transfer ring
+size_t stream_chunk_bytes();
+// device-to-host copy through the transfer ring, after the library stream's queued work
+void download(void* host_dst, const void* dev_src, size_t bytes);
 
 // phase timing (records an event on the library stream when enabled)
 void phase_begin();

@@ -625,18 +625,20 @@ static void erida_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 static int to_host(void (*dev)(int64_t, int64_t**, int64_t*), int64_t n, int64_t** out, int64_t* num)
 {
 	int64_t* d = nullptr;
+	int64_t* h = nullptr;
 	try {
 		dev(n, &d, num);
 		const size_t bytes = (size_t)*num * sizeof(int64_t);
-		int64_t* h = (int64_t*)aligned_alloc(64, (bytes + 63) & ~(size_t)63);
-		GLI_REQUIRE(h != nullptr || bytes == 0, "host allocation of the index list failed");
-		if (bytes > 0) { GLI_HIP(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost)); }
+		h = (int64_t*)gli_host_alloc(bytes);
+		GLI_REQUIRE(h != nullptr, "host allocation of the index list failed");
+		if (bytes > 0) { download(h, d, bytes); }
 		(void)hipFree(d);
 		*out = h;
 		return 0;
 	}
 	catch (const Failure&) {
 		if (d != nullptr) { (void)hipFree(d); }
+		free(h);
 		return -1;
 	}
 }

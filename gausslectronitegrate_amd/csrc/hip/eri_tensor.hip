@@ -1358,16 +1358,26 @@ static void direct_representations(EriWorkspace& w, int M)
 }
 
 // values of device-resident four-indices into device-resident values
-static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_idx, int64_t num,
-                           double tol, int method, double* d_vals)
+// Everything of one evaluation that does not depend on the index list: tables, and for the
+// key-tensor methods the three mode products up to V.  Queued on the library stream.
+struct EriPrepared {
+	EriSetup s;
+	int method = GLI_ERI_TENSOR_FP64;
+	DirectCount cnt{nullptr, nullptr};
+	size_t direct_shm = 0;
+};
+
+static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t grid[6], double tol, int method, bool any)
 {
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
-	GLI_REQUIRE(num >= 0, "negative number of indices");
 	GLI_REQUIRE(method >= GLI_ERI_TENSOR_FP64 && method <= GLI_ERI_TENSOR_FP32X2, "unknown ERI method");
 	GLI_REQUIRE(coeffs != nullptr && K >= 3, "the number of gausslet coefficients must be odd and at least 3");
 	phase_begin();
-	const EriSetup s = eri_shape(K, grid, mirror_symmetric(coeffs, K));
+	EriPrepared ep;
+	ep.s = eri_shape(K, grid, mirror_symmetric(coeffs, K));
+	ep.method = method;
+	const EriSetup& s = ep.s;
 	EriWorkspace& w = eri_ws();
 	w.coeffs.reserve(K);
 	w.f.reserve((size_t)s.R * (2 * s.K - 1));
@@ -1376,28 +1386,22 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	upload_async(w.coeffs.p, coeffs, K * sizeof(double));
 	launch_tables(s, w.coeffs.p, tol, w.f.p, w.P.p, w.B.p);
 	phase_mark("tables");
-	if (num == 0) { return; }
+	if (!any) { return ep; }
 	const int64_t Mp = s.Mp, Cp = s.Cp;
 
 	if (method == GLI_ERI_DIRECT_FP64) {
 		w.coords.reserve((size_t)s.g.N);
 		k_point_coords<<<(unsigned)ceil_div(s.g.N, 256), 256, 0, c.stream>>>(s.g.n[0], s.g.n[1], s.g.n[2], w.coords.p);
 		direct_representations(w, s.M);
-		DirectCount cnt{nullptr, nullptr};
 		if (w.count_direct) {
 			const int rows = s.R * s.R;
 			w.nnz_rows.reserve((size_t)rows);
 			k_nnz_rows<<<(unsigned)rows, 256, 0, c.stream>>>(w.P.p, rows, 4 * s.K - 3, w.nnz_rows.p);
-			cnt = DirectCount{w.direct_sums.p, w.nnz_rows.p};
+			ep.cnt = DirectCount{w.direct_sums.p, w.nnz_rows.p};
 		}
-		const int64_t blocks = std::min<int64_t>(num, 8192);
-		const size_t shm = 3 * (size_t)s.Mp * sizeof(double) + DIRECT_CHUNK * (sizeof(double) + sizeof(int));
-		GLI_REQUIRE(shm <= 65536, "grid too long for the per-element path: 6(n-1) + 2(K-1) must stay below 1700");
-		k_eri_direct<<<(unsigned)blocks, DIRECT_THREADS, shm, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.coords.p, w.B.p,
-			s.Cp, s.M, s.Mp, s.T, w.rep_off.p, w.rep_list.p, cnt, d_vals);
-		GLI_HIP(hipGetLastError());
-		phase_mark("direct");
-		return;
+		ep.direct_shm = 3 * (size_t)s.Mp * sizeof(double) + DIRECT_CHUNK * (sizeof(double) + sizeof(int));
+		GLI_REQUIRE(ep.direct_shm <= 65536, "grid too long for the per-element path: 6(n-1) + 2(K-1) must stay below 1700");
+		return ep;
 	}
 	const bool mixed = (method == GLI_ERI_TENSOR_MIXED || method == GLI_ERI_TENSOR_BF16X2);
 	const int ns = (method == GLI_ERI_TENSOR_BF16X2) ? 2 : 5;
@@ -1450,8 +1454,37 @@ static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6
 	if (mixed) { mode_product_split(w.Y2.p, Cp * Cp, Cp * Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1, false); }
 	else { mode(w.Y2.p, Cp * Cp, Cp * Cp, w.V.p, 0, 1, 1, s.C1); }
 	phase_mark("mode3");
+	return ep;
+}
+
+// values of num indices (device arrays) from a prepared evaluation, queued on the library stream
+static void eri_apply(const EriPrepared& ep, const int64_t* d_idx, int64_t num, double* d_vals, bool mark)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	if (num <= 0) { return; }
+	const EriSetup& s = ep.s;
+	EriWorkspace& w = eri_ws();
+	if (ep.method == GLI_ERI_DIRECT_FP64) {
+		const int64_t blocks = std::min<int64_t>(num, 8192);
+		k_eri_direct<<<(unsigned)blocks, DIRECT_THREADS, ep.direct_shm, c.stream>>>(d_idx, num, s.g, s.ks->lut_dev.p, w.coords.p, w.B.p,
+			s.Cp, s.M, s.Mp, s.T, w.rep_off.p, w.rep_list.p, ep.cnt, d_vals);
+		GLI_HIP(hipGetLastError());
+		if (mark) { phase_mark("direct"); }
+		return;
+	}
 	launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept, c.gather_xcd, c.gather_lds);
-	phase_mark("gather");
+	if (mark) { phase_mark("gather"); }
+}
+
+static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_idx, int64_t num,
+                           double tol, int method, double* d_vals)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	GLI_REQUIRE(num >= 0, "negative number of indices");
+	const EriPrepared ep = eri_prepare(coeffs, K, grid, tol, method, num > 0);
+	eri_apply(ep, d_idx, num, d_vals, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -2018,24 +2051,41 @@ int gli_eri_values_device(const double* coeffs, int64_t K, const int64_t grid[6]
 	}
 }
 
-int gli_hip_eri_values(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* indices, int64_t num,
-                       double tol, int method, double* values)
+int gli_hip_eri_values_copy(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* indices, int64_t num,
+                            double tol, int method, double* values, int64_t* indices_copy)
 {
 	try {
 		Context& c = ctx();
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
 		if (num == 0) { return 0; }
-		DevBuf<int64_t> di(num);
-		DevBuf<double> dv(num);
-		GLI_HIP(hipMemcpyAsync(di.p, indices, num * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
-		eri_values_dev(coeffs, K, g, di.p, num, tol, method, dv.p);
-		GLI_HIP(hipMemcpyAsync(values, dv.p, num * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+		GLI_REQUIRE(num > 0, "negative number of indices");
+		const EriPrepared ep = eri_prepare(coeffs, K, g, tol, method, true);
+		if ((size_t)num * sizeof(int64_t) <= stream_chunk_bytes()) {
+			// one chunk: plain copies on the library stream
+			DevBuf<int64_t> di(num);
+			DevBuf<double> dv(num);
+			GLI_HIP(hipMemcpyAsync(di.p, indices, num * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+			if (indices_copy != nullptr) { std::memcpy(indices_copy, indices, num * sizeof(int64_t)); }
+			eri_apply(ep, di.p, num, dv.p, false);
+			GLI_HIP(hipMemcpyAsync(values, dv.p, num * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+			GLI_HIP(hipStreamSynchronize(c.stream));
+			return 0;
+		}
+		// indices up and values down in chunks, each chunk's gather between its two copies
+		stream_chunks(indices, sizeof(int64_t), values, sizeof(double), num, indices_copy,
+		              [&](const void* d_in, void* d_out, int64_t n) { eri_apply(ep, (const int64_t*)d_in, n, (double*)d_out, false); });
 		GLI_HIP(hipStreamSynchronize(c.stream));
 		return 0;
 	}
 	catch (const Failure&) {
 		return -1;
 	}
+}
+
+int gli_hip_eri_values(const double* coeffs, int64_t K, const int64_t g[6], const int64_t* indices, int64_t num,
+                       double tol, int method, double* values)
+{
+	return gli_hip_eri_values_copy(coeffs, K, g, indices, num, tol, method, values, nullptr);
 }
 
 int gli_eri_integrals_points(const double* coeffs, int64_t K, const int64_t* points, int64_t num, double tol, int method,

@@ -2,6 +2,7 @@
  * Enumeration, values and the dense / applied / projected views of a sparse
  * set run on the GPU; single lookups are host binary searches
  * (eri_integrals.c:685-707). */
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include "eri_integrals.h"
@@ -97,13 +98,18 @@ void compute_sparse_eri_gausslet_integrals(const struct gausslet_data* gdata,
 	gli_require_coefficients(gdata, "compute_sparse_eri_gausslet_integrals");
 	eri->grid = *grid;
 	eri->num_entries = num_indices;
-	eri->four_indices = aligned_malloc(num_indices * sizeof(glong));
-	eri->integral_values = aligned_malloc(num_indices * sizeof(double));
-	if (num_indices > 0) { memcpy(eri->four_indices, four_indices, num_indices * sizeof(glong)); }
+	eri->four_indices = gli_host_alloc(num_indices * sizeof(glong));
+	eri->integral_values = gli_host_alloc(num_indices * sizeof(double));
+	if (eri->four_indices == NULL || eri->integral_values == NULL) {
+		fprintf(stderr, "gausslectronitegrate: compute_sparse_eri_gausslet_integrals: host allocation of %lld entries failed\n",
+		        (long long)num_indices);
+		abort();
+	}
 	int64_t g[6];
 	gli_grid_array(grid, g);
-	gli_check(gli_hip_eri_values(gdata->coefficients, gdata->indices.num, g, four_indices, num_indices, tol,
-	                             gli_get_eri_method(), eri->integral_values),
+	/* the index copy is made while the indices stream to the device */
+	gli_check(gli_hip_eri_values_copy(gdata->coefficients, gdata->indices.num, g, four_indices, num_indices, tol,
+	                                  gli_get_eri_method(), eri->integral_values, eri->four_indices),
 	          "compute_sparse_eri_gausslet_integrals");
 }
 

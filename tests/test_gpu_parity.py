@@ -92,6 +92,44 @@ def test_eri_enumeration_large_counts(L, orc, n):
     assert np.array_equal(orc.eri_canonical_keys(sym(n), sample), sample)
 
 
+@pytest.mark.parametrize("method", [0, 1])
+def test_host_api_streamed_matches_device(L, method):
+    """The reference entry points with host arrays at 9^3 (39,277,113 entries: ten transfer chunks, the
+    last one ragged): enumerate_symmetry_reduced_eri_indices equals the device-resident list, and
+    compute_sparse_eri_gausslet_integrals returns bit-identical values to the device-resident path
+    together with an exact copy of its input indices."""
+    say("host API streamed 9, method", method)
+    n = 9
+    g = _abi.Grid.of(sym(n))
+    ri = _abi.SparseEriIndices()
+    L.enumerate_symmetry_reduced_eri_indices(C.byref(g), C.byref(ri))
+    host = np.ctypeslib.as_array(ri.four_indices, shape=(ri.num,)).copy()
+    L.delete_sparse_eri_indices(C.byref(ri))
+    d, num = eri_enumerate_device(n)
+    dev = d.to_numpy(np.int64, num)
+    d.free()
+    assert num == burnside.orbit_count(n) and np.array_equal(host, dev)
+    c = golden("coeffs_K57_s0.npy")
+    gd = _abi.GaussletData()
+    cc = np.ascontiguousarray(c)
+    gd.coefficients = cc.ctypes.data_as(_abi.P_double)
+    gd.indices.istart = -(len(c) - 1) // 2
+    gd.indices.num = len(c)
+    L.gli_set_eri_method(method)
+    try:
+        res = _abi.SparseEri()
+        L.compute_sparse_eri_gausslet_integrals(C.byref(gd), C.byref(g), host.ctypes.data_as(_abi.P_glong), num, 1e-16,
+                                                C.byref(res))
+        vals = np.ctypeslib.as_array(res.integral_values, shape=(num,)).copy()
+        keys = np.ctypeslib.as_array(res.four_indices, shape=(num,)).copy()
+        L.delete_sparse_eri_gausslet_integrals(C.byref(res))
+    finally:
+        L.gli_set_eri_method(0)
+    assert np.array_equal(keys, host)
+    ref = eri_values_device(c, sym(n), host, 1e-16, method)
+    assert np.array_equal(vals, ref)
+
+
 def test_eri_enumeration_13_count(L):
     say("enumerate 13")
     d, num = eri_enumerate_device(13)
