@@ -401,6 +401,8 @@ def main():
                     help="comma-separated methods timed beside the main one, each with its error against the main "
                          "FP64 values over every element (e.g. mixed,bf16x2,fp32,fp32x2)")
     ap.add_argument("--no-compare", action="store_true", help="time no other method")
+    ap.add_argument("--no-host-api", action="store_true",
+                    help="skip timing the reference entry points with host arrays (PCIe included)")
     ap.add_argument("--no-plan", action="store_true",
                     help="skip the one-grid batch-plan replay reported beside the single-call rate")
     ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
@@ -587,6 +589,41 @@ def main():
         del pv
         log("one-grid plan replay: %.3f ms/step (plan creation %.0f ms), gather %s" % (plan_ms, create_ms, chosen))
 
+    # the reference entry points with host arrays: indices in and values out over PCIe, the host
+    # copies included (reported beside `value`, never as it)
+    host_api = None
+    if not args.no_host_api:
+        hg = _abi.Grid.of(((-(args.n - 1) // 2, args.n),) * 3)
+        gd = _abi.GaussletData()
+        gd.coefficients = c.ctypes.data_as(_abi.P_double)
+        gd.indices.istart = -(len(c) - 1) // 2
+        gd.indices.num = len(c)
+        _lib.check(L.gli_set_eri_method(method), "method")
+        t_enum, t_val = [], []
+        diff = None
+        for rep in range(2):   # the first call of each also allocates the library's transfer buffers
+            ri = _abi.SparseEriIndices()
+            t0 = time.perf_counter()
+            L.enumerate_symmetry_reduced_eri_indices(C.byref(hg), C.byref(ri))
+            t_enum.append((time.perf_counter() - t0) * 1e3)
+            L.delete_sparse_eri_indices(C.byref(ri))
+            res = _abi.SparseEri()
+            t0 = time.perf_counter()
+            L.compute_sparse_eri_gausslet_integrals(C.byref(gd), C.byref(hg), host_idx.ctypes.data_as(_abi.P_glong), num,
+                                                    args.tol, C.byref(res))
+            t_val.append((time.perf_counter() - t0) * 1e3)
+            if rep == 1:
+                hv = np.ctypeslib.as_array(res.integral_values, shape=(num,))
+                diff = float(np.max(np.abs(hv - vals)))
+                del hv
+            L.delete_sparse_eri_gausslet_integrals(C.byref(res))
+        _lib.check(L.gli_set_eri_method(0), "method")
+        host_api = {"enumerate_ms": t_enum[1], "values_ms": t_val[1], "values_elements_per_s": num / (t_val[1] / 1e3),
+                    "max_abs_diff_vs_device_path": diff,
+                    "note": "enumerate_symmetry_reduced_eri_indices and compute_sparse_eri_gausslet_integrals with "
+                            "host arrays (2nd call), wall clock"}
+        log("host API: enumerate %.1f ms, values %.1f ms (diff vs device path %.3g)" % (t_enum[1], t_val[1], diff))
+
     # roofline of the dominant kernel
     workload = "eri_%dx%dx%d_K%d_seed%d%s" % (args.n, args.n, args.n, args.K, args.seed, "_asym" if args.asymmetric else "")
     dom = max(phases.items(), key=lambda kv: kv[1]) if phases else ("step", step_ms)
@@ -638,6 +675,7 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms": phases,
         "enumeration_ms": enum_ms,
+        "host_api": host_api,
         "fp64_direct": {"elements_per_s": direct_rate, "sample": len(s_idx), "ms": direct_ms,
                         "reference_form_flops_per_element": w_ref.value / len(s_idx),
                         "executed_flops_per_element": w_exe.value / len(s_idx),
