@@ -45,7 +45,9 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 }
 
 // Is the centred 4-tuple P (coordinates relative to the grid centre) the minimum of its orbit?
-// i = t's first index (an orbit-minimal point), IJ = i*N + j, KL = k*N + l.
+// i = t's first index (an orbit-minimal point), IJ = i*N + j, KL = k*N + l, lp = the linear
+// indices (i, j, k, l) of the four points.  A bias placement shifts every point by the same
+// vector, which stays inside the grid, so its linear indices are lp minus one offset.
 //
 // An image's four-index starts with the smallest of its four image points.  For a
 // placement whose points q all have orbit minimum >= i, every image point is >= i, so an
@@ -58,7 +60,7 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 // the identity image of the unshifted tuple is t itself and is skipped (element 0 is the
 // identity).  W[g] holds the signed weights of the original axes in the image's linear
 // index, lin(g q) = W[g] . q + H, so an image point costs three 24-bit multiply-adds.
-__device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], int i, int IJ, int KL,
+__device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], const int lp[4], int i, int IJ, int KL,
                                  const int* __restrict__ minimg, const unsigned long long* __restrict__ canon,
                                  const int4* __restrict__ W)
 {
@@ -69,21 +71,27 @@ __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], int i, in
 		const int mx = max(max(P[0][a], P[1][a]), max(P[2][a], P[3][a]));
 		c[a] = mn + mx;
 	}
+	const int cx = __mul24(c[0], n2), cy = __mul24(c[1], g.n), cz = c[2];
 	for (int bx = 0; bx <= abs(c[0]); ++bx) {
 		for (int by = 0; by <= abs(c[1]); ++by) {
 			for (int bz = 0; bz <= abs(c[2]); ++bz) {
-				int Q[4][3];
+				// placement P - b*c: every point's linear index moves by the same offset
+				const int off = (bx ? cx : 0) + (by ? cy : 0) + (bz ? cz : 0);
 				unsigned long long emask = 0;
 				for (int p = 0; p < 4; ++p) {
-					Q[p][0] = P[p][0] - bx * c[0];
-					Q[p][1] = P[p][1] - by * c[1];
-					Q[p][2] = P[p][2] - bz * c[2];
-					const int lq = __mul24(Q[p][0], n2) + __mul24(Q[p][1], g.n) + Q[p][2] + H;
+					const int lq = lp[p] - off;
 					const int mi = minimg[lq];
 					if (mi < i) { return false; }
 					if (mi == i) { emask |= canon[lq]; }
 				}
 				if ((bx | by | bz) == 0) { emask &= ~1ull; }
+				if (emask == 0ull) { continue; }
+				int Q[4][3];
+				for (int p = 0; p < 4; ++p) {
+					Q[p][0] = P[p][0] - (bx ? c[0] : 0);
+					Q[p][1] = P[p][1] - (by ? c[1] : 0);
+					Q[p][2] = P[p][2] - (bz ? c[2] : 0);
+				}
 				for (; emask; emask &= emask - 1) {
 					const int4 w = W[__builtin_ctzll(emask)];
 					int L[4];
@@ -135,7 +143,7 @@ __device__ inline void axis_interval(int h, int pi, int pj, int pk, int& start, 
 __device__ inline uint32_t udiv_small(uint32_t v, uint32_t d)
 {
 	uint32_t q = (uint32_t)((float)v * __builtin_amdgcn_rcpf((float)d));
-	const int r = (int)v - (int)(q * d);
+	const int r = (int)v - (int)__umul24(q, d);
 	if (r < 0) { q -= 1; }
 	else if (r >= (int)d) { q += 1; }
 	return q;
@@ -235,24 +243,30 @@ __device__ uint32_t enum_load_rows(const EnumGrid& g, const EnumItem& it, EnumRo
 	return total;
 }
 
-// candidate q < total of the loaded item -> k, its coordinates K and the point L (relative to the centre)
-__device__ inline void enum_candidate(const EnumGrid& g, const EnumRows& s, int rows, uint32_t q, int& k, int K[3], int L[3])
+// the row of candidate q < total of the loaded item: the last row r with pre[r] <= q (it has candidates)
+__device__ inline int enum_row(const EnumRows& s, int rows, uint32_t q)
 {
-	int lo = 0, hi = rows;   // the last row r with pre[r] <= q (it has candidates)
+	int lo = 0, hi = rows;
 	while (hi - lo > 1) {
 		const int mid = (lo + hi) >> 1;
 		if (s.pre[mid] <= q) { lo = mid; }
 		else { hi = mid; }
 	}
-	const uint32_t ksw = s.ks[lo], iv = s.iv[lo], kc = s.kc[lo];
+	return lo;
+}
+
+// candidate q of row r -> k, its coordinates K and the point L (relative to the centre)
+__device__ inline void enum_decode(const EnumGrid& g, const EnumRows& s, int r, uint32_t q, int& k, int K[3], int L[3])
+{
+	const uint32_t ksw = s.ks[r], iv = s.iv[r], kc = s.kc[r];
 	k = (int)(ksw & 0xffffu);
 	for (int a = 0; a < 3; ++a) { K[a] = (int)((kc >> (5 * a)) & 31u) - g.h; }
-	const uint32_t v = q - s.pre[lo] + (ksw >> 16);
+	const uint32_t v = q - s.pre[r] + (ksw >> 16);
 	const uint32_t lz = (iv >> 25) & 31u, ly = (iv >> 15) & 31u;
 	const uint32_t vz = udiv_small(v, lz);
-	L[2] = (int)((iv >> 20) & 31u) - g.h + (int)(v - vz * lz);
+	L[2] = (int)((iv >> 20) & 31u) - g.h + (int)(v - __umul24(vz, lz));
 	const uint32_t vy = udiv_small(vz, ly);
-	L[1] = (int)((iv >> 10) & 31u) - g.h + (int)(vz - vy * ly);
+	L[1] = (int)((iv >> 10) & 31u) - g.h + (int)(vz - __umul24(vy, ly));
 	L[0] = (int)(iv & 31u) - g.h + (int)vy;
 }
 
@@ -295,11 +309,12 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __r
 		bool keep = false;
 		if (q < tot) {
 			int k, L[3];
-			enum_candidate(g, s, it.rows, q, k, P[2], L);
-			const int l = lin_c(g, L[0], L[1], L[2]);
+			enum_decode(g, s, enum_row(s, it.rows, q), q, k, P[2], L);
+			const int l = __mul24(__mul24(L[0] + g.h, g.n) + L[1] + g.h, g.n) + L[2] + g.h;
 			if (minimg[l] >= it.i) {
 				P[3][0] = L[0]; P[3][1] = L[1]; P[3][2] = L[2];
-				keep = eri_is_canonical(g, P, it.i, it.IJ, __mul24(k, g.N) + l, minimg, canon, W);
+				const int lp[4] = { it.i, it.j, k, l };
+				keep = eri_is_canonical(g, P, lp, it.i, it.IJ, __mul24(k, g.N) + l, minimg, canon, W);
 			}
 		}
 		const unsigned long long m = __ballot(keep);
@@ -338,9 +353,13 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_expand(EnumGrid g, const int2* _
 		unsigned long long m = (w < nw) ? bits[word + w] : 0ull;
 		uint32_t total;
 		uint32_t rank = enum_block_scan((uint32_t)__popcll(m), s.wsum, total);
+		// the bits of one word are 64 consecutive candidates: one search, then forward steps
+		int r = (m != 0ull) ? enum_row(s, it.rows, w * 64 + (uint32_t)__builtin_ctzll(m)) : 0;
 		for (; m; m &= m - 1) {
+			const uint32_t q = w * 64 + (uint32_t)__builtin_ctzll(m);
+			while (r + 1 < it.rows && s.pre[r + 1] <= q) { ++r; }
 			int k, K[3], L[3];
-			enum_candidate(g, s, it.rows, w * 64 + (uint32_t)__builtin_ctzll(m), k, K, L);
+			enum_decode(g, s, r, q, k, K, L);
 			out[pos + rank++] = (ij * g.N + k) * g.N + lin_c(g, L[0], L[1], L[2]);
 		}
 		pos += total;
