@@ -183,6 +183,11 @@ void stream_chunks(const void* host_in, size_t in_bytes, void* host_out, size_t 
 size_t stream_chunk_bytes();
 // device-to-host copy through the transfer ring, after the library stream's queued work
 void download(void* host_dst, const void* dev_src, size_t bytes);
+// host-to-device copy through the transfer ring; the library stream's later work sees the data
+void upload(void* dev_dst, const void* host_src, size_t bytes);
+// synchronous copies of result or input arrays of any size: direct below 4 MB, else through the ring
+void copy_to_host(void* host_dst, const void* dev_src, size_t bytes);
+void copy_to_device(void* dev_dst, const void* host_src, size_t bytes);
 
 // phase timing (records an event on the library stream when enabled)
 void phase_begin();

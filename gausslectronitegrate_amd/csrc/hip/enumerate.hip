@@ -872,8 +872,8 @@ static void sparse_to_device(const int64_t gs[6], const int64_t* keys, const dou
 	GLI_HIP(hipMemcpyAsync(d.minimg.p, minimg.data(), minimg.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(d.canon.p, canon.data(), canon.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
 	if (num > 0) {
-		GLI_HIP(hipMemcpyAsync(d.keys.p, keys, num * sizeof(int64_t), hipMemcpyHostToDevice, st));
-		GLI_HIP(hipMemcpyAsync(d.vals.p, vals, num * sizeof(double), hipMemcpyHostToDevice, st));
+		copy_to_device(d.keys.p, keys, (size_t)num * sizeof(int64_t));
+		copy_to_device(d.vals.p, vals, (size_t)num * sizeof(double));
 	}
 	GLI_HIP(hipStreamSynchronize(st));
 }
@@ -929,7 +929,7 @@ int gli_hip_eri_apply(const int64_t gs[6], const int64_t* keys, const double* va
 		const int64_t N2 = (int64_t)sd.g.N * sd.g.N;
 		if (S == 0) { return 0; }
 		DevBuf<double> dst(N2 * S), dret(N2 * S);
-		GLI_HIP(hipMemcpyAsync(dst.p, states, N2 * S * sizeof(double), hipMemcpyHostToDevice, c.stream));
+		copy_to_device(dst.p, states, (size_t)(N2 * S) * sizeof(double));
 		const int64_t rows_per_launch = 1 << 20;
 		for (int64_t s0 = 0; s0 < S; s0 += APPLY_S) {
 			const int sn = (int)std::min<int64_t>(APPLY_S, S - s0);
@@ -940,8 +940,7 @@ int gli_hip_eri_apply(const int64_t gs[6], const int64_t* keys, const double* va
 				GLI_HIP(hipGetLastError());
 			}
 		}
-		GLI_HIP(hipMemcpyAsync(ret, dret.p, N2 * S * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-		GLI_HIP(hipStreamSynchronize(c.stream));
+		copy_to_host(ret, dret.p, (size_t)(N2 * S) * sizeof(double));
 		return 0;
 	}
 	catch (const Failure&) {

@@ -155,8 +155,7 @@ static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 	mode_product(Y2.p, (int64_t)C[0] * C[1], (int64_t)C[0] * C[1], F2.p, C[2], C[2], U[2], V.p, C[2], 0, 0, 0, 0, 0);
 	k_nuclear_scatter<<<(unsigned)ceil_div(N * N, 256), 256, 0, st>>>(V.p, n[0], n[1], n[2], M.p);
 	GLI_HIP(hipGetLastError());
-	GLI_HIP(hipMemcpyAsync(out, M.p, N * N * sizeof(double), hipMemcpyDeviceToHost, st));
-	GLI_HIP(hipStreamSynchronize(st));
+	copy_to_host(out, M.p, (size_t)(N * N) * sizeof(double));
 }
 
 // ---------------------------------------------------------------------------
@@ -208,8 +207,7 @@ static void kinetic_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 	k_kinetic_1d<<<R, 256, 0, st>>>(dc.p, (int)K, t.p);
 	k_kinetic_fill<<<(unsigned)ceil_div(N * N, 256), 256, 0, st>>>(t.p, (int)g[1], (int)g[3], (int)g[5], M.p);
 	GLI_HIP(hipGetLastError());
-	GLI_HIP(hipMemcpyAsync(out, M.p, N * N * sizeof(double), hipMemcpyDeviceToHost, st));
-	GLI_HIP(hipStreamSynchronize(st));
+	copy_to_host(out, M.p, (size_t)(N * N) * sizeof(double));
 }
 
 // ---------------------------------------------------------------------------

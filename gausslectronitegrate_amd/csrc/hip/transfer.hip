@@ -12,6 +12,7 @@ namespace {
 
 constexpr int kSlots = 4;
 constexpr size_t kChunkBytes = (size_t)32 << 20;
+constexpr size_t kRingMinBytes = (size_t)4 << 20;   // smaller copies go directly, without the ring
 
 struct XferRing {
 	char* pin_in[kSlots] = {};
@@ -121,6 +122,56 @@ void download(void* host_dst, const void* dev_src, size_t bytes)
 			GLI_HIP(hipEventRecord(r.ev_done[b], r.down));
 		}
 	}
+}
+
+void upload(void* dev_dst, const void* host_src, size_t bytes)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	if (bytes == 0) { return; }
+	XferRing& r = ring();
+	// the destination may still be read by the library stream's queued work
+	GLI_HIP(hipEventRecord(r.ev_work[0], c.stream));
+	GLI_HIP(hipStreamWaitEvent(r.up, r.ev_work[0], 0));
+	const int64_t nc = (int64_t)ceil_div((int64_t)bytes, (int64_t)kChunkBytes);
+	auto len = [&](int64_t ch) { return std::min(kChunkBytes, bytes - (size_t)ch * kChunkBytes); };
+	for (int64_t ch = 0; ch < nc; ++ch) {
+		const int b = (int)(ch % kSlots);
+		if (ch >= kSlots) { GLI_HIP(hipEventSynchronize(r.ev_up[b])); }   // pinned buffer b is free again
+		gli_host_copy(r.pin_in[b], nullptr, (const char*)host_src + (size_t)ch * kChunkBytes, len(ch));
+		GLI_HIP(hipMemcpyAsync((char*)dev_dst + (size_t)ch * kChunkBytes, r.pin_in[b], len(ch), hipMemcpyHostToDevice, r.up));
+		GLI_HIP(hipEventRecord(r.ev_up[b], r.up));
+	}
+	// later work on the library stream sees the data
+	GLI_HIP(hipEventRecord(r.ev_work[0], r.up));
+	GLI_HIP(hipStreamWaitEvent(c.stream, r.ev_work[0], 0));
+	GLI_HIP(hipStreamSynchronize(r.up));   // the source may be reused when this returns
+}
+
+void copy_to_host(void* host_dst, const void* dev_src, size_t bytes)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	if (bytes == 0) { return; }
+	if (bytes < kRingMinBytes) {
+		GLI_HIP(hipMemcpyAsync(host_dst, dev_src, bytes, hipMemcpyDeviceToHost, c.stream));
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		return;
+	}
+	download(host_dst, dev_src, bytes);
+}
+
+void copy_to_device(void* dev_dst, const void* host_src, size_t bytes)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	if (bytes == 0) { return; }
+	if (bytes < kRingMinBytes) {
+		GLI_HIP(hipMemcpyAsync(dev_dst, host_src, bytes, hipMemcpyHostToDevice, c.stream));
+		GLI_HIP(hipStreamSynchronize(c.stream));
+		return;
+	}
+	upload(dev_dst, host_src, bytes);
 }
 
 }  // namespace gli

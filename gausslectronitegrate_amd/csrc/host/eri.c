@@ -191,8 +191,8 @@ void project_sparse_eri_gausslet_integrals(const struct sparse_eri_gausslet_inte
 	const glong S = num_states, S2 = S * S;
 	memset(eri_proj, 0, S2 * S2 * sizeof(double));
 	if (S == 0) { return; }
-	double* x = aligned_malloc(N * N * S2 * sizeof(double));
-	double* R = aligned_malloc(N * N * S2 * sizeof(double));
+	double* x = gli_host_alloc(N * N * S2 * sizeof(double));
+	double* R = gli_host_alloc(N * N * S2 * sizeof(double));
 	double* T = aligned_calloc(N * S * S2 * sizeof(double));
 	#pragma omp parallel for
 	for (glong j = 0; j < N; ++j) {

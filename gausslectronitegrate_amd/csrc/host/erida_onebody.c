@@ -216,7 +216,7 @@ void compute_nuclear_gausslet_integrals(const struct gausslet_data* gdata, const
 	if (num_nuclei > 0) { memcpy(ngi->nuclei, nuclei, num_nuclei * sizeof(struct atomic_nucleus)); }
 	ngi->num_nuclei = num_nuclei;
 	const glong N = cartesian_grid_3d_num_points(grid);
-	ngi->integral_values = aligned_malloc(N * N * sizeof(double));
+	ngi->integral_values = gli_host_alloc(N * N * sizeof(double));
 	double* pos = malloc((3 * num_nuclei + 1) * sizeof(double));
 	int* chg = malloc((num_nuclei + 1) * sizeof(int));
 	nucleus_arrays(nuclei, num_nuclei, pos, chg);
@@ -274,7 +274,7 @@ void compute_kinetic_gausslet_integrals(const struct gausslet_data* gdata, const
 {
 	kgi->grid = *grid;
 	const glong N = cartesian_grid_3d_num_points(grid);
-	kgi->integral_values = aligned_malloc(N * N * sizeof(double));
+	kgi->integral_values = gli_host_alloc(N * N * sizeof(double));
 	int64_t g[6];
 	gli_grid_array(grid, g);
 	gli_check(gli_hip_kinetic_matrix(gdata->coefficients, gdata->indices.num, g, kgi->integral_values), "compute_kinetic_gausslet_integrals");
