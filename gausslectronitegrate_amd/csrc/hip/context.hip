@@ -268,6 +268,10 @@ int gli_set_tuning(const char* knob, int value)
 			GLI_REQUIRE(value == 0 || value == 1, "batch_positions must be 0 or 1");
 			c.batch_positions = value;
 		}
+		else if (k == "batch_sorted") {
+			GLI_REQUIRE(value == 0 || value == 1, "batch_sorted must be 0 or 1");
+			c.batch_sorted = value;
+		}
 		else if (k == "batch_branches") {
 			GLI_REQUIRE(value >= 0 && value <= 2, "batch_branches must be 0, 1 or 2");
 			c.batch_branches = value;
@@ -310,6 +314,7 @@ int gli_get_tuning(const char* knob)
 		if (k == "mode1_half") { return c.mode1_half; }
 		if (k == "batch_branches") { return c.batch_branches; }
 		if (k == "batch_positions") { return c.batch_positions; }
+		if (k == "batch_sorted") { return c.batch_sorted; }
 		if (k == "gemm_shape") { return c.gemm_shape; }
 		GLI_REQUIRE(false, "unknown tuning knob");
 	}

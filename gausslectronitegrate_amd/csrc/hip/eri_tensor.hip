@@ -579,6 +579,7 @@ struct GatherDesc {
 	int nlut, C1;          // entries of the key table and of the tetrahedral offsets
 	uint32_t* pos;         // key-tensor position of every element (written by k_positions, read by
 	                       // k_gather_cached; kNoPosition for indices outside the grid), or nullptr
+	const uint16_t* perm;  // with positions sorted inside windows: each position's place in its window
 };
 constexpr uint32_t kNoPosition = 0xffffffffu;
 
@@ -780,6 +781,77 @@ __global__ __launch_bounds__(256) void k_gather_cached_grouped(const GatherDesc*
 	for (int u = 0; u < GATHER_EPT; ++u) {
 		const int64_t e = wave_base + u * 64 + lane;
 		if (e < d.num) { d.out[e] = v[u]; }
+	}
+}
+
+// Positions sorted inside windows of kSortWin consecutive elements (a plan's one-time pass): the
+// gather then visits each window in key-tensor order, so the 64 reads of a wave fall on fewer
+// cache lines, and stages the values in LDS to store them in list order.
+constexpr int kSortWin = 4096;   // = 256 lanes x 16 elements: the gather pieces of elements-per-lane 16
+
+// sort each window's positions ascending (bitonic, in LDS); perm receives each one's place
+__global__ __launch_bounds__(256) void k_sort_windows(uint32_t* __restrict__ pos, uint16_t* __restrict__ perm, int64_t num)
+{
+	__shared__ unsigned long long key[kSortWin];
+	const int64_t base = (int64_t)blockIdx.x * kSortWin;
+	const int cnt = (int)min((int64_t)kSortWin, num - base);
+	for (int i = threadIdx.x; i < kSortWin; i += blockDim.x) {
+		key[i] = (i < cnt) ? (((unsigned long long)pos[base + i] << 16) | (unsigned)i) : ~0ull;
+	}
+	__syncthreads();
+	for (int k = 2; k <= kSortWin; k <<= 1) {
+		for (int j = k >> 1; j > 0; j >>= 1) {
+			for (int i = threadIdx.x; i < kSortWin; i += blockDim.x) {
+				const int p = i ^ j;
+				if (p > i) {
+					const unsigned long long a = key[i], b = key[p];
+					if ((a > b) == ((i & k) == 0)) { key[i] = b; key[p] = a; }
+				}
+			}
+			__syncthreads();
+		}
+	}
+	for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+		const unsigned long long v = key[i];
+		pos[base + i] = (uint32_t)(v >> 16);
+		perm[base + i] = (uint16_t)(v & 0xffffu);
+	}
+}
+
+// values from window-sorted positions; all grids of a batch in one launch (pieces = windows)
+__global__ __launch_bounds__(256) void k_gather_sorted_grouped(const GatherDesc* __restrict__ descs, int n, int64_t xcd_chunks)
+{
+	__shared__ double so[kSortWin];
+	const int64_t piece = gather_piece(xcd_chunks);
+	int gi = 0;
+	while (gi + 1 < n && piece >= descs[gi + 1].piece_begin) { ++gi; }
+	const GatherDesc d = descs[gi];
+	const int64_t base = (piece - d.piece_begin) * kSortWin;
+	if (piece < d.piece_begin || base >= d.num) { return; }   // whole workgroup idle
+	const int cnt = (int)min((int64_t)kSortWin, d.num - base);
+	constexpr int E = kSortWin / 256;
+	uint32_t q[E];
+	uint16_t p[E];
+#pragma unroll
+	for (int r = 0; r < E; ++r) {
+		const int i = r * 256 + threadIdx.x;
+		q[r] = (i < cnt) ? d.pos[base + i] : kNoPosition;
+		p[r] = (i < cnt) ? d.perm[base + i] : 0;
+	}
+	double v[E];
+#pragma unroll
+	for (int r = 0; r < E; ++r) { v[r] = (q[r] != kNoPosition) ? d.V[q[r]] : __builtin_nan(""); }
+	const double sc = (d.scale != nullptr) ? *d.scale : 1.0;
+#pragma unroll
+	for (int r = 0; r < E; ++r) {
+		const int i = r * 256 + threadIdx.x;
+		if (i < cnt) { so[p[r]] = (d.scale != nullptr) ? v[r] * sc : v[r]; }
+	}
+	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < E; ++r) {
+		const int i = r * 256 + threadIdx.x;
+		if (i < cnt) { d.out[base + i] = so[i]; }
 	}
 }
 
@@ -1144,6 +1216,7 @@ static GatherDesc gather_desc(const EriSetup& s, const int64_t* d_idx, int64_t n
 	d.nlut = s.R * s.R * (4 * s.R - 3);
 	d.C1 = s.C1;
 	d.pos = nullptr;
+	d.perm = nullptr;
 	return d;
 }
 
@@ -1229,6 +1302,16 @@ static void launch_gather_cached_grouped(const GatherDesc* d_descs, int n, int64
 		case 8: k_gather_cached_grouped<8><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); break;
 		default: k_gather_cached_grouped<4><<<(unsigned)blocks, 256, 0, st>>>(d_descs, n, chunks); break;
 	}
+	GLI_HIP(hipGetLastError());
+}
+
+static void launch_gather_sorted_grouped(const GatherDesc* d_descs, int n, int64_t total_pieces, int xcd)
+{
+	if (total_pieces == 0) { return; }
+	const int64_t chunks = xcd ? ceil_div(total_pieces, 8) : 0;
+	const int64_t blocks = xcd ? 8 * chunks : total_pieces;
+	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
+	k_gather_sorted_grouped<<<(unsigned)blocks, 256, 0, ctx().stream>>>(d_descs, n, chunks);
 	GLI_HIP(hipGetLastError());
 }
 
@@ -1505,6 +1588,7 @@ struct BatchGrid {
 	double* d_vals = nullptr;
 	DevBuf<double> f, P, B, Y1, Y2, V;
 	DevBuf<uint32_t> pos;   // key-tensor position of every element (plans with cached positions)
+	DevBuf<uint16_t> perm;  // place of each sorted position in its window (plans with sorted windows)
 	const double* T3 = nullptr;
 	size_t coeff_off = 0, scale_off = 0;
 };
@@ -1532,6 +1616,7 @@ struct BatchPlan {
 	int lds_bytes = 0;   // LDS for the largest key table of the batch
 	bool half = false;   // half-stored first mode (FP64 plans)
 	bool cached = false; // the gather reads positions computed once at creation instead of decoding
+	bool sorted = false; // those positions are sorted inside windows (k_sort_windows, k_gather_sorted_grouped)
 	// two-branch graph: the grid with the most elements on the library stream, the others on a
 	// second stream (default priority) forked after the tables and joined before the end
 	bool branches = false;
@@ -1585,7 +1670,8 @@ static ModeArgs batch_mode_args(const BatchGrid& bg, int mode, bool half)
 static void batch_launch_gather(const BatchPlan& plan, const GatherDesc* descs, int n, int64_t pieces, int ept, int order,
                                 int lds)
 {
-	if (plan.cached) { launch_gather_cached_grouped(descs, n, pieces, ept, order); }
+	if (plan.sorted) { launch_gather_sorted_grouped(descs, n, pieces, order); }   // pieces of 16 elements per lane
+	else if (plan.cached) { launch_gather_cached_grouped(descs, n, pieces, ept, order); }
 	else { launch_gather_grouped(descs, n, pieces, ept, order, lds, plan.lds_bytes); }
 }
 
@@ -1598,6 +1684,7 @@ static void batch_gather_descs(BatchPlan& plan, int ept)
 	for (auto& g : plan.grids) {
 		GatherDesc d = gather_desc(g->s, g->d_idx, g->num, g->V.p, g->d_vals, plan.d_params.p + g->scale_off);
 		d.pos = plan.cached ? g->pos.p : nullptr;
+		d.perm = plan.sorted ? g->perm.p : nullptr;
 		d.piece_begin = pieces;
 		pieces += gather_pieces(g->num, ept);
 		h.push_back(d);
@@ -1634,6 +1721,7 @@ static void batch_gather_descs_branches(BatchPlan& plan, int ept)
 		const BatchGrid& g = *plan.grids[gi];
 		GatherDesc d = gather_desc(g.s, g.d_idx, g.num, g.V.p, g.d_vals, plan.d_params.p + g.scale_off);
 		d.pos = plan.cached ? g.pos.p : nullptr;
+		d.perm = plan.sorted ? g.perm.p : nullptr;
 		const bool is_lead = (int)gi == plan.lead;
 		int64_t& pieces = is_lead ? pl : pr;
 		d.piece_begin = pieces;
@@ -1825,6 +1913,7 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 		if (lds && (plan.cached || plan.lds_bytes > kGatherLdsMax)) { continue; }   // no key tables with cached positions
 		for (int ept : { 2, 4, 8, 16 }) {
 			if (plan.cached && ept == 2) { continue; }
+			if (plan.sorted && ept != 16) { continue; }   // the windows are the pieces of 16 elements per lane
 			for (int order = 0; order < 2; ++order) {
 				batch_gather_descs(plan, ept);
 				const double t = time_ms([&] {
@@ -1843,7 +1932,8 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 		}
 	}
 	rep += "], \"gather_chosen\": {\"ept\": " + std::to_string(ch.gather_ept) + ", \"order\": " + std::to_string(ch.gather_order) +
-	       ", \"lds\": " + std::to_string(ch.gather_lds) + ", \"cached_positions\": " + (plan.cached ? "true" : "false") + "}}";
+	       ", \"lds\": " + std::to_string(ch.gather_lds) + ", \"cached_positions\": " + (plan.cached ? "true" : "false") +
+	       ", \"sorted_windows\": " + (plan.sorted ? "true" : "false") + "}}";
 	plan.report = rep;
 	return ch;
 }
@@ -1922,9 +2012,15 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 			GatherDesc d = gather_desc(g->s, g->d_idx, g->num, nullptr, nullptr, nullptr);
 			d.pos = g->pos.p;
 			launch_positions(d);
+			if (c.batch_sorted && g->num > 0) {
+				g->perm.alloc((size_t)g->num);
+				k_sort_windows<<<(unsigned)ceil_div(g->num, kSortWin), 256, 0, c.stream>>>(g->pos.p, g->perm.p, g->num);
+				GLI_HIP(hipGetLastError());
+			}
 		}
 		GLI_HIP(hipStreamSynchronize(c.stream));
 		plan->cached = true;
+		plan->sorted = c.batch_sorted != 0;
 	}
 
 	BatchChoice ch;
@@ -1939,7 +2035,7 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 				if (!gemm_shape_fits(ch.gemm_shape[m], a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0 || a.sym_x, a.p_base)) { ch.gemm_shape[m] = 0; }
 			}
 		}
-		ch.gather_ept = c.gather_ept;
+		ch.gather_ept = plan->sorted ? 16 : c.gather_ept;
 		ch.gather_order = c.gather_xcd;
 		ch.gather_lds = c.gather_lds;
 		plan->report = "{\"autotuned\": false}";

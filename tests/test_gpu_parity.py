@@ -520,18 +520,21 @@ def _batch_arrays(entries, coeffs=None):
 
 
 @pytest.mark.parametrize("method", [0, 1])
-@pytest.mark.parametrize("branches,positions", [(1, 1), (2, 0)])
-def test_batch_plan_matches_single_grids(L, method, branches, positions):
+@pytest.mark.parametrize("branches,positions,sorted_windows", [(1, 1, 1), (2, 0, 1), (1, 1, 0)])
+def test_batch_plan_matches_single_grids(L, method, branches, positions, sorted_windows):
     """a 4-grid plan (uneven K, one non-cubic grid with an unsorted sample): every replay equals the
     single-grid path bit for bit (FP64; the same kernels in the same summation order) or within
     1e-10 of FP64 (mixed, whose autotuner may keep FP64 products where they are faster); output
     scales, new coefficients and a symmetry change behave as documented.  branches = 2 forces the
     two-branch graph (largest grid beside the others on a second stream) wherever it applies;
-    positions = 1 caches every element's key-tensor position at plan creation, 0 decodes per replay."""
-    say("batch plan method", method, "branches", branches, "positions", positions)
-    saved = L.gli_get_tuning(b"batch_branches"), L.gli_get_tuning(b"batch_positions")
+    positions = 1 caches every element's key-tensor position at plan creation, 0 decodes per replay;
+    sorted_windows = 1 also sorts the cached positions inside windows of 4096 elements (the 3^3 list and
+    the last window of every list are partial)."""
+    say("batch plan method", method, "branches", branches, "positions", positions, "sorted", sorted_windows)
+    saved = L.gli_get_tuning(b"batch_branches"), L.gli_get_tuning(b"batch_positions"), L.gli_get_tuning(b"batch_sorted")
     assert L.gli_set_tuning(b"batch_branches", branches) == 0
     assert L.gli_set_tuning(b"batch_positions", positions) == 0
+    assert L.gli_set_tuning(b"batch_sorted", sorted_windows) == 0
     rng = np.random.default_rng(11)
     ragged = rng.integers(0, (5 * 3 * 4) ** 4, size=5000)
     specs = [(sym(3), golden("coeffs_K29_s0.npy"), None),
@@ -548,6 +551,7 @@ def test_batch_plan_matches_single_grids(L, method, branches, positions):
     finally:
         L.gli_set_tuning(b"batch_branches", saved[0])
         L.gli_set_tuning(b"batch_positions", saved[1])
+        L.gli_set_tuning(b"batch_sorted", saved[2])
     try:
         import json
         buf = C.create_string_buffer(1 << 16)
@@ -555,6 +559,7 @@ def test_batch_plan_matches_single_grids(L, method, branches, positions):
         report = json.loads(buf.value.decode())
         assert len(report["modes"]) == 3 and report["gather"]
         assert report["gather_chosen"]["cached_positions"] == bool(positions)
+        assert report["gather_chosen"]["sorted_windows"] == bool(positions and sorted_windows)
         split_any = any(m["chosen"] != "fp64" for m in report["modes"])
         if not split_any:
             assert report["branches"]["chosen"] == ("two" if branches == 2 else report["branches"]["chosen"])
