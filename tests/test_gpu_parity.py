@@ -482,6 +482,9 @@ def test_tuning_variants_bit_identical(L):
         assert L.gli_set_tuning(b"gather_ept", 3) == -1
         assert L.gli_set_tuning(b"gather_lds", 2) == -1
         assert L.gli_set_tuning(b"mode1_half", 2) == -1
+        assert L.gli_set_tuning(b"batch_sorted", 2) == -1
+        assert L.gli_set_tuning(b"batch_positions", -1) == -1
+        assert L.gli_get_tuning(b"batch_sorted") in (0, 1)
         assert L.gli_set_tuning(b"no_such_knob", 1) == -1
         assert L.gli_get_tuning(b"no_such_knob") == -1
     finally:
