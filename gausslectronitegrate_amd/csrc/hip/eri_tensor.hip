@@ -786,11 +786,13 @@ __global__ __launch_bounds__(256) void k_gather_cached_grouped(const GatherDesc*
 
 // Positions sorted inside windows of kSortWin consecutive elements (a plan's one-time pass): the
 // gather then visits each window in key-tensor order, so the 64 reads of a wave fall on fewer
-// cache lines, and stages the values in LDS to store them in list order.
-constexpr int kSortWin = 4096;   // = 256 lanes x 16 elements: the gather pieces of elements-per-lane 16
+// cache lines, and stages the values in LDS (64 KB per workgroup) to store them in list order.
+constexpr int kSortThreads = 1024;
+constexpr int kSortWin = 8192;   // = 1024 lanes x 8 elements; as gather pieces: 256 lanes x 32 elements
+constexpr int kSortEpt = kSortWin / 256;   // the elements-per-lane value whose pieces are the windows
 
 // sort each window's positions ascending (bitonic, in LDS); perm receives each one's place
-__global__ __launch_bounds__(256) void k_sort_windows(uint32_t* __restrict__ pos, uint16_t* __restrict__ perm, int64_t num)
+__global__ __launch_bounds__(kSortThreads) void k_sort_windows(uint32_t* __restrict__ pos, uint16_t* __restrict__ perm, int64_t num)
 {
 	__shared__ unsigned long long key[kSortWin];
 	const int64_t base = (int64_t)blockIdx.x * kSortWin;
@@ -819,7 +821,7 @@ __global__ __launch_bounds__(256) void k_sort_windows(uint32_t* __restrict__ pos
 }
 
 // values from window-sorted positions; all grids of a batch in one launch (pieces = windows)
-__global__ __launch_bounds__(256) void k_gather_sorted_grouped(const GatherDesc* __restrict__ descs, int n, int64_t xcd_chunks)
+__global__ __launch_bounds__(kSortThreads) void k_gather_sorted_grouped(const GatherDesc* __restrict__ descs, int n, int64_t xcd_chunks)
 {
 	__shared__ double so[kSortWin];
 	const int64_t piece = gather_piece(xcd_chunks);
@@ -829,12 +831,12 @@ __global__ __launch_bounds__(256) void k_gather_sorted_grouped(const GatherDesc*
 	const int64_t base = (piece - d.piece_begin) * kSortWin;
 	if (piece < d.piece_begin || base >= d.num) { return; }   // whole workgroup idle
 	const int cnt = (int)min((int64_t)kSortWin, d.num - base);
-	constexpr int E = kSortWin / 256;
+	constexpr int E = kSortWin / kSortThreads;
 	uint32_t q[E];
 	uint16_t p[E];
 #pragma unroll
 	for (int r = 0; r < E; ++r) {
-		const int i = r * 256 + threadIdx.x;
+		const int i = r * kSortThreads + threadIdx.x;
 		q[r] = (i < cnt) ? d.pos[base + i] : kNoPosition;
 		p[r] = (i < cnt) ? d.perm[base + i] : 0;
 	}
@@ -844,13 +846,13 @@ __global__ __launch_bounds__(256) void k_gather_sorted_grouped(const GatherDesc*
 	const double sc = (d.scale != nullptr) ? *d.scale : 1.0;
 #pragma unroll
 	for (int r = 0; r < E; ++r) {
-		const int i = r * 256 + threadIdx.x;
+		const int i = r * kSortThreads + threadIdx.x;
 		if (i < cnt) { so[p[r]] = (d.scale != nullptr) ? v[r] * sc : v[r]; }
 	}
 	__syncthreads();
 #pragma unroll
 	for (int r = 0; r < E; ++r) {
-		const int i = r * 256 + threadIdx.x;
+		const int i = r * kSortThreads + threadIdx.x;
 		if (i < cnt) { d.out[base + i] = so[i]; }
 	}
 }
@@ -1311,7 +1313,7 @@ static void launch_gather_sorted_grouped(const GatherDesc* d_descs, int n, int64
 	const int64_t chunks = xcd ? ceil_div(total_pieces, 8) : 0;
 	const int64_t blocks = xcd ? 8 * chunks : total_pieces;
 	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
-	k_gather_sorted_grouped<<<(unsigned)blocks, 256, 0, ctx().stream>>>(d_descs, n, chunks);
+	k_gather_sorted_grouped<<<(unsigned)blocks, kSortThreads, 0, ctx().stream>>>(d_descs, n, chunks);
 	GLI_HIP(hipGetLastError());
 }
 
@@ -1670,7 +1672,7 @@ static ModeArgs batch_mode_args(const BatchGrid& bg, int mode, bool half)
 static void batch_launch_gather(const BatchPlan& plan, const GatherDesc* descs, int n, int64_t pieces, int ept, int order,
                                 int lds)
 {
-	if (plan.sorted) { launch_gather_sorted_grouped(descs, n, pieces, order); }   // pieces of 16 elements per lane
+	if (plan.sorted) { launch_gather_sorted_grouped(descs, n, pieces, order); }   // pieces of kSortEpt elements per lane
 	else if (plan.cached) { launch_gather_cached_grouped(descs, n, pieces, ept, order); }
 	else { launch_gather_grouped(descs, n, pieces, ept, order, lds, plan.lds_bytes); }
 }
@@ -1911,9 +1913,9 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 	bool first = true;
 	for (int lds = 0; lds < 2; ++lds) {
 		if (lds && (plan.cached || plan.lds_bytes > kGatherLdsMax)) { continue; }   // no key tables with cached positions
-		for (int ept : { 2, 4, 8, 16 }) {
+		for (int ept : { 2, 4, 8, 16, kSortEpt }) {
 			if (plan.cached && ept == 2) { continue; }
-			if (plan.sorted && ept != 16) { continue; }   // the windows are the pieces of 16 elements per lane
+			if (plan.sorted != (ept == kSortEpt)) { continue; }   // sorted plans: the windows are the pieces
 			for (int order = 0; order < 2; ++order) {
 				batch_gather_descs(plan, ept);
 				const double t = time_ms([&] {
@@ -2014,7 +2016,7 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 			launch_positions(d);
 			if (c.batch_sorted && g->num > 0) {
 				g->perm.alloc((size_t)g->num);
-				k_sort_windows<<<(unsigned)ceil_div(g->num, kSortWin), 256, 0, c.stream>>>(g->pos.p, g->perm.p, g->num);
+				k_sort_windows<<<(unsigned)ceil_div(g->num, kSortWin), kSortThreads, 0, c.stream>>>(g->pos.p, g->perm.p, g->num);
 				GLI_HIP(hipGetLastError());
 			}
 		}
@@ -2035,7 +2037,7 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 				if (!gemm_shape_fits(ch.gemm_shape[m], a.Cp, a.tri_cols || a.tri_rows || a.tet_C1 > 0 || a.sym_x, a.p_base)) { ch.gemm_shape[m] = 0; }
 			}
 		}
-		ch.gather_ept = plan->sorted ? 16 : c.gather_ept;
+		ch.gather_ept = plan->sorted ? kSortEpt : c.gather_ept;
 		ch.gather_order = c.gather_xcd;
 		ch.gather_lds = c.gather_lds;
 		plan->report = "{\"autotuned\": false}";

@@ -55,7 +55,7 @@ int gli_synchronize(void);
  *                 at creation, and each replay reads those 4-byte positions instead of decoding the
  *                 8-byte four-indices; 0: every replay decodes
  *   "batch_sorted" 1 (default): with batch_positions, the plan also sorts the positions inside windows of
- *                 4096 consecutive elements (keeping each one's place), so every replay reads the key tensor
+ *                 8192 consecutive elements (keeping each one's place), so every replay reads the key tensor
  *                 in ascending order within a window and stores the values back in list order through LDS;
  *                 0: positions in list order
  *   "gather_order" 0: workgroups take consecutive pieces of the list; 1 (default): each XCD takes one

@@ -531,7 +531,7 @@ def test_batch_plan_matches_single_grids(L, method, branches, positions, sorted_
     scales, new coefficients and a symmetry change behave as documented.  branches = 2 forces the
     two-branch graph (largest grid beside the others on a second stream) wherever it applies;
     positions = 1 caches every element's key-tensor position at plan creation, 0 decodes per replay;
-    sorted_windows = 1 also sorts the cached positions inside windows of 4096 elements (the 3^3 list and
+    sorted_windows = 1 also sorts the cached positions inside windows of 8192 elements (the 3^3 list and
     the last window of every list are partial)."""
     say("batch plan method", method, "branches", branches, "positions", positions, "sorted", sorted_windows)
     saved = L.gli_get_tuning(b"batch_branches"), L.gli_get_tuning(b"batch_positions"), L.gli_get_tuning(b"batch_sorted")
