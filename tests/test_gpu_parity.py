@@ -58,7 +58,9 @@ def test_mode_product_vs_numpy(L, Kd, P, Cc):
 # enumeration
 
 def test_eri_enumeration_bit_exact_small(L):
-    say("enumerate 3, 5")
+    say("enumerate 1, 3, 5")
+    d, n = eri_enumerate_device(1)   # one point: the single four-index 0 (Burnside count 1)
+    assert n == burnside.orbit_count(1) == 1 and d.to_numpy(np.int64, n).tolist() == [0]
     g = gli.SparseERI  # noqa: F841 (surface import check)
     ref3 = golden("ref_eri_3x3x3.npz")["four_indices"]
     d, n = eri_enumerate_device(3)
