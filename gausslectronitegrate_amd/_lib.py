@@ -33,6 +33,9 @@ EXTRA = {
     "gli_erida_integrals_points": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64), C.c_int64,
                                              C.c_double, C.POINTER(C.c_double)]),
     "gli_direct_flops": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gli_kinetic_matrix": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "gli_nuclear_matrix": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int), C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "gli_device_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
     "gli_device_free": (None, [C.c_void_p]),
     "gli_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),

@@ -111,6 +111,18 @@ __global__ void k_nuclear_scatter(const double* __restrict__ V, int nx, int ny, 
 	out[e] = V[(qx * ((int64_t)ny * ny) + qy) * ((int64_t)nz * nz) + qz];
 }
 
+// work buffers of the one-body assemblies, grown on demand and kept for later calls (the context
+// lock is held while they are used)
+struct OneBodyWorkspace {
+	DevBuf<double> dc, f, W, F0, F1, F2, Y1, Y2, V, M, t;
+};
+
+static OneBodyWorkspace& onebody_ws()
+{
+	static OneBodyWorkspace* w = new OneBodyWorkspace();
+	return *w;
+}
+
 static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], const double* pos, const int* charges,
                            int num_nuclei, double tol, double* out)
 {
@@ -136,11 +148,28 @@ static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 		for (int a = 0; a < 3; ++a) { nuc.pos[3 * k + a] = pos[3 * k + a]; }
 		nuc.charge[k] = (double)charges[k];
 	}
-	DevBuf<double> dc(K), f((size_t)R * L), W((size_t)U[0] * U[1] * U[2]);
-	DevBuf<double> F0((size_t)U[0] * C[0]), F1((size_t)U[1] * C[1]), F2((size_t)U[2] * C[2]);
-	DevBuf<double> Y1((size_t)U[1] * U[2] * C[0]), Y2((size_t)U[2] * C[0] * C[1]), V((size_t)C[0] * C[1] * C[2]);
+	OneBodyWorkspace& w = onebody_ws();
+	DevBuf<double>& dc = w.dc;
+	DevBuf<double>& f = w.f;
+	DevBuf<double>& W = w.W;
+	DevBuf<double>& F0 = w.F0;
+	DevBuf<double>& F1 = w.F1;
+	DevBuf<double>& F2 = w.F2;
+	DevBuf<double>& Y1 = w.Y1;
+	DevBuf<double>& Y2 = w.Y2;
+	DevBuf<double>& V = w.V;
+	DevBuf<double>& M = w.M;
 	const int64_t N = (int64_t)n[0] * n[1] * n[2];
-	DevBuf<double> M(N * N);
+	dc.reserve(K);
+	f.reserve((size_t)R * L);
+	W.reserve((size_t)U[0] * U[1] * U[2]);
+	F0.reserve((size_t)U[0] * C[0]);
+	F1.reserve((size_t)U[1] * C[1]);
+	F2.reserve((size_t)U[2] * C[2]);
+	Y1.reserve((size_t)U[1] * U[2] * C[0]);
+	Y2.reserve((size_t)U[2] * C[0] * C[1]);
+	V.reserve((size_t)C[0] * C[1] * C[2]);
+	M.reserve((size_t)(N * N));
 	GLI_HIP(hipMemcpyAsync(dc.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, st));
 	k_factors_tol<<<dim3((unsigned)ceil_div(L, 128), R), 128, 0, st>>>(dc.p, (int)K, tol, f.p);
 	k_nuclear_potential<<<dim3((unsigned)ceil_div(U[2], 64), U[1], U[0]), 64, 0, st>>>(nuc, u0[0], u0[1], u0[2], U[0], U[1], U[2], W.p);
@@ -202,7 +231,13 @@ static void kinetic_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 	hipStream_t st = c.stream;
 	const int R = (int)std::max(g[1], std::max(g[3], g[5]));
 	const int64_t N = g[1] * g[3] * g[5];
-	DevBuf<double> dc(K), t(R), M(N * N);
+	OneBodyWorkspace& w = onebody_ws();
+	DevBuf<double>& dc = w.dc;
+	DevBuf<double>& t = w.t;
+	DevBuf<double>& M = w.M;
+	dc.reserve(K);
+	t.reserve(R);
+	M.reserve((size_t)(N * N));
 	GLI_HIP(hipMemcpyAsync(dc.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, st));
 	k_kinetic_1d<<<R, 256, 0, st>>>(dc.p, (int)K, t.p);
 	k_kinetic_fill<<<(unsigned)ceil_div(N * N, 256), 256, 0, st>>>(t.p, (int)g[1], (int)g[3], (int)g[5], M.p);
@@ -338,6 +373,32 @@ int gli_hip_kinetic_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 {
 	try {
 		kinetic_matrix(coeffs, K, g, out);
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_kinetic_matrix(const double* coeffs, int64_t K, const int64_t grid[6], double* out)
+{
+	try {
+		GLI_REQUIRE(coeffs != nullptr && grid != nullptr && out != nullptr, "null argument");
+		kinetic_matrix(coeffs, K, grid, out);
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_nuclear_matrix(const double* coeffs, int64_t K, const int64_t grid[6], const double* positions, const int* charges,
+                       int num_nuclei, double tol, double* out)
+{
+	try {
+		GLI_REQUIRE(coeffs != nullptr && grid != nullptr && out != nullptr, "null argument");
+		GLI_REQUIRE(num_nuclei == 0 || (positions != nullptr && charges != nullptr), "null nuclei arrays");
+		nuclear_matrix(coeffs, K, grid, positions, charges, num_nuclei, tol, out);
 		return 0;
 	}
 	catch (const Failure&) {

@@ -81,6 +81,11 @@ def _nuclei(positions, charges, syntax):
     return arr, len(chg)
 
 
+def _grid6(g):
+    """the grid as {istart_x, num_x, istart_y, num_y, istart_z, num_z}"""
+    return np.array([v for a in range(3) for v in (g.coord_range[a].istart, g.coord_range[a].num)], dtype=np.int64)
+
+
 def _symmetric(g, syntax):
     spec = g.spec()
     extent = max(max(abs(s), abs(s + n - 1)) for s, n in spec)
@@ -136,11 +141,11 @@ def compute_kinetic_gausslet_integrals(gausslet_coeffs, grid):
     gd, keep = _coeffs(gausslet_coeffs, syntax)
     g = _grid(grid, syntax)
     L = require_gpu()
-    out = _abi.KineticIntegrals()
-    L.compute_kinetic_gausslet_integrals(C.byref(gd), C.byref(g), C.byref(out))
     N = grid_num_points(grid)
-    m = np.ctypeslib.as_array(out.integral_values, shape=(N, N)).copy()
-    L.delete_kinetic_gausslet_integrals(C.byref(out))
+    m = np.empty((N, N))
+    g6 = _grid6(g)
+    _lib.check(L.gli_kinetic_matrix(gd.coefficients, gd.indices.num, g6.ctypes.data_as(_abi.P_glong),
+                                    m.ctypes.data_as(_abi.P_double)), "compute_kinetic_gausslet_integrals")
     return m
 
 
@@ -160,11 +165,14 @@ def compute_nuclear_gausslet_integrals(gausslet_coeffs, grid, nuclear_positions,
     nuc, nn = _nuclei(nuclear_positions, nuclear_charges, syntax)
     tol = _tol(tol, syntax)
     L = require_gpu()
-    out = _abi.NuclearIntegrals()
-    L.compute_nuclear_gausslet_integrals(C.byref(gd), C.byref(g), nuc, nn, tol, C.byref(out))
     N = grid_num_points(grid)
-    m = np.ctypeslib.as_array(out.integral_values, shape=(N, N)).copy()
-    L.delete_nuclear_gausslet_integrals(C.byref(out))
+    m = np.empty((N, N))
+    g6 = _grid6(g)
+    pos = np.ascontiguousarray([[nuc[k].pos[a] for a in range(3)] for k in range(nn)], dtype=np.float64).reshape(-1)
+    chg = np.ascontiguousarray([nuc[k].charge for k in range(nn)], dtype=np.int32)
+    _lib.check(L.gli_nuclear_matrix(gd.coefficients, gd.indices.num, g6.ctypes.data_as(_abi.P_glong),
+                                    pos.ctypes.data_as(_abi.P_double), chg.ctypes.data_as(C.POINTER(C.c_int)), nn, tol,
+                                    m.ctypes.data_as(_abi.P_double)), "compute_nuclear_gausslet_integrals")
     return m
 
 

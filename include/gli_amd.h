@@ -108,6 +108,14 @@ int gli_eri_integrals_points(const double* coeffs, int64_t K, const int64_t* poi
 int gli_erida_integrals_points(const double* coeffs, int64_t K, const int64_t* points, int64_t num, double tol,
                                double* values);
 
+/* The one-body matrices of compute_kinetic_gausslet_integrals (kinetic_integrals.c:44-121) and
+ * compute_nuclear_gausslet_integrals (nuclear_integrals.c:133-266) written into a caller's host array
+ * out[N * N] (row-major, N = the grid's point count) instead of a result struct; for callers that own
+ * their output buffer.  positions[3 * n + a] is axis a of nucleus n, charges[n] its charge; synchronous. */
+int gli_kinetic_matrix(const double* coeffs, int64_t K, const int64_t grid[6], double* out);
+int gli_nuclear_matrix(const double* coeffs, int64_t K, const int64_t grid[6], const double* positions, const int* charges,
+                       int num_nuclei, double tol, double* out);
+
 /* several grids in one call (for example 5^3, 7^3, 9^3 and 11^3 with different K): a batch plan
  * without autotuning, run once; synchronous */
 int gli_eri_values_batch_device(int num_grids, const double* const* coeffs, const int64_t* K, const int64_t* grids,

@@ -359,6 +359,17 @@ def test_kinetic(L):
     assert np.array_equal(kg, kg.T)
     g = golden("golden_n3_K57s0.npz")
     assert np.max(np.abs(gli.compute_kinetic_gausslet_integrals(g["coeffs"], sym(3)) - g["kinetic"])) <= 1e-13
+    # the reference's struct-returning entry point gives the same matrix bit for bit
+    gd = _abi.GaussletData()
+    cc = np.ascontiguousarray(g10)
+    gd.coefficients = cc.ctypes.data_as(_abi.P_double)
+    gd.indices.istart = -(len(g10) - 1) // 2
+    gd.indices.num = len(g10)
+    grid = _abi.Grid.of(((-1, 3), (-2, 5), (-2, 4)))
+    out = _abi.KineticIntegrals()
+    L.compute_kinetic_gausslet_integrals(C.byref(gd), C.byref(grid), C.byref(out))
+    assert np.array_equal(np.ctypeslib.as_array(out.integral_values, shape=kg.shape), kg)
+    L.delete_kinetic_gausslet_integrals(C.byref(out))
 
 
 def test_nuclear(L):
@@ -376,6 +387,22 @@ def test_nuclear(L):
     g = golden("golden_n3_K57s0.npz")
     m3 = gli.compute_nuclear_gausslet_integrals(g["coeffs"], sym(3), [[0, 0, 0]], [1], float(g["tol"]))
     assert np.max(np.abs(m3 - g["nuclear"])) <= 1e-13
+    # the reference's struct-returning entry point gives the same matrix bit for bit
+    gd = _abi.GaussletData()
+    cc = np.ascontiguousarray(g10)
+    gd.coefficients = cc.ctypes.data_as(_abi.P_double)
+    gd.indices.istart = -(len(g10) - 1) // 2
+    gd.indices.num = len(g10)
+    grid = _abi.Grid.of(((-1, 3), (0, 2), (-2, 4)))
+    nuc = (_abi.Nucleus * 2)()
+    for q in range(2):
+        for a in range(3):
+            nuc[q].pos[a] = pos[q][a]
+        nuc[q].charge = [2, 1][q]
+    out = _abi.NuclearIntegrals()
+    L.compute_nuclear_gausslet_integrals(C.byref(gd), C.byref(grid), nuc, 2, 1e-16, C.byref(out))
+    assert np.array_equal(np.ctypeslib.as_array(out.integral_values, shape=ngi.shape), ngi)
+    L.delete_nuclear_gausslet_integrals(C.byref(out))
 
 
 def test_nuclear_golden_7(L):
