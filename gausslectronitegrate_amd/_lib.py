@@ -33,6 +33,8 @@ EXTRA = {
     "gli_erida_integrals_points": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64), C.c_int64,
                                              C.c_double, C.POINTER(C.c_double)]),
     "gli_direct_flops": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gli_sparse_eri_compute": (C.c_int, [C.POINTER(_abi.GaussletData), C.POINTER(_abi.Grid), C.c_double,
+                                         C.POINTER(_abi.SparseEri)]),
     "gli_kinetic_matrix": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "gli_nuclear_matrix": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int), C.c_int, C.c_double, C.POINTER(C.c_double)]),

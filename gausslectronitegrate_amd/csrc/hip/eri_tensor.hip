@@ -19,6 +19,7 @@
 // and each element gathers V at its sorted key triple.  This is an exact
 // re-association of the reference sum: same terms, different order.
 #include "common.hpp"
+#include "eri_integrals.h"
 #include "../gli_device.h"
 #include "gli_amd.h"
 
@@ -2176,6 +2177,56 @@ int gli_hip_eri_values_copy(const double* coeffs, int64_t K, const int64_t g[6],
 		return 0;
 	}
 	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_sparse_eri_compute(const struct gausslet_data* gdata, const struct cartesian_grid_3d* grid, double tol,
+                           struct sparse_eri_gausslet_integrals* eri)
+{
+	int64_t* d_idx = nullptr;
+	double* d_val = nullptr;
+	int64_t* h_idx = nullptr;
+	double* h_val = nullptr;
+	try {
+		GLI_REQUIRE(gdata != nullptr && grid != nullptr && eri != nullptr, "null argument");
+		const int64_t K = gdata->indices.num;
+		GLI_REQUIRE(gdata->coefficients != nullptr && K >= 3 && K % 2 == 1 && gdata->indices.istart == -(K - 1) / 2,
+		            "the gausslet coefficients need an odd count K >= 3 with indices -(K-1)/2 .. (K-1)/2");
+		const int64_t n = grid->coord_range[0].num;
+		for (int a = 0; a < 3; ++a) {
+			GLI_REQUIRE(grid->coord_range[a].num == n && n % 2 == 1 && grid->coord_range[a].istart == -(n - 1) / 2,
+			            "the enumeration needs a cubic grid centred at the origin with an odd number of points per axis");
+		}
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		int64_t num = 0;
+		if (gli_eri_enumerate_device(n, &d_idx, &num) != 0) { throw Failure{"enumeration"}; }
+		int64_t g6[6];
+		for (int a = 0; a < 3; ++a) {
+			g6[2 * a] = grid->coord_range[a].istart;
+			g6[2 * a + 1] = grid->coord_range[a].num;
+		}
+		GLI_HIP(hipMalloc((void**)&d_val, (size_t)std::max<int64_t>(num, 1) * sizeof(double)));
+		eri_values_dev(gdata->coefficients, K, g6, d_idx, num, tol, gli_get_eri_method(), d_val);
+		h_idx = (int64_t*)gli_host_alloc((size_t)num * sizeof(int64_t));
+		h_val = (double*)gli_host_alloc((size_t)num * sizeof(double));
+		GLI_REQUIRE(h_idx != nullptr && h_val != nullptr, "host allocation of the result arrays failed");
+		copy_to_host(h_idx, d_idx, (size_t)num * sizeof(int64_t));
+		copy_to_host(h_val, d_val, (size_t)num * sizeof(double));
+		GLI_HIP(hipFree(d_idx));
+		GLI_HIP(hipFree(d_val));
+		eri->grid = *grid;
+		eri->num_entries = num;
+		eri->four_indices = h_idx;
+		eri->integral_values = h_val;
+		return 0;
+	}
+	catch (const Failure&) {
+		if (d_idx != nullptr) { (void)hipFree(d_idx); }
+		if (d_val != nullptr) { (void)hipFree(d_val); }
+		free(h_idx);
+		free(h_val);
 		return -1;
 	}
 }

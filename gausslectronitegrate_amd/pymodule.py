@@ -253,12 +253,12 @@ class SparseERI:
         _symmetric(g, syntax)
         L = require_gpu()
         self._perm = _Perms(g)
-        idx = _abi.SparseEriIndices()
-        L.enumerate_symmetry_reduced_eri_indices(C.byref(g), C.byref(idx))
-        self._eri = _abi.SparseEri()
-        L.compute_sparse_eri_gausslet_integrals(C.byref(self._gd), C.byref(g), idx.four_indices, idx.num,
-                                                self._tol, C.byref(self._eri))
-        L.delete_sparse_eri_indices(C.byref(idx))
+        # enumerate_symmetry_reduced_eri_indices + compute_sparse_eri_gausslet_integrals in one call, the
+        # index list staying on the GPU in between
+        eri = _abi.SparseEri()
+        _lib.check(L.gli_sparse_eri_compute(C.byref(self._gd), C.byref(g), self._tol, C.byref(eri)),
+                   "compute_sparse_eri_gausslet_integrals")
+        self._eri = eri
 
     @classmethod
     def _from_arrays(cls, eri_struct, coeffs, tol):

@@ -108,6 +108,16 @@ int gli_eri_integrals_points(const double* coeffs, int64_t K, const int64_t* poi
 int gli_erida_integrals_points(const double* coeffs, int64_t K, const int64_t* points, int64_t num, double tol,
                                double* values);
 
+/* enumerate_symmetry_reduced_eri_indices (eri_integrals.c:382-526) followed by
+ * compute_sparse_eri_gausslet_integrals (eri_integrals.c:544-678) on the same symmetric grid, with the
+ * index list kept on the GPU in between: one download of indices and values instead of a download, an
+ * upload and a host copy.  Fills *eri exactly like the two calls (release with
+ * delete_sparse_eri_gausslet_integrals); the method is gli_get_eri_method().  Returns 0, or -1 with
+ * gli_last_error() set (for a grid that is not cubic, centred and odd, or coefficients that are not). */
+struct sparse_eri_gausslet_integrals;
+int gli_sparse_eri_compute(const struct gausslet_data* gdata, const struct cartesian_grid_3d* grid, double tol,
+                           struct sparse_eri_gausslet_integrals* eri);
+
 /* The one-body matrices of compute_kinetic_gausslet_integrals (kinetic_integrals.c:44-121) and
  * compute_nuclear_gausslet_integrals (nuclear_integrals.c:133-266) written into a caller's host array
  * out[N * N] (row-major, N = the grid's point count) instead of a result struct; for callers that own

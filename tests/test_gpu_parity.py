@@ -130,6 +130,19 @@ def test_host_api_streamed_matches_device(L, method):
     assert np.array_equal(keys, host)
     ref = eri_values_device(c, sym(n), host, 1e-16, method)
     assert np.array_equal(vals, ref)
+    # the fused entry (index list kept on the GPU) fills the same struct
+    L.gli_set_eri_method(method)
+    try:
+        fused = _abi.SparseEri()
+        _lib.check(L.gli_sparse_eri_compute(C.byref(gd), C.byref(g), 1e-16, C.byref(fused)), "gli_sparse_eri_compute")
+        assert fused.num_entries == num and fused.grid.spec() == g.spec()
+        assert np.array_equal(np.ctypeslib.as_array(fused.four_indices, shape=(num,)), host)
+        assert np.array_equal(np.ctypeslib.as_array(fused.integral_values, shape=(num,)), ref)
+        L.delete_sparse_eri_gausslet_integrals(C.byref(fused))
+    finally:
+        L.gli_set_eri_method(0)
+    bad = _abi.Grid.of(((-4, 9), (-4, 9), (-3, 9)))   # not centred: an error status, no abort
+    assert L.gli_sparse_eri_compute(C.byref(gd), C.byref(bad), 1e-16, C.byref(_abi.SparseEri())) == -1
 
 
 def test_eri_enumeration_13_count(L):
