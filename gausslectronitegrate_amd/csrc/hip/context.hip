@@ -280,6 +280,10 @@ int gli_set_tuning(const char* knob, int value)
 			GLI_REQUIRE(value == 0 || value == 1, "mode1_half must be 0 or 1");
 			c.mode1_half = value;
 		}
+		else if (k == "gather_threads") {
+			GLI_REQUIRE(value == 256 || value == 512 || value == 1024, "gather_threads must be 256, 512 or 1024");
+			c.gather_threads = value;
+		}
 		else if (k == "gather_lds") {
 			GLI_REQUIRE(value == 0 || value == 1, "gather_lds must be 0 or 1");
 			c.gather_lds = value;
@@ -311,6 +315,7 @@ int gli_get_tuning(const char* knob)
 		if (k == "gather_ept") { return c.gather_ept; }
 		if (k == "gather_order") { return c.gather_xcd; }
 		if (k == "gather_lds") { return c.gather_lds; }
+		if (k == "gather_threads") { return c.gather_threads; }
 		if (k == "mode1_half") { return c.mode1_half; }
 		if (k == "batch_branches") { return c.batch_branches; }
 		if (k == "batch_positions") { return c.batch_positions; }

@@ -733,7 +733,7 @@ __device__ inline void gather_block(const GatherDesc& d, int64_t wave_base)
 }
 
 template <int GATHER_EPT, bool kLds>
-__global__ __launch_bounds__(256) void k_gather(GatherDesc d, int64_t xcd_chunks)
+__global__ __launch_bounds__(1024) void k_gather(GatherDesc d, int64_t xcd_chunks)
 {
 	const int64_t piece = gather_piece(xcd_chunks);
 	const int64_t wave_base = (piece * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * GATHER_EPT);
@@ -1250,7 +1250,8 @@ static void gather_dispatch(int ept, bool lds, F&& f)
 
 static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 {
-	const int64_t pieces = gather_pieces(d.num, ept);
+	const int threads = ctx().gather_threads;
+	const int64_t pieces = ceil_div(d.num, (int64_t)threads * ept);
 	if (pieces == 0) { return; }
 	const int64_t chunks = xcd ? ceil_div(pieces, 8) : 0;
 	const int64_t blocks = xcd ? 8 * chunks : pieces;
@@ -1259,7 +1260,7 @@ static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 	const size_t shm = use_lds ? (size_t)gather_lds_bytes(d.nlut, d.C1) : 0;
 	hipStream_t st = ctx().stream;
 	gather_dispatch(ept, use_lds, [&](auto E, auto Lds) {
-		k_gather<decltype(E)::value, decltype(Lds)::value><<<(unsigned)blocks, 256, shm, st>>>(d, chunks);
+		k_gather<decltype(E)::value, decltype(Lds)::value><<<(unsigned)blocks, threads, shm, st>>>(d, chunks);
 	});
 	GLI_HIP(hipGetLastError());
 }

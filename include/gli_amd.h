@@ -46,6 +46,8 @@ int gli_synchronize(void);
  *                 that does not divide the key padding of a triangular product falls back to 0)
  *   "gather_ept"  elements per lane in the per-element gather: 2, 4, 8 (default) or 16
  *   "gather_lds"  1 (default): each gather workgroup copies the key tables into LDS first; 0: global reads
+ *   "gather_threads" workgroup size of the single-grid gather: 256, 512 or 1024 (default; its 16 waves take
+ *                 8192 consecutive elements on one CU, whose L1 then serves more of their key-tensor reads)
  *   "mode1_half"  1 (default): the FP64 paths compute the first mode product's (m2, m3)-symmetric output
  *                 once (m3 >= m2) and the second reads it mirrored; 0: full output
  *   "batch_branches" batch plans of two or more grids with FP64 products: 1 (default) the autotuner also

@@ -493,7 +493,7 @@ def test_eri_mixed_full_7(L):
 # tuning knobs: every variant computes the same terms in the same order per element
 
 def test_tuning_variants_bit_identical(L):
-    """tile order of the mode products, gather order and elements per lane change the schedule only:
+    """tile order of the mode products, gather order, elements per lane and workgroup size change the schedule only:
     the full 7^3 list (and an unsorted, ragged copy of it) must come out bit-identical"""
     say("tuning variants")
     c = golden("coeffs_K57_s0.npy")
@@ -502,7 +502,7 @@ def test_tuning_variants_bit_identical(L):
     rng = np.random.default_rng(5)
     shuffled = idx[rng.permutation(n)][: n - 77]
     knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1)), (b"gather_ept", (2, 4, 8, 16)), (b"gather_lds", (0, 1)),
-             (b"gemm_shape", (0, 3, 4, 5)), (b"mode1_half", (0, 1))]
+             (b"gemm_shape", (0, 3, 4, 5)), (b"mode1_half", (0, 1)), (b"gather_threads", (256, 512, 1024))]
     saved = {k: L.gli_get_tuning(k) for k, _ in knobs}
     try:
         ref, ref_s = {}, {}
@@ -510,8 +510,9 @@ def test_tuning_variants_bit_identical(L):
             if (to + go + ept + lds + shape) % 2 and ept not in (4, 16):
                 continue   # a spread of the combinations keeps the test short; every value of every knob is used
             half = (to + lds + shape + ept // 4) % 2   # the half-stored first mode: both values with every shape
-            for k, v in zip((b"tile_order", b"gather_order", b"gather_ept", b"gather_lds", b"gemm_shape", b"mode1_half"),
-                            (to, go, ept, lds, shape, half)):
+            threads = (256, 512, 1024)[(to + go + shape + ept) % 3]
+            for k, v in zip((b"tile_order", b"gather_order", b"gather_ept", b"gather_lds", b"gemm_shape", b"mode1_half",
+                             b"gather_threads"), (to, go, ept, lds, shape, half, threads)):
                 assert L.gli_set_tuning(k, v) == 0
             for method in (0, 1):
                 got = eri_values_device(c, sym(7), idx, 1e-16, method)
@@ -525,6 +526,7 @@ def test_tuning_variants_bit_identical(L):
         assert L.gli_set_tuning(b"gather_lds", 2) == -1
         assert L.gli_set_tuning(b"mode1_half", 2) == -1
         assert L.gli_set_tuning(b"batch_sorted", 2) == -1
+        assert L.gli_set_tuning(b"gather_threads", 128) == -1
         assert L.gli_set_tuning(b"batch_positions", -1) == -1
         assert L.gli_get_tuning(b"batch_sorted") in (0, 1)
         assert L.gli_set_tuning(b"no_such_knob", 1) == -1
