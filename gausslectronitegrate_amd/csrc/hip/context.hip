@@ -293,7 +293,7 @@ int gli_set_tuning(const char* knob, int value)
 			c.gemm_shape = value;
 		}
 		else if (k == "gather_order") {
-			GLI_REQUIRE(value == 0 || value == 1, "gather_order must be 0 or 1");
+			GLI_REQUIRE(value >= 0 && value <= 2, "gather_order must be 0, 1 or 2");
 			c.gather_xcd = value;
 		}
 		else {

@@ -1248,6 +1248,18 @@ static void gather_dispatch(int ept, bool lds, F&& f)
 	}
 }
 
+// Gather order for a key tensor of nV entries.  One chunk of the list per XCD keeps each XCD's
+// key-tensor lines in its own L2; once the tensor is about as large as the shared last-level cache
+// (256 MB), consecutive pieces win instead: all XCDs then read neighbouring parts of the list and
+// share the cached lines (13^3: 7.93 against 8.22 ms; 11^3: 1.444 against 1.451 ms, for twice the
+// L2-to-fabric bytes).
+static int resolve_gather_order(int knob, int64_t nV)
+{
+	constexpr int64_t kSharedCacheEntries = ((int64_t)128 << 20) / (int64_t)sizeof(double);
+	if (knob != 2) { return knob; }
+	return nV > kSharedCacheEntries ? 0 : 1;
+}
+
 static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 {
 	const int threads = ctx().gather_threads;
@@ -1560,7 +1572,8 @@ static void eri_apply(const EriPrepared& ep, const int64_t* d_idx, int64_t num, 
 		if (mark) { phase_mark("direct"); }
 		return;
 	}
-	launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept, c.gather_xcd, c.gather_lds);
+	launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept,
+	              resolve_gather_order(c.gather_xcd, tet_index(0, 0, s.C1)), c.gather_lds);
 	if (mark) { phase_mark("gather"); }
 }
 
@@ -2040,7 +2053,9 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 			}
 		}
 		ch.gather_ept = plan->sorted ? kSortEpt : c.gather_ept;
-		ch.gather_order = c.gather_xcd;
+		int64_t nV_max = 0;
+		for (auto& g : plan->grids) { nV_max = std::max(nV_max, tet_index(0, 0, g->s.C1)); }
+		ch.gather_order = resolve_gather_order(c.gather_xcd, nV_max);
 		ch.gather_lds = c.gather_lds;
 		plan->report = "{\"autotuned\": false}";
 	}

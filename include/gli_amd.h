@@ -60,8 +60,9 @@ int gli_synchronize(void);
  *                 8192 consecutive elements (keeping each one's place), so every replay reads the key tensor
  *                 in ascending order within a window and stores the values back in list order through LDS;
  *                 0: positions in list order
- *   "gather_order" 0: workgroups take consecutive pieces of the list; 1 (default): each XCD takes one
- *                 contiguous chunk of the list (its L2 then holds the key-tensor entries it reuses)
+ *   "gather_order" 0: workgroups take consecutive pieces of the list; 1: each XCD takes one contiguous
+ *                 chunk of the list (its L2 then holds the key-tensor entries it reuses); 2 (default): 1
+ *                 while the key tensor is at most 128 MB (11^3: 71 MB), else 0 (13^3: 299 MB)
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
 int gli_set_tuning(const char* knob, int value);
 int gli_get_tuning(const char* knob);

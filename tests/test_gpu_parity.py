@@ -501,12 +501,12 @@ def test_tuning_variants_bit_identical(L):
     idx = d.to_numpy(np.int64, n)
     rng = np.random.default_rng(5)
     shuffled = idx[rng.permutation(n)][: n - 77]
-    knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1)), (b"gather_ept", (2, 4, 8, 16)), (b"gather_lds", (0, 1)),
+    knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1, 2)), (b"gather_ept", (2, 4, 8, 16)), (b"gather_lds", (0, 1)),
              (b"gemm_shape", (0, 3, 4, 5)), (b"mode1_half", (0, 1)), (b"gather_threads", (256, 512, 1024))]
     saved = {k: L.gli_get_tuning(k) for k, _ in knobs}
     try:
         ref, ref_s = {}, {}
-        for to, go, ept, lds, shape in itertools.product((0, 1), (0, 1), (2, 4, 8, 16), (0, 1), (0, 3, 4, 5)):
+        for to, go, ept, lds, shape in itertools.product((0, 1), (0, 1, 2), (2, 4, 8, 16), (0, 1), (0, 3, 4, 5)):
             if (to + go + ept + lds + shape) % 2 and ept not in (4, 16):
                 continue   # a spread of the combinations keeps the test short; every value of every knob is used
             half = (to + lds + shape + ept // 4) % 2   # the half-stored first mode: both values with every shape
@@ -527,6 +527,7 @@ def test_tuning_variants_bit_identical(L):
         assert L.gli_set_tuning(b"mode1_half", 2) == -1
         assert L.gli_set_tuning(b"batch_sorted", 2) == -1
         assert L.gli_set_tuning(b"gather_threads", 128) == -1
+        assert L.gli_set_tuning(b"gather_order", 3) == -1
         assert L.gli_set_tuning(b"batch_positions", -1) == -1
         assert L.gli_get_tuning(b"batch_sorted") in (0, 1)
         assert L.gli_set_tuning(b"no_such_knob", 1) == -1
