@@ -23,6 +23,9 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
+# OpenMP threads of the host-side baseline stay close to each other (libgomp reads this when the
+# first library linked with it is loaded, which happens below, after this line)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 from gausslectronitegrate_amd import _abi, _lib  # noqa: E402
 from gausslectronitegrate_amd.coeffs import synthetic_coefficients  # noqa: E402
@@ -96,46 +99,54 @@ def phase_times(L):
     return {names[i].decode(): ms[i] for i in range(max(n, 0))}
 
 
-def cpu_baseline(c, n, all_idx, tol, target_s, seed, label):
-    """Reference C build (oracle/_ref) if present, else the oracle restatement, on all host threads."""
-    from oracle import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-    kind = "port"
-    run = None
-    if os.path.exists(orc.REF_LIB_PATH):
-        ref = _abi.bind(C.CDLL(orc.REF_LIB_PATH))
-        kind = "reference"
+def host_cpu():
+    """(model name, set of ISA flags) of the host CPU from /proc/cpuinfo"""
+    model, flags = None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if model is None and line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                elif not flags and line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                if model is not None and flags:
+                    break
+    except OSError:
+        pass
+    return model, flags
 
-        def run(idx):
-            gd = _abi.GaussletData()
-            cc = np.ascontiguousarray(c)
-            gd.coefficients = cc.ctypes.data_as(_abi.P_double)
-            gd.indices.istart = -(len(cc) - 1) // 2
-            gd.indices.num = len(cc)
-            h = (n - 1) // 2
-            grid = _abi.Grid.of(((-h, n), (-h, n), (-h, n)))
-            ii = np.ascontiguousarray(idx)
-            out = _abi.SparseEri()
-            ref.compute_sparse_eri_gausslet_integrals(C.byref(gd), C.byref(grid), ii.ctypes.data_as(_abi.P_glong),
-                                                      len(ii), tol, C.byref(out))
-            v = np.ctypeslib.as_array(out.integral_values, shape=(len(ii),)).copy()
-            ref.delete_sparse_eri_gausslet_integrals(C.byref(out))
-            return v
-    else:
-        h = (n - 1) // 2
 
-        def run(idx):
-            return orc.eri_sparse(c, ((-h, n), (-h, n), (-h, n)), idx, tol)
-    # calibrate on a small seeded sample, then time a seeded sample sized to about target_s
-    rng = np.random.default_rng(2024)
-    probe_pos = np.sort(rng.choice(len(all_idx), size=min(len(all_idx), max(threads * 64, 1024)), replace=False))
-    t0 = time.perf_counter()
-    run(all_idx[probe_pos])
-    per = (time.perf_counter() - t0) / len(probe_pos)
-    count = int(min(len(all_idx), max(len(probe_pos), target_s / max(per, 1e-9))))
-    count = max(count, 8)
-    pos = np.sort(rng.choice(len(all_idx), size=count, replace=False))
-    # 8 sub-batches (a seeded split of the sample) for a confidence interval of the rate
+# x86-64 micro-architecture levels of the prebuilt reference variants (oracle/Makefile) and the CPU
+# flags each one needs
+ISA_LEVELS = [("x86-64-v4", "_v4", {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl", "avx2", "fma", "bmi2"}),
+              ("x86-64-v3", "_v3", {"avx2", "fma", "bmi1", "bmi2", "movbe", "f16c", "abm"})]
+
+
+def ref_runner(path, c, n, tol):
+    """values of an index sample through a build of the reference library at `path`"""
+    ref = _abi.bind(C.CDLL(path))
+    cc = np.ascontiguousarray(c)
+    h = (n - 1) // 2
+
+    def run(idx):
+        gd = _abi.GaussletData()
+        gd.coefficients = cc.ctypes.data_as(_abi.P_double)
+        gd.indices.istart = -(len(cc) - 1) // 2
+        gd.indices.num = len(cc)
+        grid = _abi.Grid.of(((-h, n), (-h, n), (-h, n)))
+        ii = np.ascontiguousarray(idx)
+        out = _abi.SparseEri()
+        ref.compute_sparse_eri_gausslet_integrals(C.byref(gd), C.byref(grid), ii.ctypes.data_as(_abi.P_glong),
+                                                  len(ii), tol, C.byref(out))
+        v = np.ctypeslib.as_array(out.integral_values, shape=(len(ii),)).copy()
+        ref.delete_sparse_eri_gausslet_integrals(C.byref(out))
+        return v
+    return run
+
+
+def timed_sample(run, all_idx, pos, rng):
+    """run the sample all_idx[pos] as 8 seeded sub-batches: (values, seconds, per-batch rates)"""
+    count = len(pos)
     part = rng.permutation(count) % 8
     vals = np.empty(count)
     dt = 0.0
@@ -148,31 +159,72 @@ def cpu_baseline(c, n, all_idx, tol, target_s, seed, label):
         d = time.perf_counter() - t0
         dt += d
         rates.append(len(sel) / d)
-    rates = np.array(rates)
-    half = 2.365 * rates.std(ddof=1) / np.sqrt(8)   # Student t, 7 degrees of freedom, 95%
+    return vals, dt, np.array(rates)
+
+
+def ci95(rates):
+    half = 2.365 * rates.std(ddof=1) / np.sqrt(len(rates))   # Student t, 7 degrees of freedom, 95%
+    return [float(rates.mean() - half), float(rates.mean() + half)]
+
+
+def cpu_baseline(c, n, all_idx, tol, target_s, seed, label):
+    """Reference C build (oracle/_ref, the reference's Release flags) if present, else the oracle
+    restatement, on all host threads.  Where a build for a wider ISA level of the host CPU exists too,
+    it runs the same sample as `isa_variant`."""
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    model, flags = host_cpu()
+    if os.path.exists(orc.REF_LIB_PATH):
+        run = ref_runner(orc.REF_LIB_PATH, c, n, tol)
+        kind = "reference"
+    else:
+        h = (n - 1) // 2
+
+        def run(idx):
+            return orc.eri_sparse(c, ((-h, n), (-h, n), (-h, n)), idx, tol)
+        kind = "port"
+    # calibrate on a small seeded sample, then time a seeded sample sized to about target_s
+    rng = np.random.default_rng(2024)
+    probe_pos = np.sort(rng.choice(len(all_idx), size=min(len(all_idx), max(threads * 64, 1024)), replace=False))
+    t0 = time.perf_counter()
+    run(all_idx[probe_pos])
+    per = (time.perf_counter() - t0) / len(probe_pos)
+    count = int(min(len(all_idx), max(len(probe_pos), target_s / max(per, 1e-9))))
+    count = max(count, 8)
+    pos = np.sort(rng.choice(len(all_idx), size=count, replace=False))
+    vals, dt, rates = timed_sample(run, all_idx, pos, np.random.default_rng(7))
     rate = count / dt
-    return {"value": rate, "unit": "elements/s", "cores": threads, "kind": kind,
-            "ci95": [float(rates.mean() - half), float(rates.mean() + half)],
-            "full_set_s": len(all_idx) / rate,
-            "sample": "%d seeded uniformly drawn symmetry-unique %s four-indices (K=%d, seed %d, tol %g) in 8 "
-                      "sub-batches, %.1f s" % (count, label, len(c), seed, tol, dt)}, pos, vals
+    sample = ("%d seeded uniformly drawn symmetry-unique %s four-indices (K=%d, seed %d, tol %g) in 8 sub-batches, "
+              "%.1f s" % (count, label, len(c), seed, tol, dt))
+    out = {"value": rate, "unit": "elements/s", "cores": threads, "kind": kind, "ci95": ci95(rates),
+           "full_set_s": len(all_idx) / rate, "sample": sample, "cpu_model": model,
+           "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "build": "reference sources, gcc -O3 -DNDEBUG -fopenmp (the reference's Release flags)" if kind == "reference"
+                    else "oracle restatement, gcc -O3 -fopenmp"}
+    if kind == "reference":
+        for isa, suffix, need in ISA_LEVELS:
+            path = orc.REF_LIB_PATH.replace(".so", suffix + ".so")
+            if os.path.exists(path) and need <= flags:
+                run_isa = ref_runner(path, c, n, tol)
+                run_isa(all_idx[probe_pos])
+                v2, dt2, rates2 = timed_sample(run_isa, all_idx, pos, np.random.default_rng(7))
+                out["isa_variant"] = {"value": count / dt2, "unit": "elements/s", "isa": isa, "ci95": ci95(rates2),
+                                      "build": "reference sources, gcc -O3 -DNDEBUG -fopenmp -march=%s" % isa,
+                                      "seconds": dt2, "max_abs_diff_vs_release_build": float(np.max(np.abs(v2 - vals)))}
+                break
+    return out, pos, vals
 
 
 # grids of the mixed-size batch: (n, K, coefficient seed, spacing a in bohr); outputs are scaled by 1/a
 BATCH = [(5, 29, 0, 0.5), (7, 43, 0, 0.75), (9, 57, 1, 1.0), (11, 57, 0, 1.5)]
 
 
-def batch_bench(args):
+def batch_measure(L, args, stream):
     """One step = every unique four-index of every grid of the batch, from the coefficients to the
     scaled values in HBM, as one replay of the plan's captured graph.  The error is taken over every
-    element against the single-grid FP64 path (times 1/a)."""
+    element against the single-grid FP64 path (times 1/a).  Returns the measurement as a dict."""
     import torch
-    L = _lib.require_gpu()
-    for kv in args.tune:
-        knob, value = kv.split("=", 1)
-        _lib.check(L.gli_set_tuning(knob.encode(), int(value)), "gli_set_tuning " + kv)
-    stream = torch.cuda.ExternalStream(L.gli_stream())
-    method = METHODS[args.method]
+    method = METHODS[args.method] if args.method in ("tensor", "mixed") else 0
     n_g = len(BATCH)
     coeffs, grids, idxs, nums, vals = [], [], [], [], []
     enum_ms = 0.0
@@ -240,25 +292,40 @@ def batch_bench(args):
     L.gli_eri_batch_destroy(plan)
     for p in idxs:
         L.gli_device_free(p)
+    return {"workload": "eri_batch_5-7-9-11", "grids": BATCH, "method": "tensor" if method == 0 else "mixed",
+            "unique_elements": total, "ms_per_step": step_ms, "value": total / (step_ms / 1e3), "unit": "elements/s",
+            "steps": args.steps, "plan_create_ms": create_ms, "enumeration_ms": enum_ms, "autotune": tuned,
+            "errors": errs}
+
+
+def batch_bench(args):
+    """the mixed-size batch alone, as its own JSON line"""
+    import torch
+    L = _lib.require_gpu()
+    for kv in args.tune:
+        knob, value = kv.split("=", 1)
+        _lib.check(L.gli_set_tuning(knob.encode(), int(value)), "gli_set_tuning " + kv)
+    stream = torch.cuda.ExternalStream(L.gli_stream())
+    b = batch_measure(L, args, stream)
     line = {
         "metric": "unique ERI elements/s, mixed-size batch (5^3 K29, 7^3 K43, 9^3 K57, 11^3 K57), one graph replay per step",
-        "value": total / (step_ms / 1e3),
+        "value": b["value"],
         "unit": "elements/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": step_ms,
+        "ms_per_step": b["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPES[args.method],
+        "dtype": DTYPES[b["method"]],
         "data": "synthetic (seeded gausslet coefficients)",
-        "config": {"workload": "eri_batch_5-7-9-11", "grids": BATCH, "tol": args.tol, "method": args.method,
-                   "unique_elements": total},
-        "plan_create_ms": create_ms,
-        "enumeration_ms": enum_ms,
-        "autotune": tuned,
-        "errors": errs,
+        "config": {"workload": b["workload"], "grids": BATCH, "tol": args.tol, "method": b["method"],
+                   "unique_elements": b["unique_elements"]},
+        "plan_create_ms": b["plan_create_ms"],
+        "enumeration_ms": b["enumeration_ms"],
+        "autotune": b["autotune"],
+        "errors": b["errors"],
     }
     print(json.dumps(line), flush=True)
 
@@ -405,6 +472,8 @@ def main():
                     help="skip timing the reference entry points with host arrays (PCIe included)")
     ap.add_argument("--no-plan", action="store_true",
                     help="skip the one-grid batch-plan replay reported beside the single-call rate")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the mixed-size batch (5^3/7^3/9^3/11^3 plan replay) reported beside the single-call rate")
     ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
                     help="set a kernel tuning knob (gli_set_tuning) before timing; repeatable")
     ap.add_argument("--batch", action="store_true",
@@ -589,6 +658,12 @@ def main():
         del pv
         log("one-grid plan replay: %.3f ms/step (plan creation %.0f ms), gather %s" % (plan_ms, create_ms, chosen))
 
+    # the mixed-size batch of grids (5^3 K29, 7^3 K43, 9^3 K57 seed 1, 11^3 K57) as one plan replay
+    batch_line = None
+    if not args.no_batch:
+        batch_line = batch_measure(L, args, stream)
+        batch_line.pop("autotune", None)
+
     # the reference entry points with host arrays: indices in and values out over PCIe, the host
     # copies included (reported beside `value`, never as it)
     host_api = None
@@ -634,8 +709,12 @@ def main():
     Cp = (C1 + 63) // 64 * 64
     if name == "gather":
         achieved = 16.0 * num / (dom_ms / 1e3) / 1e9   # index in + value out, bytes per element
+        traffic = measured_traffic("k_gather", workload)
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_gather", workload),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "traffic_source": ("profiles/k_gather_%s_traffic.md: rocprofv3 FETCH_SIZE + WRITE_SIZE of one dispatch, "
+                                   "calibrated with a copy kernel of known size (a committed profile, not measured in "
+                                   "this run)" % workload) if traffic is not None else None,
                 "kernel": "k_gather", "algorithmic_bytes_per_launch": 16 * num}
     else:
         flops = {"mode1": 2.0 * Mp * Mp * Cp * Mp, "mode2": 2.0 * Mp * Cp * Cp * Mp / 2,
@@ -687,8 +766,10 @@ def main():
         "max_abs_err_vs_cpu_reference": cpu_err,
         "speedup_vs_fp64_direct": value / direct_rate,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "speedup_vs_cpu_host_api": (host_api["values_elements_per_s"] / cpu["value"]) if (cpu and host_api) else None,
         "compare": compare,
         "plan_replay": plan_line,
+        "batch": batch_line,
     }
     L.gli_device_free(d_idx)
     print(json.dumps(line), flush=True)

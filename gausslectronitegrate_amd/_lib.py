@@ -41,6 +41,7 @@ EXTRA = {
     "gli_device_malloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64]),
     "gli_device_free": (None, [C.c_void_p]),
     "gli_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int]),
+    "gli_memset": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
     "gli_eri_enumerate_device": (C.c_int, [C.c_int64, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "gli_erida_enumerate_device": (C.c_int, [C.c_int64, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "gli_eri_values_device": (C.c_int, [_abi.P_double, C.c_int64, _abi.P_glong, C.c_void_p, C.c_int64,

@@ -100,7 +100,24 @@ def build(verbose=False):
          ["-Wl,-Bsymbolic", "-Wl,--no-undefined", "-Wl,--version-script=" + vmap, "-lgomp", "-lm"])
     os.replace(tmp, LIB)
     build_hdf5(incs)
+    build_c_caller()
     return LIB
+
+
+C_CALLER_SRC = os.path.join(REPO, "tests", "c", "gli_caller.c")
+C_CALLER = os.path.join(REPO, "tests", "c", "gli_caller")
+
+
+def build_c_caller():
+    """tests/c/gli_caller: a plain C11 program built with gcc against include/ and linked with
+    -lgli_amd, the way an unchanged C caller of the reference links (run by the GPU tests)."""
+    if not os.path.exists(C_CALLER_SRC):
+        return None
+    tmp = C_CALLER + ".tmp"
+    _run([CC, "-std=c11", "-O2", "-Wall", "-Wextra", "-Werror", "-I" + INCLUDE, C_CALLER_SRC, "-o", tmp,
+          "-L" + LIB_DIR, "-lgli_amd", "-Wl,-rpath,$ORIGIN/../../gausslectronitegrate_amd/lib", "-Wl,--no-undefined"])
+    os.replace(tmp, C_CALLER)
+    return C_CALLER
 
 
 def build_hdf5(incs):

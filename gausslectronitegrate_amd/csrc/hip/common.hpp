@@ -129,6 +129,7 @@ struct Context {
 	int gather_xcd = 2;     // gather order: 0 consecutive pieces, 1 one contiguous chunk of the list per XCD,
 	                        // 2 chunks unless the key tensor outgrows the shared last-level cache
 	int gemm_shape = 0;     // FP64 mode-product tile shape (0: 64x64, 1: 128x64, 2: 64x128, 3: 128x128)
+	int project_chunk_kib = 262144; // host memory per chunk of product states in the ERI projection (256 MB)
 };
 
 Context& ctx();

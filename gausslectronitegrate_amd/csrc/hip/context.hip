@@ -296,6 +296,10 @@ int gli_set_tuning(const char* knob, int value)
 			GLI_REQUIRE(value >= 0 && value <= 2, "gather_order must be 0, 1 or 2");
 			c.gather_xcd = value;
 		}
+		else if (k == "project_chunk_kib") {
+			GLI_REQUIRE(value >= 1, "project_chunk_kib must be at least 1");
+			c.project_chunk_kib = value;
+		}
 		else {
 			GLI_REQUIRE(false, "unknown tuning knob");
 		}
@@ -321,6 +325,7 @@ int gli_get_tuning(const char* knob)
 		if (k == "batch_positions") { return c.batch_positions; }
 		if (k == "batch_sorted") { return c.batch_sorted; }
 		if (k == "gemm_shape") { return c.gemm_shape; }
+		if (k == "project_chunk_kib") { return c.project_chunk_kib; }
 		GLI_REQUIRE(false, "unknown tuning knob");
 	}
 	catch (const Failure&) {
@@ -399,6 +404,19 @@ int gli_memcpy(void* dst, const void* src, int64_t bytes, int kind)
 	try {
 		const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
 		GLI_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, ctx().stream));
+		GLI_HIP(hipStreamSynchronize(ctx().stream));
+		return 0;
+	}
+	catch (const Failure&) {
+		return -1;
+	}
+}
+
+int gli_memset(void* dst, int value, int64_t bytes)
+{
+	try {
+		GLI_REQUIRE(bytes >= 0 && (dst != nullptr || bytes == 0), "gli_memset needs a device pointer and a byte count >= 0");
+		GLI_HIP(hipMemsetAsync(dst, value, (size_t)bytes, ctx().stream));
 		GLI_HIP(hipStreamSynchronize(ctx().stream));
 		return 0;
 	}

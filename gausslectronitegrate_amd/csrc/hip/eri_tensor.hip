@@ -1249,15 +1249,16 @@ static void gather_dispatch(int ept, bool lds, F&& f)
 }
 
 // Gather order for a key tensor of nV entries.  One chunk of the list per XCD keeps each XCD's
-// key-tensor lines in its own L2; once the tensor is about as large as the shared last-level cache
-// (256 MB), consecutive pieces win instead: all XCDs then read neighbouring parts of the list and
-// share the cached lines (13^3: 7.93 against 8.22 ms; 11^3: 1.444 against 1.451 ms, for twice the
-// L2-to-fabric bytes).
+// key-tensor lines in its own L2.  Past a threshold of 128 MB of key tensor (half of the 256 MB
+// shared last-level cache) consecutive pieces win instead: all XCDs then read neighbouring parts
+// of the list and share the cached lines (13^3, 299 MB: 7.93 against 8.22 ms; 11^3, 71 MB: 1.444
+// against 1.451 ms, for twice the L2-to-fabric bytes).  Enumerated grids have odd sides, so no
+// tensor between these two sizes has been measured.
 static int resolve_gather_order(int knob, int64_t nV)
 {
-	constexpr int64_t kSharedCacheEntries = ((int64_t)128 << 20) / (int64_t)sizeof(double);
+	constexpr int64_t kChunkedOrderMaxEntries = ((int64_t)128 << 20) / (int64_t)sizeof(double);
 	if (knob != 2) { return knob; }
-	return nV > kSharedCacheEntries ? 0 : 1;
+	return nV > kChunkedOrderMaxEntries ? 0 : 1;
 }
 
 static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
@@ -1859,12 +1860,14 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 	BatchChoice ch;
 	std::string rep = "{\"modes\": [";
 	for (auto& g : plan.grids) { launch_tables(g->s, plan.d_params.p + g->coeff_off, plan.tol, g->f.p, g->P.p, g->B.p); }
+	// a mixed plan always runs the split-precision products (the autotuner picks among their
+	// variants only), so that the method a plan reports is the arithmetic it executes
 	const bool mixed = plan.method == GLI_ERI_TENSOR_MIXED;
 	for (int m = 0; m < 3; ++m) {
 		double best = 1e30;
 		rep += std::string(m ? ", " : "") + "{\"mode\": " + std::to_string(m + 1) + ", \"candidates\": [";
 		bool first = true;
-		for (int shape = 0; shape < kGemmShapes; ++shape) {
+		for (int shape = 0; shape < kGemmShapes && !mixed; ++shape) {
 			bool fits = true;
 			for (auto& g : plan.grids) {
 				const ModeArgs a = batch_mode_args(*g, m, plan.half);
@@ -2230,8 +2233,13 @@ int gli_sparse_eri_compute(const struct gausslet_data* gdata, const struct carte
 		GLI_REQUIRE(h_idx != nullptr && h_val != nullptr, "host allocation of the result arrays failed");
 		copy_to_host(h_idx, d_idx, (size_t)num * sizeof(int64_t));
 		copy_to_host(h_val, d_val, (size_t)num * sizeof(double));
-		GLI_HIP(hipFree(d_idx));
-		GLI_HIP(hipFree(d_val));
+		// each pointer is cleared before its free can fail, so the error path frees it at most once
+		int64_t* fi = d_idx;
+		d_idx = nullptr;
+		GLI_HIP(hipFree(fi));
+		double* fv = d_val;
+		d_val = nullptr;
+		GLI_HIP(hipFree(fv));
 		eri->grid = *grid;
 		eri->num_entries = num;
 		eri->four_indices = h_idx;

@@ -42,26 +42,34 @@ __device__ inline double norm3(double v0, double v1, double v2)
 	return big * sqrt(1 + r0 * r0 + r1 * r1);
 }
 
-struct Nuclei {
-	int num;
-	double pos[3 * 64];
-	double charge[64];
-};
+// The nuclei live in device memory as (x, y, z, Z) per nucleus, any number of them
+// (nuclear_integrals.c:133-143 copies an arbitrary num_nuclei); every lattice point sums them in
+// the caller's order, as the reference's loop over nuclei does (nuclear_integrals.c:234-244).
+// The nuclei are read in chunks of kNucleiChunk through LDS, so the wave's loads are uniform.
+constexpr int kNucleiChunk = 64;
 
-__global__ void k_nuclear_potential(Nuclei nuc, int ux0, int uy0, int uz0, int Ux, int Uy, int Uz, double* __restrict__ W)
+__global__ void k_nuclear_potential(const double4* __restrict__ nuc, int num, int ux0, int uy0, int uz0, int Ux, int Uy,
+                                    int Uz, double* __restrict__ W)
 {
+	__shared__ double4 chunk[kNucleiChunk];
 	const int iz = blockIdx.x * blockDim.x + threadIdx.x;
 	const int iy = blockIdx.y, ix = blockIdx.z;
-	if (iz >= Uz) { return; }
 	const double rx = (ux0 + ix) / 6.0, ry = (uy0 + iy) / 6.0, rz = (uz0 + iz) / 6.0;
 	const double w = 1. / 3;
 	double val = 0;
-	for (int k = 0; k < nuc.num; ++k) {
-		const double d = norm3(rx - nuc.pos[3 * k], ry - nuc.pos[3 * k + 1], rz - nuc.pos[3 * k + 2]);
-		const double g = (d > 0) ? erf(d / w) / d : 2 / (sqrt(M_PI) * w);
-		val += nuc.charge[k] * g;
+	for (int k0 = 0; k0 < num; k0 += kNucleiChunk) {
+		const int cnt = min(kNucleiChunk, num - k0);
+		__syncthreads();   // the previous chunk is no longer read
+		for (int q = threadIdx.x; q < cnt; q += blockDim.x) { chunk[q] = nuc[k0 + q]; }
+		__syncthreads();
+		for (int k = 0; k < cnt; ++k) {
+			const double4 nk = chunk[k];
+			const double d = norm3(rx - nk.x, ry - nk.y, rz - nk.z);
+			const double g = (d > 0) ? erf(d / w) / d : 2 / (sqrt(M_PI) * w);
+			val += nk.w * g;
+		}
 	}
-	W[((int64_t)ix * Uy + iy) * Uz + iz] = val;
+	if (iz < Uz) { W[((int64_t)ix * Uy + iy) * Uz + iz] = val; }
 }
 
 // f'[s][m] for s = 0..S-1, zeroed where |f| <= tol (gausslet_factors.c:12-77 with the skips of
@@ -115,6 +123,8 @@ __global__ void k_nuclear_scatter(const double* __restrict__ V, int nx, int ny, 
 // lock is held while they are used)
 struct OneBodyWorkspace {
 	DevBuf<double> dc, f, W, F0, F1, F2, Y1, Y2, V, M, t;
+	DevBuf<double4> nuc;            // (x, y, z, Z) of every nucleus
+	std::vector<double4> h_nuc;     // host copy of the same, uploaded on the library stream
 };
 
 static OneBodyWorkspace& onebody_ws()
@@ -130,7 +140,8 @@ static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	GLI_REQUIRE(K >= 3 && (K % 2) == 1, "the number of gausslet coefficients must be odd and at least 3");
 	GLI_REQUIRE(g[1] > 0 && g[3] > 0 && g[5] > 0, "grid axes must have a positive number of points");
-	GLI_REQUIRE(num_nuclei >= 0 && num_nuclei <= 64, "between 0 and 64 nuclei are supported");
+	GLI_REQUIRE(num_nuclei >= 0, "the number of nuclei must not be negative");
+	GLI_REQUIRE(num_nuclei == 0 || (pos != nullptr && charges != nullptr), "nuclei without positions or charges");
 	hipStream_t st = c.stream;
 	const int n[3] = { (int)g[1], (int)g[3], (int)g[5] };
 	const int g0[3] = { (int)g[0], (int)g[2], (int)g[4] };
@@ -142,13 +153,12 @@ static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 		U[a] = 6 * (n[a] - 1) + 2 * ((int)K - 1) + 1;
 		C[a] = n[a] * n[a];
 	}
-	Nuclei nuc;
-	nuc.num = num_nuclei;
-	for (int k = 0; k < num_nuclei; ++k) {
-		for (int a = 0; a < 3; ++a) { nuc.pos[3 * k + a] = pos[3 * k + a]; }
-		nuc.charge[k] = (double)charges[k];
-	}
 	OneBodyWorkspace& w = onebody_ws();
+	w.h_nuc.resize((size_t)std::max(num_nuclei, 1));
+	for (int k = 0; k < num_nuclei; ++k) {
+		w.h_nuc[(size_t)k] = make_double4(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2], (double)charges[k]);
+	}
+	w.nuc.reserve(w.h_nuc.size());
 	DevBuf<double>& dc = w.dc;
 	DevBuf<double>& f = w.f;
 	DevBuf<double>& W = w.W;
@@ -171,8 +181,10 @@ static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 	V.reserve((size_t)C[0] * C[1] * C[2]);
 	M.reserve((size_t)(N * N));
 	GLI_HIP(hipMemcpyAsync(dc.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, st));
+	upload_async(w.nuc.p, w.h_nuc.data(), (size_t)num_nuclei * sizeof(double4));
 	k_factors_tol<<<dim3((unsigned)ceil_div(L, 128), R), 128, 0, st>>>(dc.p, (int)K, tol, f.p);
-	k_nuclear_potential<<<dim3((unsigned)ceil_div(U[2], 64), U[1], U[0]), 64, 0, st>>>(nuc, u0[0], u0[1], u0[2], U[0], U[1], U[2], W.p);
+	k_nuclear_potential<<<dim3((unsigned)ceil_div(U[2], 64), U[1], U[0]), 64, 0, st>>>(w.nuc.p, num_nuclei, u0[0], u0[1], u0[2],
+	                                                                                    U[0], U[1], U[2], W.p);
 	double* Fs[3] = { F0.p, F1.p, F2.p };
 	for (int a = 0; a < 3; ++a) {
 		k_pair_factors<<<dim3((unsigned)ceil_div(C[a], 64), U[a]), 64, 0, st>>>(f.p, (int)K, n[a], g0[a], u0[a], U[a], Fs[a]);

@@ -68,3 +68,12 @@ void gli_require_coefficients(const struct gausslet_data* gdata, const char* wha
 		abort();
 	}
 }
+
+void* gli_require_host(void* p, size_t bytes, const char* what)
+{
+	if (p == NULL) {
+		fprintf(stderr, "gausslectronitegrate: %s: host allocation of %zu bytes failed\n", what, bytes);
+		abort();
+	}
+	return p;
+}

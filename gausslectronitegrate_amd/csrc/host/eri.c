@@ -98,13 +98,10 @@ void compute_sparse_eri_gausslet_integrals(const struct gausslet_data* gdata,
 	gli_require_coefficients(gdata, "compute_sparse_eri_gausslet_integrals");
 	eri->grid = *grid;
 	eri->num_entries = num_indices;
-	eri->four_indices = gli_host_alloc(num_indices * sizeof(glong));
-	eri->integral_values = gli_host_alloc(num_indices * sizeof(double));
-	if (eri->four_indices == NULL || eri->integral_values == NULL) {
-		fprintf(stderr, "gausslectronitegrate: compute_sparse_eri_gausslet_integrals: host allocation of %lld entries failed\n",
-		        (long long)num_indices);
-		abort();
-	}
+	eri->four_indices = gli_require_host(gli_host_alloc(num_indices * sizeof(glong)), num_indices * sizeof(glong),
+	                                     "compute_sparse_eri_gausslet_integrals");
+	eri->integral_values = gli_require_host(gli_host_alloc(num_indices * sizeof(double)), num_indices * sizeof(double),
+	                                        "compute_sparse_eri_gausslet_integrals");
 	int64_t g[6];
 	gli_grid_array(grid, g);
 	/* the index copy is made while the indices stream to the device */
@@ -183,35 +180,48 @@ void apply_sparse_eri_tensor(const struct sparse_eri_gausslet_integrals* eri, co
 }
 
 /* eri_proj[p,q,r,s] = sum_{ijkl} b[i,p] b[j,q] b[k,r] b[l,s] (ij|kl)  (eri_integrals.c:714-830):
- * the GPU applies the tensor to the S^2 product states x[(j,l),(q,s)] = b[j,q] b[l,s], which gives
- * R[(i,k),(q,s)]; the two remaining modes i, k are contracted here */
+ * the GPU applies the tensor to the product states x[(j,l),(q,s)] = b[j,q] b[l,s], which gives
+ * R[(i,k),(q,s)]; the two remaining modes i, k are contracted here.  The S^2 product states go
+ * through the GPU in chunks of at most the "project_chunk_kib" tuning knob (256 MB by default) of
+ * states, and as many of results, so the host memory of a call stays bounded by the chunk, not by
+ * 2 N^2 S^2 doubles (11 GB at 11^3 with 20 states).  Each state column is applied and contracted
+ * on its own, so the chunking does not change any value. */
+
 void project_sparse_eri_gausslet_integrals(const struct sparse_eri_gausslet_integrals* eri, const double* basis, const glong num_states, double* eri_proj)
 {
 	const glong N = cartesian_grid_3d_num_points(&eri->grid);
 	const glong S = num_states, S2 = S * S;
 	memset(eri_proj, 0, S2 * S2 * sizeof(double));
 	if (S == 0) { return; }
-	double* x = gli_host_alloc(N * N * S2 * sizeof(double));
-	double* R = gli_host_alloc(N * N * S2 * sizeof(double));
-	double* T = aligned_calloc(N * S * S2 * sizeof(double));
-	#pragma omp parallel for
-	for (glong j = 0; j < N; ++j) {
-		for (glong l = 0; l < N; ++l) {
-			for (glong q = 0; q < S; ++q) {
-				for (glong s = 0; s < S; ++s) { x[(j * N + l) * S2 + q * S + s] = basis[j * S + q] * basis[l * S + s]; }
+	const size_t chunk_bytes = (size_t)gli_get_tuning("project_chunk_kib") << 10;
+	glong cs = (glong)(chunk_bytes / ((size_t)N * N * sizeof(double)));
+	cs = cs < 1 ? 1 : (cs > S2 ? S2 : cs);
+	const char* what = "project_sparse_eri_gausslet_integrals";
+	double* x = gli_require_host(gli_host_alloc(N * N * cs * sizeof(double)), N * N * cs * sizeof(double), what);
+	double* R = gli_require_host(gli_host_alloc(N * N * cs * sizeof(double)), N * N * cs * sizeof(double), what);
+	double* T = gli_require_host(aligned_calloc(N * S * S2 * sizeof(double)), N * S * S2 * sizeof(double), what);
+	for (glong c0 = 0; c0 < S2; c0 += cs) {
+		const glong cn = (S2 - c0 < cs) ? S2 - c0 : cs;
+		#pragma omp parallel for
+		for (glong j = 0; j < N; ++j) {
+			for (glong l = 0; l < N; ++l) {
+				for (glong c = 0; c < cn; ++c) {
+					const glong q = (c0 + c) / S, s = (c0 + c) % S;
+					x[(j * N + l) * cn + c] = basis[j * S + q] * basis[l * S + s];
+				}
 			}
 		}
-	}
-	apply_sparse_eri_tensor(eri, x, S2, R);
-	/* T[i, r, qs] = sum_k b[k, r] R[(i, k), qs] */
-	#pragma omp parallel for
-	for (glong i = 0; i < N; ++i) {
-		for (glong k = 0; k < N; ++k) {
-			const double* Rik = &R[(i * N + k) * S2];
-			for (glong r = 0; r < S; ++r) {
-				const double b = basis[k * S + r];
-				double* Tir = &T[(i * S + r) * S2];
-				for (glong qs = 0; qs < S2; ++qs) { Tir[qs] += b * Rik[qs]; }
+		apply_sparse_eri_tensor(eri, x, cn, R);
+		/* T[i, r, qs] = sum_k b[k, r] R[(i, k), qs] */
+		#pragma omp parallel for
+		for (glong i = 0; i < N; ++i) {
+			for (glong k = 0; k < N; ++k) {
+				const double* Rik = &R[(i * N + k) * cn];
+				for (glong r = 0; r < S; ++r) {
+					const double b = basis[k * S + r];
+					double* Tir = &T[(i * S + r) * S2 + c0];
+					for (glong c = 0; c < cn; ++c) { Tir[c] += b * Rik[c]; }
+				}
 			}
 		}
 	}

@@ -83,8 +83,10 @@ void compute_sparse_erida_gausslet_integrals(const struct gausslet_data* gdata,
 	gli_require_coefficients(gdata, "compute_sparse_erida_gausslet_integrals");
 	erida->grid = *grid;
 	erida->num_entries = num_indices;
-	erida->two_indices = aligned_malloc(num_indices * sizeof(glong));
-	erida->integral_values = aligned_malloc(num_indices * sizeof(double));
+	erida->two_indices = gli_require_host(aligned_malloc(num_indices * sizeof(glong)), num_indices * sizeof(glong),
+	                                      "compute_sparse_erida_gausslet_integrals");
+	erida->integral_values = gli_require_host(aligned_malloc(num_indices * sizeof(double)), num_indices * sizeof(double),
+	                                          "compute_sparse_erida_gausslet_integrals");
 	if (num_indices > 0) { memcpy(erida->two_indices, two_indices, num_indices * sizeof(glong)); }
 	int64_t g[6];
 	gli_grid_array(grid, g);
@@ -155,19 +157,19 @@ void project_sparse_erida_gausslet_integrals(const struct sparse_erida_gausslet_
 	const glong S = num_states, S2 = S * S;
 	glong* perm[48];
 	evaluate_octahedral_grid_permutations(&erida->grid, perm);
-	double* V = aligned_malloc(N * N * sizeof(double));
+	double* V = gli_require_host(aligned_malloc(N * N * sizeof(double)), N * N * sizeof(double), "project_sparse_erida_gausslet_integrals");
 	#pragma omp parallel for schedule(dynamic)
 	for (glong i = 0; i < N; ++i) {
 		for (glong j = 0; j < N; ++j) { V[i * N + j] = erida_dense_value(erida, (const glong* const*)perm, &erida->grid, i, j); }
 	}
 	/* Khatri-Rao rows KR[i, (p,q)] = b[i,p] b[i,q];  proj = KR^T V KR */
-	double* KR = aligned_malloc(N * S2 * sizeof(double));
+	double* KR = gli_require_host(aligned_malloc(N * S2 * sizeof(double)), N * S2 * sizeof(double), "project_sparse_erida_gausslet_integrals");
 	for (glong i = 0; i < N; ++i) {
 		for (glong p = 0; p < S; ++p) {
 			for (glong q = 0; q < S; ++q) { KR[i * S2 + p * S + q] = basis[i * S + p] * basis[i * S + q]; }
 		}
 	}
-	double* VK = aligned_malloc(N * S2 * sizeof(double));
+	double* VK = gli_require_host(aligned_malloc(N * S2 * sizeof(double)), N * S2 * sizeof(double), "project_sparse_erida_gausslet_integrals");
 	#pragma omp parallel for
 	for (glong i = 0; i < N; ++i) {
 		for (glong rs = 0; rs < S2; ++rs) {
@@ -212,13 +214,17 @@ void compute_nuclear_gausslet_integrals(const struct gausslet_data* gdata, const
 {
 	gli_require_coefficients(gdata, "compute_nuclear_gausslet_integrals");
 	ngi->grid = *grid;
-	ngi->nuclei = aligned_malloc(num_nuclei * sizeof(struct atomic_nucleus));
+	ngi->nuclei = gli_require_host(aligned_malloc(num_nuclei * sizeof(struct atomic_nucleus)), num_nuclei * sizeof(struct atomic_nucleus),
+	                               "compute_nuclear_gausslet_integrals");
 	if (num_nuclei > 0) { memcpy(ngi->nuclei, nuclei, num_nuclei * sizeof(struct atomic_nucleus)); }
 	ngi->num_nuclei = num_nuclei;
 	const glong N = cartesian_grid_3d_num_points(grid);
-	ngi->integral_values = gli_host_alloc(N * N * sizeof(double));
-	double* pos = malloc((3 * num_nuclei + 1) * sizeof(double));
-	int* chg = malloc((num_nuclei + 1) * sizeof(int));
+	ngi->integral_values = gli_require_host(gli_host_alloc(N * N * sizeof(double)), N * N * sizeof(double),
+	                                        "compute_nuclear_gausslet_integrals");
+	double* pos = gli_require_host(malloc((3 * num_nuclei + 1) * sizeof(double)), (3 * num_nuclei + 1) * sizeof(double),
+	                               "compute_nuclear_gausslet_integrals");
+	int* chg = gli_require_host(malloc((num_nuclei + 1) * sizeof(int)), (num_nuclei + 1) * sizeof(int),
+	                            "compute_nuclear_gausslet_integrals");
 	nucleus_arrays(nuclei, num_nuclei, pos, chg);
 	int64_t g[6];
 	gli_grid_array(grid, g);
@@ -244,9 +250,11 @@ double compute_nuclear_gausslet_integral(const struct gausslet_data* gdata, cons
 	for (int p = 0; p < 2; ++p) {
 		lin[p] = ((points[p].x - g[0]) * g[3] + (points[p].y - g[2])) * g[5] + (points[p].z - g[4]);
 	}
-	double* M = aligned_malloc(N * N * sizeof(double));
-	double* pos = malloc((3 * num_nuclei + 1) * sizeof(double));
-	int* chg = malloc((num_nuclei + 1) * sizeof(int));
+	double* M = gli_require_host(aligned_malloc(N * N * sizeof(double)), N * N * sizeof(double), "compute_nuclear_gausslet_integral");
+	double* pos = gli_require_host(malloc((3 * num_nuclei + 1) * sizeof(double)), (3 * num_nuclei + 1) * sizeof(double),
+	                               "compute_nuclear_gausslet_integral");
+	int* chg = gli_require_host(malloc((num_nuclei + 1) * sizeof(int)), (num_nuclei + 1) * sizeof(int),
+	                            "compute_nuclear_gausslet_integral");
 	nucleus_arrays(nuclei, num_nuclei, pos, chg);
 	gli_check(gli_hip_nuclear_matrix(gdata->coefficients, gdata->indices.num, g, pos, chg, num_nuclei, tol, M),
 	          "compute_nuclear_gausslet_integral");
@@ -274,7 +282,8 @@ void compute_kinetic_gausslet_integrals(const struct gausslet_data* gdata, const
 {
 	kgi->grid = *grid;
 	const glong N = cartesian_grid_3d_num_points(grid);
-	kgi->integral_values = gli_host_alloc(N * N * sizeof(double));
+	kgi->integral_values = gli_require_host(gli_host_alloc(N * N * sizeof(double)), N * N * sizeof(double),
+	                                        "compute_kinetic_gausslet_integrals");
 	int64_t g[6];
 	gli_grid_array(grid, g);
 	gli_check(gli_hip_kinetic_matrix(gdata->coefficients, gdata->indices.num, g, kgi->integral_values), "compute_kinetic_gausslet_integrals");
@@ -293,7 +302,7 @@ double compute_kinetic_gausslet_integral(const struct gausslet_data* gdata, cons
 	for (int p = 0; p < 2; ++p) {
 		lin[p] = ((points[p].x - g[0]) * g[3] + (points[p].y - g[2])) * g[5] + (points[p].z - g[4]);
 	}
-	double* M = aligned_malloc(N * N * sizeof(double));
+	double* M = gli_require_host(aligned_malloc(N * N * sizeof(double)), N * N * sizeof(double), "compute_kinetic_gausslet_integral");
 	gli_check(gli_hip_kinetic_matrix(gdata->coefficients, gdata->indices.num, g, M), "compute_kinetic_gausslet_integral");
 	const double v = M[lin[0] * N + lin[1]];
 	aligned_free(M);

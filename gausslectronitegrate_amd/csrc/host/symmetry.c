@@ -3,6 +3,7 @@
  * the signed-axis-permutation codes of the group elements. */
 #include "symmetry.h"
 #include "aligned_memory.h"
+#include "host_internal.h"
 #include "oh_tables.inc"
 
 const struct imat3x3 octahedral_matrep[48] = { GLI_OH_MATRICES };
@@ -24,7 +25,7 @@ void evaluate_octahedral_grid_permutations(const struct cartesian_grid_3d* grid,
 {
 	const glong n = cartesian_grid_3d_num_points(grid);
 	for (int e = 0; e < 48; ++e) {
-		perm[e] = aligned_malloc(n * sizeof(glong));
+		perm[e] = gli_require_host(aligned_malloc(n * sizeof(glong)), n * sizeof(glong), "evaluate_octahedral_grid_permutations");
 		compute_cartesian_grid_3d_permutation(grid, &octahedral_matrep[e], perm[e]);
 	}
 }

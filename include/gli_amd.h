@@ -63,6 +63,8 @@ int gli_synchronize(void);
  *   "gather_order" 0: workgroups take consecutive pieces of the list; 1: each XCD takes one contiguous
  *                 chunk of the list (its L2 then holds the key-tensor entries it reuses); 2 (default): 1
  *                 while the key tensor is at most 128 MB (11^3: 71 MB), else 0 (13^3: 299 MB)
+ *   "project_chunk_kib" host memory (KiB, default 262144 = 256 MB) of one chunk of product states in
+ *                 project_sparse_eri_gausslet_integrals; the states and as many results are held per chunk
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
 int gli_set_tuning(const char* knob, int value);
 int gli_get_tuning(const char* knob);
@@ -88,6 +90,8 @@ int gli_direct_flops(double* reference_form, double* executed);
 int gli_device_malloc(void** p, int64_t bytes);
 void gli_device_free(void* p);
 int gli_memcpy(void* dst, const void* src, int64_t bytes, int kind);
+/* every byte of a device range set to 'value' (library stream, synchronous) */
+int gli_memset(void* dst, int value, int64_t bytes);
 
 /* device-resident enumeration of a symmetric n^3 grid; *d_out is released with gli_device_free */
 int gli_eri_enumerate_device(int64_t n, int64_t** d_out, int64_t* num);

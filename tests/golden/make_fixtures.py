@@ -9,9 +9,11 @@ needs h5dump and the reference build oracle/_ref/libgli_ref.so):
   2. the seeded synthetic gausslet coefficient sets of the workloads;
   3. golden vectors of the reference build for the synthetic workloads:
      full results on the small grids, seeded samples of canonical indices
-     on the large ones.
+     on the large ones, all ERI-DA pairs of the 13^3 grid;
+  4. count, SHA-256 and a strided subsample of the reference build's
+     complete 9^3 and 11^3 enumeration lists.
 
-Usage: python tests/golden/make_fixtures.py [--skip-ref-runs]
+Usage: python tests/golden/make_fixtures.py [--skip-ref-runs] [--parts a,b,...]
 """
 import argparse
 import ctypes as C
@@ -206,9 +208,6 @@ def sample_canonical(lib, n, count, seed):
 def golden_synthetic(lib):
     tol = 1e-16
     c57s0 = synthetic_coefficients(57, 0)
-    c57s1 = synthetic_coefficients(57, 1)
-    c43s0 = synthetic_coefficients(43, 0)
-    c29s0 = synthetic_coefficients(29, 0)
     # small grid: everything in full (3^3, K=57, seed 0)
     n = 3
     idx = ref_enumerate(lib, n)
@@ -237,9 +236,18 @@ def golden_synthetic(lib):
                         erida_indices=eda_idx, erida_values=eda, nuc_pos=np.array(pos),
                         nuc_charges=np.array([1, 1]), nuc_rows=rows, nuc_cols=cols,
                         nuc_sample=nuc[rows, cols], nuc_rowsum=nuc.sum(axis=1), nuc_trace=np.trace(nuc))
-    # sampled canonical ERI indices with reference values at the large grids
-    cases = [(5, 29, c29s0, "K29s0", 256), (7, 43, c43s0, "K43s0", 256), (9, 57, c57s1, "K57s1", 192),
-             (11, 57, c57s0, "K57s0", 128), (13, 57, c57s0, "K57s0", 64)]
+
+
+def golden_samples(lib):
+    """Seeded samples of canonical ERI indices with reference values at the large grids: the
+    mixed-size batch (5^3 K29, 7^3 K43, 9^3 K57 seed 1, 11^3 K57) and 13^3.  A larger count
+    with the same seed extends the smaller sample (the draws are the same sequence)."""
+    tol = 1e-16
+    cases = [(5, 29, synthetic_coefficients(29, 0), "K29s0", 4096),
+             (7, 43, synthetic_coefficients(43, 0), "K43s0", 4096),
+             (9, 57, synthetic_coefficients(57, 1), "K57s1", 4096),
+             (11, 57, synthetic_coefficients(57, 0), "K57s0", 4096),
+             (13, 57, synthetic_coefficients(57, 0), "K57s0", 2048)]
     for n, K, c, tag, count in cases:
         idx = sample_canonical(lib, n, count, seed=1000 + n)
         vals = ref_eri_values(lib, c, n, idx, tol)
@@ -248,15 +256,69 @@ def golden_synthetic(lib):
         print("sample n=%d K=%d: %d values, max |v| %.3g" % (n, K, len(vals), np.abs(vals).max()))
 
 
+def golden_erida13(lib):
+    """every one of the 455 unique ERI-DA pairs of the 13^3 grid (K = 57, seed 0) with the
+    reference build's values"""
+    tol = 1e-16
+    c = synthetic_coefficients(57, 0)
+    idx = ref_enumerate_erida(lib, 13)
+    vals = ref_erida_values(lib, c, 13, idx, tol)
+    np.savez_compressed(os.path.join(HERE, "golden_erida_n13_K57s0.npz"), coeffs=c, tol=tol,
+                        erida_indices=idx, erida_values=vals)
+    print("erida 13^3: %d pairs" % len(idx))
+
+
+ENUM_STRIDE = 1 << 16
+
+
+def enum_digests(lib, grids=(9, 11)):
+    """Count, SHA-256 of the little-endian int64 bytes and a strided subsample (every 2^16-th entry,
+    plus the first and last 1024) of the reference build's complete enumeration lists.  The 11^3
+    run holds about 17 GB of list nodes."""
+    import hashlib
+    import time
+    out = {}
+    for n in grids:
+        t0 = time.time()
+        idx = ref_enumerate(lib, n)
+        dt = time.time() - t0
+        out["n%d_count" % n] = np.int64(len(idx))
+        out["n%d_sha256" % n] = np.array(hashlib.sha256(idx.astype("<i8").tobytes()).hexdigest())
+        out["n%d_strided" % n] = idx[::ENUM_STRIDE].copy()
+        out["n%d_head" % n] = idx[:1024].copy()
+        out["n%d_tail" % n] = idx[-1024:].copy()
+        print("enumeration %d^3: %d entries in %.1f s, sha256 %s" % (n, len(idx), dt, out["n%d_sha256" % n]))
+        del idx
+    out["stride"] = np.int64(ENUM_STRIDE)
+    np.savez_compressed(os.path.join(HERE, "golden_enum_digests.npz"), **out)
+
+
 def main():
+    parts_all = ("convert", "coeffs", "synthetic", "samples", "erida13", "enum-digests")
     ap = argparse.ArgumentParser()
-    ap.add_argument("--skip-ref-runs", action="store_true")
+    ap.add_argument("--skip-ref-runs", action="store_true", help="only the parts that need no reference build")
+    ap.add_argument("--parts", default=",".join(parts_all),
+                    help="comma-separated subset of %s" % ", ".join(parts_all))
     args = ap.parse_args()
-    convert_reference_fixtures()
-    for K, seed in ((29, 0), (43, 0), (57, 0), (57, 1)):
-        np.save(os.path.join(HERE, "coeffs_K%d_s%d.npy" % (K, seed)), synthetic_coefficients(K, seed))
+    parts = set(args.parts.split(","))
+    unknown = parts - set(parts_all)
+    if unknown:
+        ap.error("unknown parts: %s" % ", ".join(sorted(unknown)))
+    if "convert" in parts:
+        convert_reference_fixtures()
+    if "coeffs" in parts:
+        for K, seed in ((29, 0), (43, 0), (57, 0), (57, 1)):
+            np.save(os.path.join(HERE, "coeffs_K%d_s%d.npy" % (K, seed)), synthetic_coefficients(K, seed))
     if not args.skip_ref_runs:
-        golden_synthetic(load_ref())
+        lib = load_ref()
+        if "synthetic" in parts:
+            golden_synthetic(lib)
+        if "samples" in parts:
+            golden_samples(lib)
+        if "erida13" in parts:
+            golden_erida13(lib)
+        if "enum-digests" in parts:
+            enum_digests(lib)
     print("fixtures written to", HERE)
 
 
