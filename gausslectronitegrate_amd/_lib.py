@@ -48,6 +48,8 @@ EXTRA = {
                                         C.c_double, C.c_int, C.c_void_p]),
     "gli_erida_values_device": (C.c_int, [_abi.P_double, C.c_int64, _abi.P_glong, C.c_void_p, C.c_int64,
                                           C.c_double, C.c_void_p]),
+    "gli_eri_values_device_noalloc": (C.c_int, [_abi.P_double, C.c_int64, _abi.P_glong, C.c_void_p, C.c_int64,
+                                                C.c_double, C.c_int, C.c_void_p]),
     "gli_eri_values_batch_device": (C.c_int, [C.c_int, C.POINTER(_abi.P_double), _abi.P_glong, _abi.P_glong,
                                               C.POINTER(C.c_void_p), _abi.P_glong, C.c_double, C.c_int,
                                               C.POINTER(C.c_void_p)]),

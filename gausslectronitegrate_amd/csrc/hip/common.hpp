@@ -137,6 +137,13 @@ Context& ctx();
 // ERI / ERIDA Coulomb tables of at least n entries, device resident
 const double* eri_table(int64_t n);
 const double* erida_table(int64_t n);
+// the ERI Coulomb table of at least n entries in host memory (kept by the context; a later call
+// that needs more entries may move it)
+const double* eri_table_host(int64_t n);
+
+// looks up the device address of the allocation-free ERI entry point's scratch on `device` (the
+// current device) and keeps it; a failure is recorded and reported by that entry point
+void eri_scratch_init(int device);
 
 // Output-tile order of the mode products.  Workgroups are dispatched round-robin over the 8 XCDs
 // (workgroup b runs on XCD b % 8), each with its own L2.  With the grouped order the column tiles

@@ -34,6 +34,8 @@ Context& ctx()
 		GLI_HIP(hipGetDevice(&c->device));
 		// default priority: non-default priorities are not used anywhere in this library
 		GLI_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+		// the device address of the allocation-free entry point's scratch on this device
+		eri_scratch_init(c->device);
 	});
 	return *c;
 }
@@ -71,6 +73,14 @@ static const double* table(std::vector<double>& host, DevBuf<double>& dev, int64
 		dev = std::move(fresh);
 	}
 	return dev.p;
+}
+
+const double* eri_table_host(int64_t n)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	extend_table(c.eri_table_host, n, 1.L / 3, 6.L);
+	return c.eri_table_host.data();
 }
 
 const double* eri_table(int64_t n)

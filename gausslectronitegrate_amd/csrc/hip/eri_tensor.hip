@@ -1036,14 +1036,18 @@ struct KeySet {
 	int C1 = 0;
 	std::vector<int4> keys;  // (a, b, cd, 0) of each distinct folded column
 	std::vector<int> lut;    // (a * R + b) * (4R - 3) + cd + 2(R - 1) -> key id
+	std::vector<short> lut16;    // the same table as int16 (key ids stay below 32768)
+	std::vector<uint32_t> tet3;  // tet_index(0, 0, k) per key
+	bool tet32 = false;         // every tetrahedral index fits 32 bits
+	bool on_device = false;     // the DevBuf copies below exist
 	DevBuf<int4> keys_dev;
 	DevBuf<int> lut_dev;
-	DevBuf<short> lut16_dev;  // the same table as int16 (key ids stay below 32768)
-	DevBuf<uint32_t> tet3_dev;  // tet_index(0, 0, k) per key
-	bool tet32 = false;         // every tetrahedral index fits 32 bits
+	DevBuf<short> lut16_dev;
+	DevBuf<uint32_t> tet3_dev;
 };
 
-static KeySet& key_set(int R, bool symmetric)
+// the key set of per-axis length R in host memory, built once per (R, symmetric) and kept
+static KeySet& key_tables(int R, bool symmetric)
 {
 	static std::map<std::pair<int, bool>, KeySet*> cache;
 	auto it = cache.find({R, symmetric});
@@ -1078,22 +1082,32 @@ static KeySet& key_set(int R, bool symmetric)
 		}
 	}
 	ks->C1 = (int)ks->keys.size();
-	ks->keys_dev.alloc(ks->keys.size());
-	ks->lut_dev.alloc(ks->lut.size());
-	GLI_HIP(hipMemcpy(ks->keys_dev.p, ks->keys.data(), ks->keys.size() * sizeof(int4), hipMemcpyHostToDevice));
-	GLI_HIP(hipMemcpy(ks->lut_dev.p, ks->lut.data(), ks->lut.size() * sizeof(int), hipMemcpyHostToDevice));
 	GLI_REQUIRE(ks->C1 < 32768, "too many axis keys for the 16-bit key table");
-	std::vector<short> l16(ks->lut.begin(), ks->lut.end());
-	ks->lut16_dev.alloc(l16.size());
-	GLI_HIP(hipMemcpy(ks->lut16_dev.p, l16.data(), l16.size() * sizeof(short), hipMemcpyHostToDevice));
+	ks->lut16.assign(ks->lut.begin(), ks->lut.end());
 	// C(k3 + 2, 3): tetrahedral offset of the largest key, as long as every index fits 32 bits
 	ks->tet32 = tet_index(0, 0, ks->C1) <= (int64_t)UINT32_MAX;
-	std::vector<uint32_t> t3((size_t)ks->C1);
-	for (int k = 0; k < ks->C1; ++k) { t3[k] = ks->tet32 ? (uint32_t)tet_index(0, 0, k) : 0u; }
-	ks->tet3_dev.alloc(t3.size());
-	GLI_HIP(hipMemcpy(ks->tet3_dev.p, t3.data(), t3.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+	ks->tet3.resize((size_t)ks->C1);
+	for (int k = 0; k < ks->C1; ++k) { ks->tet3[(size_t)k] = ks->tet32 ? (uint32_t)tet_index(0, 0, k) : 0u; }
 	cache[{R, symmetric}] = ks;
 	return *ks;
+}
+
+// the same key set with its device copies (allocated once per key set)
+static KeySet& key_set(int R, bool symmetric)
+{
+	KeySet& ks = key_tables(R, symmetric);
+	if (!ks.on_device) {
+		ks.keys_dev.alloc(ks.keys.size());
+		ks.lut_dev.alloc(ks.lut.size());
+		ks.lut16_dev.alloc(ks.lut16.size());
+		ks.tet3_dev.alloc(ks.tet3.size());
+		GLI_HIP(hipMemcpy(ks.keys_dev.p, ks.keys.data(), ks.keys.size() * sizeof(int4), hipMemcpyHostToDevice));
+		GLI_HIP(hipMemcpy(ks.lut_dev.p, ks.lut.data(), ks.lut.size() * sizeof(int), hipMemcpyHostToDevice));
+		GLI_HIP(hipMemcpy(ks.lut16_dev.p, ks.lut16.data(), ks.lut16.size() * sizeof(short), hipMemcpyHostToDevice));
+		GLI_HIP(hipMemcpy(ks.tet3_dev.p, ks.tet3.data(), ks.tet3.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+		ks.on_device = true;
+	}
+	return ks;
 }
 
 struct EriWorkspace {
@@ -1129,11 +1143,17 @@ struct EriSetup {
 	int K, R, M, Mp, C1, Cp;
 	GridDev g;
 	const KeySet* ks;
+	// device copies of the Coulomb table and of the key set's tables (the library's cached buffers,
+	// or the allocation-free entry point's scratch)
 	const double* T;
+	const int4* d_keys;
+	const short* d_lut16;
+	const uint32_t* d_tet3;
 };
 
-// shapes, key set and Coulomb table for K coefficients on a grid (no device work but the table)
-static EriSetup eri_shape(int64_t K, const int64_t grid[6], bool sym)
+// shapes, key set and Coulomb table for K coefficients on a grid (no device work but the tables).
+// device = false: host tables only, with every device pointer left null for the caller to set.
+static EriSetup eri_shape(int64_t K, const int64_t grid[6], bool sym, bool device = true)
 {
 	GLI_REQUIRE(K >= 3 && (K % 2) == 1, "the number of gausslet coefficients must be odd and at least 3");
 	GLI_REQUIRE(grid[1] > 0 && grid[3] > 0 && grid[5] > 0, "grid axes must have a positive number of points");
@@ -1148,7 +1168,7 @@ static EriSetup eri_shape(int64_t K, const int64_t grid[6], bool sym)
 	s.M = 6 * (s.R - 1) + 2 * (s.K - 1);
 	s.Mp = (int)round_up(s.M + 1, 16);
 	GLI_REQUIRE(s.Mp <= 2048, "grid too long: the folded kernel range 6(n-1) + 2(K-1) must stay below 2048");
-	const KeySet& ks = key_set(s.R, sym);
+	const KeySet& ks = device ? key_set(s.R, sym) : key_tables(s.R, sym);
 	s.ks = &ks;
 	s.C1 = ks.C1;
 	s.Cp = (int)round_up(s.C1, KEY_ALIGN);
@@ -1159,7 +1179,10 @@ static EriSetup eri_shape(int64_t K, const int64_t grid[6], bool sym)
 	s.g.R = s.R;
 	s.g.N2 = (int64_t)s.g.N * s.g.N;
 	s.g.N4 = s.g.N2 * s.g.N2;
-	s.T = eri_table(3 * (int64_t)s.M * s.M + 1);
+	s.T = device ? eri_table(3 * (int64_t)s.M * s.M + 1) : nullptr;
+	s.d_keys = device ? ks.keys_dev.p : nullptr;
+	s.d_lut16 = device ? ks.lut16_dev.p : nullptr;
+	s.d_tet3 = device ? ks.tet3_dev.p : nullptr;
 	return s;
 }
 
@@ -1171,7 +1194,7 @@ static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol,
 	const int L = 2 * s.K - 1, LP = 4 * s.K - 3;
 	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, 0, st>>>(d_coeffs, s.K, f);
 	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, 0, st>>>(f, s.K, s.R, tol, P);
-	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.ks->keys_dev.p, s.C1, s.K, s.R, s.M, s.Cp, B);
+	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, s.Cp, B);
 	GLI_HIP(hipGetLastError());
 }
 
@@ -1210,8 +1233,8 @@ static GatherDesc gather_desc(const EriSetup& s, const int64_t* d_idx, int64_t n
 	d.num = num;
 	d.g = s.g;
 	d.dc = decode_consts(s);
-	d.lut = s.ks->lut16_dev.p;
-	d.tet3 = s.ks->tet3_dev.p;
+	d.lut = s.d_lut16;
+	d.tet3 = s.d_tet3;
 	d.V = V;
 	d.out = out;
 	d.scale = scale;
@@ -1467,6 +1490,33 @@ struct EriPrepared {
 	size_t direct_shm = 0;
 };
 
+// The FP64 key tensor V from the folded columns B and the kernel tensor T3: three MFMA mode products
+//   Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]   (symmetric in (m2, m3): with half, only m3 >= m2
+//                                                        is computed, and mode 2 reads the rest mirrored)
+//   Y2[m3][k1][k2] = sum_m2 Y1[m2][m3][k1] B[m2][k2]   (k2 >= k1 only)
+//   V[k1 <= k2 <= k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3], written in tetrahedral layout
+// queued on the library stream.  Every FP64 entry point runs exactly this sequence.
+static void key_tensor_fp64(const EriSetup& s, const double* B, const double* T3, double* Y1, double* Y2, double* V, bool half)
+{
+	const int64_t Mp = s.Mp, Cp = s.Cp;
+	if (half) {
+		ModeArgs a{T3, Mp * Mp, Mp * Mp, B, Cp, s.Cp, s.Mp, Y1, Cp, s.Cp, 0, 0, 0, 0};
+		a.half_n = (int)Mp;
+		mode_product_args(a);
+	}
+	else { mode_product(T3, Mp * Mp, Mp * Mp, B, Cp, s.Cp, s.Mp, Y1, Cp, s.Cp, 0, 0, 0, 0); }
+	phase_mark("mode1");
+	if (half) {
+		ModeArgs a{Y1, Mp * Cp, Mp * Cp, B, Cp, s.Cp, s.Mp, Y2, Cp, s.Cp, 1, 0, 0, 0};
+		a.sym_x = 1;
+		mode_product_args(a);
+	}
+	else { mode_product(Y1, Mp * Cp, Mp * Cp, B, Cp, s.Cp, s.Mp, Y2, Cp, s.Cp, 1, 0, 0, 0); }
+	phase_mark("mode2");
+	mode_product(Y2, Cp * Cp, Cp * Cp, B, Cp, s.Cp, s.Mp, V, 0, s.Cp, 1, 1, 0, s.C1);
+	phase_mark("mode3");
+}
+
 static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t grid[6], double tol, int method, bool any)
 {
 	Context& c = ctx();
@@ -1526,35 +1576,130 @@ static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t gr
 		split_factor(w.B.p, Cp, s.Cp, s.Mp, nullptr, ns);
 		phase_mark("split_factor");
 	}
-	// Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]; symmetric in (m2, m3), so the FP64 path
-	// computes m3 >= m2 only and mode 2 reads the other half mirrored
 	const bool half = !mixed && !fp32 && c.mode1_half;
 	w.Y1.reserve((size_t)Mp * Mp * Cp);
-	if (mixed) { mode_product_split(w.T3.p, Mp * Mp, Mp * Mp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0, true); }
-	else if (half) {
-		ModeArgs a{w.T3.p, Mp * Mp, Mp * Mp, w.B.p, Cp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0};
-		a.half_n = (int)Mp;
-		mode_product_args(a);
-	}
-	else { mode(w.T3.p, Mp * Mp, Mp * Mp, w.Y1.p, Cp, 0, 0, 0); }
-	phase_mark("mode1");
-	// Y2[m3][k1][k2] = sum_m2 Y1[m2][m3][k1] B[m2][k2]   (k2 >= k1 only)
 	w.Y2.reserve((size_t)Mp * Cp * Cp);
-	if (mixed) { mode_product_split(w.Y1.p, Mp * Cp, Mp * Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0, false); }
-	else if (half) {
-		ModeArgs a{w.Y1.p, Mp * Cp, Mp * Cp, w.B.p, Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0};
-		a.sym_x = 1;
-		mode_product_args(a);
-	}
-	else { mode(w.Y1.p, Mp * Cp, Mp * Cp, w.Y2.p, Cp, 1, 0, 0); }
-	phase_mark("mode2");
-	// V[k1 <= k2 <= k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3], written in tetrahedral layout
 	const int64_t nV = tet_index(0, 0, s.C1);
 	w.V.reserve((size_t)nV);
+	if (!mixed && !fp32) {
+		key_tensor_fp64(s, w.B.p, w.T3.p, w.Y1.p, w.Y2.p, w.V.p, half);
+		return ep;
+	}
+	// split-precision and FP32 comparison methods
+	if (mixed) { mode_product_split(w.T3.p, Mp * Mp, Mp * Mp, s.Cp, s.Mp, w.Y1.p, Cp, s.Cp, 0, 0, 0, 0, true); }
+	else { mode(w.T3.p, Mp * Mp, Mp * Mp, w.Y1.p, Cp, 0, 0, 0); }
+	phase_mark("mode1");
+	if (mixed) { mode_product_split(w.Y1.p, Mp * Cp, Mp * Cp, s.Cp, s.Mp, w.Y2.p, Cp, s.Cp, 1, 0, 0, 0, false); }
+	else { mode(w.Y1.p, Mp * Cp, Mp * Cp, w.Y2.p, Cp, 1, 0, 0); }
+	phase_mark("mode2");
 	if (mixed) { mode_product_split(w.Y2.p, Cp * Cp, Cp * Cp, s.Cp, s.Mp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1, false); }
 	else { mode(w.Y2.p, Cp * Cp, Cp * Cp, w.V.p, 0, 1, 1, s.C1); }
 	phase_mark("mode3");
 	return ep;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free evaluation (gli_eri_values_device_noalloc), for callers that cannot allocate device
+// memory.  The whole workspace of one FP64 evaluation (coefficients, factor / product / fold tables,
+// Coulomb table, kernel tensor, the two intermediate products, the key tensor and the key tables) is
+// carved out of one file-scope device array, `scratch`.  Every GPU the library runs on holds its own
+// copy of the array from module load to process end.  In host code the array's name is only a
+// host-side stand-in: its address on a device comes from hipGetSymbolAddress, once per device at
+// context set-up (eri_scratch_init, with that device current), and only that address is passed to
+// kernels and copies.  Each buffer is written by its producing kernel before anything reads it, so
+// nothing has to start at zero; the array keeps the last call's contents, and calls that overlap on
+// one GPU share it and race.
+constexpr size_t kEriScratchBytes = (size_t)96 << 20;   // grids up to 7^3 with K <= 57 (82 MB)
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kEriScratchBytes];
+static void* g_scratch_ptr[kScratchMaxDevices] = {};
+static hipError_t g_scratch_err[kScratchMaxDevices] = {};
+
+void eri_scratch_init(int device)
+{
+	if (device < 0 || device >= kScratchMaxDevices) { return; }
+	void* p = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	g_scratch_err[device] = e;
+	g_scratch_ptr[device] = (e == hipSuccess) ? p : nullptr;
+}
+
+// consecutive 256-byte aligned pieces of the scratch
+struct ScratchCarve {
+	char* base;
+	size_t cap;
+	size_t used = 0;
+	template <typename T>
+	T* take(int64_t count)
+	{
+		const size_t at = (used + 255) & ~(size_t)255;
+		used = at + (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+		return (T*)(base + at);
+	}
+};
+
+static void eri_values_noalloc(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_idx, int64_t num,
+                               double tol, int method, double* d_vals)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64, "the allocation-free entry point computes the FP64 key-tensor method (0) only");
+	GLI_REQUIRE(coeffs != nullptr && K >= 3, "the number of gausslet coefficients must be odd and at least 3");
+	GLI_REQUIRE(num >= 0, "negative number of indices");
+	GLI_REQUIRE(num == 0 || (d_idx != nullptr && d_vals != nullptr), "null index or value array");
+	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
+	char* base = (c.device >= 0 && c.device < kScratchMaxDevices) ? (char*)g_scratch_ptr[c.device] : nullptr;
+	if (base == nullptr) {
+		const hipError_t e = (c.device >= 0 && c.device < kScratchMaxDevices) ? g_scratch_err[c.device] : hipErrorInvalidDevice;
+		GLI_REQUIRE(false, std::string("no scratch address on this device: ") + hipGetErrorString(e));
+	}
+	EriSetup s = eri_shape(K, grid, mirror_symmetric(coeffs, K), false);
+	GLI_REQUIRE(s.ks->tet32, "key tensor too large for 32-bit tetrahedral offsets");
+	const int64_t Mp = s.Mp, Cp = s.Cp, nT = 3 * (int64_t)s.M * s.M + 1;
+	const int64_t nlut = (int64_t)s.R * s.R * (4 * s.R - 3);
+	ScratchCarve sc{base, kEriScratchBytes};
+	double* d_coeffs = sc.take<double>(K);
+	double* f = sc.take<double>((int64_t)s.R * (2 * s.K - 1));
+	double* P = sc.take<double>((int64_t)s.R * s.R * (4 * s.K - 3));
+	double* B = sc.take<double>(Mp * Cp);
+	double* T = sc.take<double>(nT);
+	double* T3 = sc.take<double>(Mp * Mp * Mp);
+	double* Y1 = sc.take<double>(Mp * Mp * Cp);
+	double* Y2 = sc.take<double>(Mp * Cp * Cp);
+	double* V = sc.take<double>(tet_index(0, 0, s.C1));
+	int4* keys = sc.take<int4>(s.C1);
+	short* lut16 = sc.take<short>(nlut);
+	uint32_t* tet3 = sc.take<uint32_t>(s.C1);
+	if (sc.used > sc.cap) {
+		char msg[256];
+		snprintf(msg, sizeof(msg), "the allocation-free entry point needs %.1f MB of scratch for this grid and K, it holds %zu MB "
+		         "(grids up to 7^3 with K <= 57); use gli_eri_values_device", sc.used / 1048576.0, kEriScratchBytes >> 20);
+		GLI_REQUIRE(false, msg);
+	}
+	// small host tables into the scratch; the copies are complete before the host sources can move
+	const double* hT = eri_table_host(nT);
+	hipStream_t st = c.stream;
+	GLI_HIP(hipMemcpyAsync(d_coeffs, coeffs, (size_t)K * sizeof(double), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(T, hT, (size_t)nT * sizeof(double), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(keys, s.ks->keys.data(), (size_t)s.C1 * sizeof(int4), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(lut16, s.ks->lut16.data(), (size_t)nlut * sizeof(short), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(tet3, s.ks->tet3.data(), (size_t)s.C1 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipStreamSynchronize(st));
+	s.T = T;
+	s.d_keys = keys;
+	s.d_lut16 = lut16;
+	s.d_tet3 = tet3;
+	// the same kernels in the same order as gli_eri_values_device with the FP64 method
+	phase_begin();
+	launch_tables(s, d_coeffs, tol, f, P, B);
+	phase_mark("tables");
+	if (num == 0) { return; }
+	launch_kernel_tensor(s, T3);
+	phase_mark("kernel_tensor");
+	key_tensor_fp64(s, B, T3, Y1, Y2, V, c.mode1_half != 0);
+	launch_gather(gather_desc(s, d_idx, num, V, d_vals, nullptr), c.gather_ept, resolve_gather_order(c.gather_xcd, tet_index(0, 0, s.C1)),
+	              c.gather_lds);
+	phase_mark("gather");
 }
 
 // values of num indices (device arrays) from a prepared evaluation, queued on the library stream
@@ -2251,6 +2396,18 @@ int gli_sparse_eri_compute(const struct gausslet_data* gdata, const struct carte
 		if (d_val != nullptr) { (void)hipFree(d_val); }
 		free(h_idx);
 		free(h_val);
+		return -1;
+	}
+}
+
+int gli_eri_values_device_noalloc(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_idx, int64_t num,
+                                  double tol, int method, double* d_vals)
+{
+	try {
+		eri_values_noalloc(coeffs, K, grid, d_idx, num, tol, method, d_vals);
+		return 0;
+	}
+	catch (const Failure&) {
 		return -1;
 	}
 }

@@ -104,6 +104,18 @@ int gli_eri_values_device(const double* coeffs, int64_t K, const int64_t grid[6]
 int gli_erida_values_device(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_indices,
                             int64_t num, double tol, double* d_values);
 
+/* gli_eri_values_device without device allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It allocates nothing on the device: the whole workspace comes from one static
+ * 96 MB scratch array per GPU, held from library load to process end, which serves grids up to 7^3 with
+ * K <= 57 (larger ones return -1 with the size they need).  Method GLI_ERI_TENSOR_FP64 only; the values
+ * are bit-identical to gli_eri_values_device's.  Synchronises the library stream once (after copying
+ * the coefficients and small host tables in), then queues the kernels on it.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap (from several host threads without the library's
+ * serialisation, or with a previous call's kernels still reading it from another stream) race. */
+int gli_eri_values_device_noalloc(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_indices,
+                                  int64_t num, double tol, int method, double* d_values);
+
 /* Many single integrals in one GPU evaluation: the batched forms of compute_eri_gausslet_integral
  * (eri_integrals.c:133-232) and compute_erida_gausslet_integral (erida_integrals.c:119-201).
  * points[(e * 4 + p) * 3 + a] is axis a of point p (i, j, k, l) of element e (two points i, j for
