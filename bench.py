@@ -508,7 +508,7 @@ def main():
         knob, value = kv.split("=", 1)
         _lib.check(L.gli_set_tuning(knob.encode(), int(value)), "gli_set_tuning " + kv)
     tuning = {k: L.gli_get_tuning(k.encode()) for k in ("tile_order", "gemm_shape", "gather_ept", "gather_order", "gather_lds",
-                                                      "mode1_half", "gather_threads")}
+                                                      "mode1_half", "gather_threads", "gather_bucketed")}
     method = METHODS[args.method]
     c = synthetic_coefficients(args.K, args.seed, symmetrise=not args.asymmetric)
     g = sym_grid6(args.n)

@@ -478,9 +478,15 @@ struct DecodeConsts {
 	float inv_N;     // 1 / N
 	float inv_nz;    // 1 / n_z
 	float inv_nyz;   // 1 / (n_y n_z)
-	// reciprocals rounded up by one ulp: for a < 2^12 (float) and a < 2^24 (double, divisor
-	// N) resp. a < 2^48 (divisor N^2), trunc(a * inv_up) is the exact quotient, no correction
-	// step needed (exhaustively checked for every divisor up to 4096)
+	// Reciprocals rounded up by one ulp: inv_up = next_up(fl(1/d)) > 1/d, with inv_up = (1/d)(1 + delta),
+	// 0 < delta <= 1.5 ulp (2^-52 in double, 2^-23 in float).  For a = q d + r, 0 <= r < d:
+	//  * fl(a inv_up) >= q: the exact product exceeds a/d >= q, q is representable, rounding is monotone;
+	//  * fl(a inv_up) < q + 1: a/d <= q + 1 - 1/d, and the product's relative excess (delta plus one
+	//    rounding, below 2 ulp) times q + 1 stays under 1/d whenever (a + d) * 2 ulp < 1, i.e. for
+	//    a + d < 2^21 in float and a + d < 2^51 in double.
+	// So trunc(a * inv_up) is the exact quotient for a < 2^12 with divisors up to 4096 (float),
+	// a < 2^24 with divisor N and a < 2^48 with divisor N^2 <= 2^24 (double); a CPU test checks the float
+	// case exhaustively and the double cases at their worst numerators.
 	double inv_N2_up, inv_N_up;
 	float inv_nyz_up, inv_nz_up;
 	int uniform_pairs;   // 1: grids with N <= 4096 and tet indices below 2^32 take the fast path
@@ -868,6 +874,200 @@ __global__ __launch_bounds__(256) void k_gather_grouped(const GatherDesc* __rest
 	const int64_t first = (piece - d.piece_begin) * (blockDim.x >> 6) * (64 * GATHER_EPT);
 	if (piece < d.piece_begin || first >= d.num) { return; }   // whole workgroup idle
 	gather_block<GATHER_EPT, kLds>(d, first + (threadIdx.x >> 6) * (64 * GATHER_EPT));
+}
+
+// ---------------------------------------------------------------------------
+// Single-call gather with the key-tensor reads grouped by cache line inside windows.
+//
+// In list order, consecutive elements change their z key almost every time, so the 64 lanes of a
+// load read 64 scattered key-tensor entries and touch about 45 distinct 128-byte lines: the L1 miss
+// path, not HBM, bounds the plain gather.  Here a workgroup of 1024 lanes takes a window of 8192
+// consecutive elements, decodes every element's key-tensor position (the decode of gather_wave),
+// and counting-sorts the window by bucket = position >> shift in LDS: one LDS atomic per element
+// gives its rank inside its bucket, a scan of the bucket counts gives the bucket starts.  Reading
+// the key tensor in bucket order puts the elements that share a line into the same 64-lane load.
+// The values return to their list places through LDS and leave in coalesced stores.  Each value is
+// the same key-tensor entry gather_wave reads, so the results are bit-identical.
+constexpr int kBucketEpt = 8;           // elements per lane: each wave takes 8 groups of 64 consecutive elements
+constexpr int kBucketMaxThreads = 1024;
+
+// LDS bytes of the bucketed gather's region for workgroups of `threads` lanes: it holds the values of
+// the window (8 bytes per element); before that, the sort buffers (4-byte position and 2-byte place
+// per element, one 4-byte bin per 2 elements); before that, the key tables
+__host__ __device__ constexpr int bucket_lds_bytes(int threads)
+{
+	return threads * kBucketEpt * (int)sizeof(double);
+}
+
+// key-tensor positions of one wave's kBucketEpt groups of 64 consecutive elements, kNoPosition past
+// the end of the list and for indices outside the grid (needs dc.uniform_pairs: tetrahedral indices
+// below 2^32); the decode of gather_wave
+__device__ inline void wave_positions(const GatherDesc& d, int64_t wave_base, const short* __restrict__ lut,
+                                      const uint32_t* __restrict__ tet3, uint32_t vi[kBucketEpt])
+{
+	const int lane = threadIdx.x & 63;
+	const GridDev& g = d.g;
+	const DecodeConsts& dc = d.dc;
+	int64_t t[kBucketEpt];
+#pragma unroll
+	for (int u = 0; u < kBucketEpt; ++u) {
+		const int64_t e = wave_base + u * 64 + lane;
+		t[u] = (e < d.num) ? d.idx[e] : -1;
+	}
+	const int W = 4 * g.R - 3, off = 2 * (g.R - 1);
+	const int64_t t0 = uniform_i64(t[0]);
+	int64_t pair_base = -1;
+	int base_key[3] = { 0, 0, 0 };
+	if (t0 >= 0 && t0 < g.N4) {
+		const int64_t ij = (int64_t)((double)t0 * dc.inv_N2_up);
+		const unsigned i = (unsigned)((double)(unsigned)ij * dc.inv_N_up);
+		const unsigned j = (unsigned)ij - __umul24(i, (unsigned)g.N);
+		unsigned ci[3], cj[3];
+		point_xyz(i, g, dc, ci[0], ci[1], ci[2]);
+		point_xyz(j, g, dc, cj[0], cj[1], cj[2]);
+#pragma unroll
+		for (int a = 0; a < 3; ++a) {
+			const int dd = abs((int)ci[a] - (int)cj[a]);
+			base_key[a] = __builtin_amdgcn_readfirstlane(dd * g.R * W + (int)(ci[a] + cj[a]) + off);
+		}
+		pair_base = uniform_i64(ij * g.N2);
+	}
+	bool fast = pair_base >= 0;
+#pragma unroll
+	for (int u = 0; u < kBucketEpt; ++u) { fast = fast && (uint64_t)(t[u] - pair_base) < (uint64_t)g.N2; }
+	if (__all(fast)) {
+		const unsigned short* __restrict__ ulut = (const unsigned short*)lut;
+		unsigned kk[kBucketEpt][3];
+#pragma unroll
+		for (int u = 0; u < kBucketEpt; ++u) {
+			const unsigned kl = (unsigned)(t[u] - pair_base);
+			const unsigned k = (unsigned)((double)kl * dc.inv_N_up);
+			const unsigned l = kl - __umul24(k, (unsigned)g.N);
+			unsigned ck[3], cl[3];
+			point_xyz(k, g, dc, ck[0], ck[1], ck[2]);
+			point_xyz(l, g, dc, cl[0], cl[1], cl[2]);
+#pragma unroll
+			for (int a = 0; a < 3; ++a) {
+				const unsigned li = __umul24(__usad(ck[a], cl[a], 0u), (unsigned)W) + ((unsigned)base_key[a] - ck[a] - cl[a]);
+				kk[u][a] = ulut[li];
+			}
+		}
+		unsigned k3[kBucketEpt];
+#pragma unroll
+		for (int u = 0; u < kBucketEpt; ++u) {
+			const unsigned lo = min(kk[u][0], kk[u][1]), hi = max(kk[u][0], kk[u][1]);
+			const unsigned mid = min(hi, kk[u][2]);
+			k3[u] = max(hi, kk[u][2]);
+			const unsigned k1 = min(lo, mid), k2 = max(lo, mid);
+			vi[u] = __umul24(k1, k3[u] + 1u) - (__umul24(k1, k1 - 1u) >> 1) + (k2 - k1);
+		}
+#pragma unroll
+		for (int u = 0; u < kBucketEpt; ++u) { vi[u] += tet3[k3[u]]; }
+	}
+	else {
+#pragma unroll
+		for (int u = 0; u < kBucketEpt; ++u) {
+			vi[u] = (t[u] >= 0 && t[u] < g.N4) ? (uint32_t)gather_index(g, dc, lut, t[u]) : kNoPosition;
+		}
+	}
+}
+
+// One workgroup of kThreads lanes per window of kThreads * 8 elements; bins <= 4 * kThreads buckets.
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void k_gather_bucketed(GatherDesc d, int64_t xcd_chunks, int shift, int bins)
+{
+	constexpr int kWin = kThreads * kBucketEpt;
+	__shared__ __align__(16) unsigned char lds[bucket_lds_bytes(kThreads)];
+	__shared__ unsigned wave_tot[kThreads / 64];
+	const int64_t base = gather_piece(xcd_chunks) * kWin;
+	if (base >= d.num) { return; }   // whole workgroup idle
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	// phase 1: key tables into LDS, then every element's position (registers)
+	uint32_t* st = (uint32_t*)lds;
+	short* sl = (short*)(lds + 4 * ((d.C1 + 3) & ~3));
+	for (int q = tid; q < d.C1; q += kThreads) { st[q] = d.tet3[q]; }
+	for (int q = tid; q < d.nlut; q += kThreads) { sl[q] = d.lut[q]; }
+	__syncthreads();
+	uint32_t pos[kBucketEpt];
+	const int first = wave * (64 * kBucketEpt);   // this wave's first place in the window
+	wave_positions(d, base + first, sl, st, pos);
+	__syncthreads();   // the decode tables are no longer read: the region becomes the sort buffers
+	// phase 2: counting sort of the window's positions by bucket = position >> shift
+	uint32_t* spos = (uint32_t*)lds;                         // [kWin] positions in bucket order
+	uint16_t* splace = (uint16_t*)(lds + 4 * kWin);         // [kWin] their places in the window
+	unsigned* hist = (unsigned*)(lds + 6 * kWin);           // [4 * kThreads] counts, then starts
+	for (int q = tid; q < bins; q += kThreads) { hist[q] = 0; }
+	__syncthreads();
+	unsigned rank[kBucketEpt];
+#pragma unroll
+	for (int u = 0; u < kBucketEpt; ++u) {
+		rank[u] = (pos[u] != kNoPosition) ? atomicAdd(&hist[pos[u] >> shift], 1u) : 0u;
+	}
+	__syncthreads();
+	// exclusive scan of the counts: lane tid owns bins 4 tid .. 4 tid + 3
+	unsigned c[4];
+#pragma unroll
+	for (int q = 0; q < 4; ++q) { c[q] = (4 * tid + q < bins) ? hist[4 * tid + q] : 0u; }
+	const unsigned own = c[0] + c[1] + c[2] + c[3];
+	unsigned inc = own;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const unsigned y = __shfl_up(inc, o, 64);
+		if (lane >= o) { inc += y; }
+	}
+	if (lane == 63) { wave_tot[wave] = inc; }
+	__syncthreads();
+	unsigned ex = inc - own, nvalid = 0;
+#pragma unroll
+	for (int w = 0; w < kThreads / 64; ++w) {
+		const unsigned wt = wave_tot[w];
+		ex += (w < wave) ? wt : 0u;
+		nvalid += wt;
+	}
+#pragma unroll
+	for (int q = 0; q < 4; ++q) {
+		if (4 * tid + q < bins) { hist[4 * tid + q] = ex; }
+		ex += c[q];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int u = 0; u < kBucketEpt; ++u) {
+		if (pos[u] != kNoPosition) {
+			const unsigned slot = hist[pos[u] >> shift] + rank[u];
+			spos[slot] = pos[u];
+			splace[slot] = (uint16_t)(first + u * 64 + lane);
+		}
+	}
+	__syncthreads();
+	// phase 3: key-tensor reads in bucket order, values to their places, coalesced stores
+	uint32_t qp[kBucketEpt];
+	unsigned ql[kBucketEpt];
+#pragma unroll
+	for (int r = 0; r < kBucketEpt; ++r) {
+		const unsigned sidx = (unsigned)(r * kThreads + tid);
+		qp[r] = (sidx < nvalid) ? spos[sidx] : kNoPosition;
+		ql[r] = (sidx < nvalid) ? splace[sidx] : 0u;
+	}
+	__syncthreads();   // the sort buffers are read: the region holds the values now
+	double* vals = (double*)lds;
+	double v[kBucketEpt];
+#pragma unroll
+	for (int r = 0; r < kBucketEpt; ++r) { v[r] = (qp[r] != kNoPosition) ? d.V[qp[r]] : 0.0; }
+	const double sc = (d.scale != nullptr) ? *d.scale : 1.0;
+#pragma unroll
+	for (int r = 0; r < kBucketEpt; ++r) {
+		if (qp[r] != kNoPosition) { vals[ql[r]] = (d.scale != nullptr) ? v[r] * sc : v[r]; }
+	}
+#pragma unroll
+	for (int u = 0; u < kBucketEpt; ++u) {
+		if (pos[u] == kNoPosition) { vals[first + u * 64 + lane] = __builtin_nan(""); }
+	}
+	__syncthreads();
+#pragma unroll
+	for (int u = 0; u < kBucketEpt; ++u) {
+		const int64_t e = base + first + u * 64 + lane;
+		if (e < d.num) { d.out[e] = vals[first + u * 64 + lane]; }
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -1277,15 +1477,54 @@ static void gather_dispatch(int ept, bool lds, F&& f)
 // of the list and share the cached lines (13^3, 299 MB: 7.93 against 8.22 ms; 11^3, 71 MB: 1.444
 // against 1.451 ms, for twice the L2-to-fabric bytes).  Enumerated grids have odd sides, so no
 // tensor between these two sizes has been measured.
+constexpr int64_t kLargeKeyTensorEntries = ((int64_t)128 << 20) / (int64_t)sizeof(double);
+
 static int resolve_gather_order(int knob, int64_t nV)
 {
-	constexpr int64_t kChunkedOrderMaxEntries = ((int64_t)128 << 20) / (int64_t)sizeof(double);
 	if (knob != 2) { return knob; }
-	return nV > kChunkedOrderMaxEntries ? 0 : 1;
+	return nV > kLargeKeyTensorEntries ? 0 : 1;
+}
+
+// the bucketed gather needs 32-bit positions (the fast decode's grids) and room for the key tables
+static bool gather_bucketed_fits(const GatherDesc& d, int threads)
+{
+	return d.dc.uniform_pairs && gather_lds_bytes(d.nlut, d.C1) <= bucket_lds_bytes(threads);
+}
+
+// bucket = position >> shift, with at most max_bins buckets over the nV key-tensor entries
+static void bucket_shape(int64_t nV, int max_bins, int& shift, int& bins)
+{
+	shift = 0;
+	while (((nV - 1) >> shift) + 1 > max_bins) { ++shift; }
+	bins = (int)(((nV - 1) >> shift) + 1);
+}
+
+static void launch_gather_bucketed(const GatherDesc& d, int xcd, int threads)
+{
+	const int64_t pieces = ceil_div(d.num, (int64_t)threads * kBucketEpt);
+	if (pieces == 0) { return; }
+	const int64_t chunks = xcd ? ceil_div(pieces, 8) : 0;
+	const int64_t blocks = xcd ? 8 * chunks : pieces;
+	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
+	int shift = 0, bins = 1;
+	bucket_shape(std::max<int64_t>(tet_index(0, 0, d.C1), 1), 4 * threads, shift, bins);
+	hipStream_t st = ctx().stream;
+	if (threads == 1024) { k_gather_bucketed<1024><<<(unsigned)blocks, 1024, 0, st>>>(d, chunks, shift, bins); }
+	else if (threads == 512) { k_gather_bucketed<512><<<(unsigned)blocks, 512, 0, st>>>(d, chunks, shift, bins); }
+	else { k_gather_bucketed<256><<<(unsigned)blocks, 256, 0, st>>>(d, chunks, shift, bins); }
+	GLI_HIP(hipGetLastError());
 }
 
 static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 {
+	// window-bucketed reads: on with knob 1; with knob 2 (automatic) for key tensors above 128 MB,
+	// where they measured faster (13^3: 7.50 against 7.88 ms; 11^3: 1.52 against 1.46 ms)
+	const int bk = ctx().gather_bucketed;
+	const bool bucketed = bk == 1 || (bk == 2 && tet_index(0, 0, d.C1) > kLargeKeyTensorEntries);
+	if (bucketed && gather_bucketed_fits(d, ctx().gather_threads)) {
+		launch_gather_bucketed(d, xcd, ctx().gather_threads);
+		return;
+	}
 	const int threads = ctx().gather_threads;
 	const int64_t pieces = ceil_div(d.num, (int64_t)threads * ept);
 	if (pieces == 0) { return; }

@@ -63,6 +63,10 @@ int gli_synchronize(void);
  *   "gather_order" 0: workgroups take consecutive pieces of the list; 1: each XCD takes one contiguous
  *                 chunk of the list (its L2 then holds the key-tensor entries it reuses); 2 (default): 1
  *                 while the key tensor is at most 128 MB (11^3: 71 MB), else 0 (13^3: 299 MB)
+ *   "gather_bucketed" the single-grid gather decodes a window of 8 x gather_threads elements, counting-sorts
+ *                 the window's key-tensor positions by bucket in LDS and reads the key tensor in that order
+ *                 (the elements of one cache line share a load), storing the values back in list order:
+ *                 0 never, 1 always, 2 (default) when the key tensor exceeds 128 MB (13^3: 299 MB)
  *   "project_chunk_kib" host memory (KiB, default 262144 = 256 MB) of one chunk of product states in
  *                 project_sparse_eri_gausslet_integrals; the states and as many results are held per chunk
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */

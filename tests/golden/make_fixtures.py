@@ -9,7 +9,8 @@ needs h5dump and the reference build oracle/_ref/libgli_ref.so):
   2. the seeded synthetic gausslet coefficient sets of the workloads;
   3. golden vectors of the reference build for the synthetic workloads:
      full results on the small grids, seeded samples of canonical indices
-     on the large ones, all ERI-DA pairs of the 13^3 grid;
+     on the large ones, all ERI-DA pairs of the 13^3 grid, nuclear matrices
+     for many nuclei with the reference's own re-association spread;
   4. count, SHA-256 and a strided subsample of the reference build's
      complete 9^3 and 11^3 enumeration lists.
 
@@ -268,6 +269,42 @@ def golden_erida13(lib):
     print("erida 13^3: %d pairs" % len(idx))
 
 
+def golden_nuclear_many(lib):
+    """Nuclear matrices of the reference build for many nuclei: 100 seeded nuclei (charges 1-3) on a
+    3x4x5 grid with K = 29 (tol 1e-14), the first 64 of them alone, and 70 seeded nuclei (charges 1-2)
+    on the 3^3 grid with K = 57 (tol 1e-16).  With this many nuclei an entry sums ~10^5 signed terms
+    per nucleus, so each case also stores the reference's own spread under re-association: its matrix
+    over all nuclei against the sum of its per-nucleus matrices."""
+    def spread(c, grid, pos, chg, tol, full):
+        per = np.zeros_like(full)
+        for k in range(len(chg)):
+            per += ref_nuclear(lib, c, grid, pos[k:k + 1], chg[k:k + 1], tol)
+        return float(np.max(np.abs(full - per)))
+    out = {}
+    c29 = synthetic_coefficients(29, 0)
+    rng = np.random.default_rng(100)
+    pos = rng.uniform(-2.5, 2.5, size=(100, 3))
+    chg = rng.integers(1, 4, size=100).astype(np.int32)
+    grid = ((-1, 3), (-2, 4), (-2, 5))
+    m100 = ref_nuclear(lib, c29, grid, pos, chg, 1e-14)
+    m64 = ref_nuclear(lib, c29, grid, pos[:64], chg[:64], 1e-14)
+    out.update(many_coeffs=c29, many_tol=1e-14, many_grid=np.array(grid, dtype=np.int64), many_pos=pos, many_charges=chg,
+               many_m100=m100, many_spread100=spread(c29, grid, pos, chg, 1e-14, m100),
+               many_m64=m64, many_spread64=spread(c29, grid, pos[:64], chg[:64], 1e-14, m64))
+    c57 = synthetic_coefficients(57, 0)
+    rng = np.random.default_rng(70)
+    nuc = np.zeros((70, 4))
+    nuc[:, :3] = rng.uniform(-1.5, 1.5, size=(70, 3))
+    nuc[:, 3] = rng.integers(1, 3, size=70)
+    g3 = sym_grid(3)
+    m70 = ref_nuclear(lib, c57, g3, nuc[:, :3], nuc[:, 3].astype(np.int32), 1e-16)
+    out.update(caller_coeffs=c57, caller_tol=1e-16, caller_nuclei=nuc, caller_m70=m70,
+               caller_spread70=spread(c57, g3, nuc[:, :3], nuc[:, 3].astype(np.int32), 1e-16, m70))
+    np.savez_compressed(os.path.join(HERE, "golden_nuclear_many.npz"), **out)
+    print("nuclear many: spreads %.3g (100), %.3g (64), %.3g (70)" % (out["many_spread100"], out["many_spread64"],
+                                                                       out["caller_spread70"]))
+
+
 ENUM_STRIDE = 1 << 16
 
 
@@ -294,7 +331,7 @@ def enum_digests(lib, grids=(9, 11)):
 
 
 def main():
-    parts_all = ("convert", "coeffs", "synthetic", "samples", "erida13", "enum-digests")
+    parts_all = ("convert", "coeffs", "synthetic", "samples", "erida13", "nuclear-many", "enum-digests")
     ap = argparse.ArgumentParser()
     ap.add_argument("--skip-ref-runs", action="store_true", help="only the parts that need no reference build")
     ap.add_argument("--parts", default=",".join(parts_all),
@@ -317,6 +354,8 @@ def main():
             golden_samples(lib)
         if "erida13" in parts:
             golden_erida13(lib)
+        if "nuclear-many" in parts:
+            golden_nuclear_many(lib)
         if "enum-digests" in parts:
             enum_digests(lib)
     print("fixtures written to", HERE)

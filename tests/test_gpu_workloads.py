@@ -5,7 +5,8 @@ container):
 * the mixed-size batch (5^3 K29, 7^3 K43, 9^3 K57 seed 1, 11^3 K57) as one plan over the full
   enumerated lists, replayed several times, against the reference's values of seeded canonical
   samples of every grid (eri_integrals.c:544-678 per grid);
-* the reference's sparse-completeness test on the GPU path (test/test_eri_integrals.c:342-419);
+* the reference's sparse-completeness, projection and apply tests on the GPU path
+  (test/test_eri_integrals.c:125-419);
 * every ERI-DA pair of the 13^3 grid (erida_integrals.c:429-542);
 * the precision study at 13^3 on the reference's sample values;
 * the complete 9^3 and 11^3 enumeration lists against the count and SHA-256 of the reference
@@ -136,6 +137,48 @@ def test_eri_sparse_completeness_5(L, orc):
 
 
 # ---------------------------------------------------------------------------
+# the reference's ERI projection and apply tests, on the GPU path
+
+def _g10_3_tol5():
+    """the sparse set both reference tests build: G10 coefficients of test_eri_integrals.hdf5, the
+    3^3 grid, tol 1e-5"""
+    c = golden("ref_eri.npz")["coeffs"]
+    return gli.compute_sparse_eri_gausslet_integrals(c, sym(3), 1e-5)
+
+
+def test_project_eri_reference(L):
+    """test_project_eri_integrals (test/test_eri_integrals.c:125-231): four states with the test's
+    pseudo-random entries 0.1 sin(sqrt(3) pi ((557 i) mod 311) - 8) / (1.5 + cos(sqrt(5) pi ((541 i)
+    mod 229))); the projection equals the dense tensor contracted with the basis on all four axes,
+    uniform distance <= 1e-12"""
+    say("project eri reference")
+    eri = _g10_3_tol5()
+    i = np.arange(27 * 4)
+    basis = (0.1 * np.sin(np.sqrt(3) * np.pi * ((557 * i) % 311) - 8) /
+             (1.5 + np.cos(np.sqrt(5) * np.pi * ((541 * i) % 229)))).reshape(27, 4)
+    proj = eri.project(basis)
+    dense = eri.fill_tensor(sym(3))
+    ref = np.einsum("ip,jq,kr,ls,ijkl->pqrs", basis, basis, basis, basis, dense, optimize=True)
+    assert np.max(np.abs(proj - ref)) <= 1e-12
+
+
+def test_apply_eri_reference(L):
+    """test_apply_eri_integrals (test/test_eri_integrals.c:233-340): three two-particle states with the
+    test's pseudo-random entries 0.12 sin(sqrt(7) pi ((523 i) mod 379) - 4) / (1.8 + cos(sqrt(11) pi
+    ((563 i) mod 281))); ret[(i,k), s] = sum_jl (ij|kl) states[(j,l), s] over the dense tensor, uniform
+    distance <= 1e-12"""
+    say("apply eri reference")
+    eri = _g10_3_tol5()
+    i = np.arange(27 * 27 * 3)
+    states = (0.12 * np.sin(np.sqrt(7) * np.pi * ((523 * i) % 379) - 4) /
+              (1.8 + np.cos(np.sqrt(11) * np.pi * ((563 * i) % 281)))).reshape(27 * 27, 3)
+    out = eri.apply(states)
+    dense = eri.fill_tensor(sym(3))
+    ref = np.einsum("ijkl,jls->iks", dense, states.reshape(27, 27, 3)).reshape(27 * 27, 3)
+    assert np.max(np.abs(out - ref)) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
 # ERI-DA at 13^3
 
 def test_erida_13_all_pairs_vs_reference(L):
@@ -217,45 +260,35 @@ def test_enumeration_13_sorted_canonical_and_complete(L, orc):
 # ---------------------------------------------------------------------------
 # nuclear matrix with many nuclei
 
-def nuclear_reference(orc, c, grid, pos, chg, tol):
-    """The oracle's matrix (the reference's loop order, nuclear_integrals.c:133-266) and a bound for
-    comparing another summation order with it.  With many nuclei an entry sums about 10^5 signed
-    terms per nucleus; the reference's own result moves when the same terms are associated
-    differently.  That spread is measured here: the oracle once over all nuclei, once per nucleus with
-    the per-nucleus matrices added up.  The bound is four times the spread (the two results differ
-    from the exact sum independently), and never below the reference tests' 1e-13."""
-    ref = orc.nuclear_matrix(c, grid, pos, chg, tol)
-    per = np.zeros_like(ref)
-    for k in range(len(chg)):
-        per += orc.nuclear_matrix(c, grid, pos[k:k + 1], chg[k:k + 1], tol)
-    spread = float(np.max(np.abs(ref - per)))
-    return ref, max(4.0 * spread, 1e-13)
+def nuclear_bound(spread):
+    """With many nuclei an entry of the nuclear matrix sums ~10^5 signed terms per nucleus, and the
+    reference's own result moves when the same terms are associated differently: the fixture holds
+    that spread of the reference build (its matrix over all nuclei against the sum of its per-nucleus
+    matrices).  Another summation order is compared within four times the spread, and never within
+    less than the reference tests' 1e-13."""
+    return max(4.0 * float(spread), 1e-13)
 
 
-def test_nuclear_many_nuclei(L, orc):
+def test_nuclear_many_nuclei(L):
     """100 nuclei (two LDS chunks of the potential kernel; the reference loops over any num_nuclei,
-    nuclear_integrals.c:234-244) on a 3x4x5 grid against the oracle, through the Python surface and
-    the struct-returning C entry point; the first 64 of them alone (one chunk) match the oracle as well,
-    and the matrix of all 100 is the sum of those of the first 64 and the last 36 (a dropped or
-    repeated chunk would be off by a whole nucleus's contribution, O(1))"""
+    nuclear_integrals.c:234-244) on a 3x4x5 grid against the reference build's matrix, through the
+    Python surface and the struct-returning C entry point; the first 64 of them alone (one chunk) as
+    well; and the matrix of all 100 equals the sum of those of the first 64 and the last 36 (a dropped
+    or repeated chunk would be off by a whole nucleus's contribution, O(1))"""
     say("nuclear 100 nuclei")
-    c = golden("coeffs_K29_s0.npy")
-    rng = np.random.default_rng(100)
-    pos = rng.uniform(-2.5, 2.5, size=(100, 3))
-    chg = rng.integers(1, 4, size=100).astype(np.int32)
-    grid = ((-1, 3), (-2, 4), (-2, 5))
-    ref, bound = nuclear_reference(orc, c, grid, pos, chg, 1e-14)
-    m = gli.compute_nuclear_gausslet_integrals(c, grid, pos, chg, 1e-14)
-    err = float(np.max(np.abs(m - ref)))
-    say("  100 nuclei: max |ref| %.3g, max abs err %.3g, bound %.3g" % (np.abs(ref).max(), err, bound))
+    g = golden("golden_nuclear_many.npz")
+    c, tol, pos, chg = g["many_coeffs"], float(g["many_tol"]), g["many_pos"], g["many_charges"]
+    grid = tuple(tuple(int(v) for v in ax) for ax in g["many_grid"])
+    m = gli.compute_nuclear_gausslet_integrals(c, grid, pos, chg, tol)
+    err, bound = float(np.max(np.abs(m - g["many_m100"]))), nuclear_bound(g["many_spread100"])
+    say("  100 nuclei: max |ref| %.3g, max abs err %.3g, bound %.3g" % (np.abs(g["many_m100"]).max(), err, bound))
     assert err <= bound
     assert np.array_equal(m, m.T)
-    ref64, bound64 = nuclear_reference(orc, c, grid, pos[:64], chg[:64], 1e-14)
-    m64 = gli.compute_nuclear_gausslet_integrals(c, grid, pos[:64], chg[:64], 1e-14)
-    err64 = float(np.max(np.abs(m64 - ref64)))
+    m64 = gli.compute_nuclear_gausslet_integrals(c, grid, pos[:64], chg[:64], tol)
+    err64, bound64 = float(np.max(np.abs(m64 - g["many_m64"]))), nuclear_bound(g["many_spread64"])
     say("  64 nuclei: max abs err %.3g, bound %.3g" % (err64, bound64))
     assert err64 <= bound64
-    m36 = gli.compute_nuclear_gausslet_integrals(c, grid, pos[64:], chg[64:], 1e-14)
+    m36 = gli.compute_nuclear_gausslet_integrals(c, grid, pos[64:], chg[64:], tol)
     assert np.max(np.abs(m - (m64 + m36))) <= bound
     gd = _abi.GaussletData()
     cc = np.ascontiguousarray(c)
@@ -268,7 +301,7 @@ def test_nuclear_many_nuclei(L, orc):
             nuc[q].pos[a] = pos[q][a]
         nuc[q].charge = int(chg[q])
     out = _abi.NuclearIntegrals()
-    L.compute_nuclear_gausslet_integrals(C.byref(gd), C.byref(_abi.Grid.of(grid)), nuc, 100, 1e-14, C.byref(out))
+    L.compute_nuclear_gausslet_integrals(C.byref(gd), C.byref(_abi.Grid.of(grid)), nuc, 100, tol, C.byref(out))
     assert out.num_nuclei == 100
     assert np.array_equal(np.ctypeslib.as_array(out.integral_values, shape=m.shape), m)
     L.delete_nuclear_gausslet_integrals(C.byref(out))
@@ -304,22 +337,21 @@ def test_project_chunked_bit_identical(L):
 # ---------------------------------------------------------------------------
 # a C caller of the reference API
 
-def test_c_caller(L, orc, tmp_path):
+def test_c_caller(L, tmp_path):
     """tests/c/gli_caller (plain C11, compiled against include/ and linked with -lgli_amd) runs the
     reference entry points on the 3^3 grid: its ERI list and values equal the reference build's
     golden set, lookups return the values, single-element ERIs agree with the sparse ones, the
     ERI-DA set and kinetic matrix match the golden set, and the nuclear matrix of 70 nuclei matches
-    the oracle within the bound of nuclear_reference"""
+    the reference build's within the bound of nuclear_bound"""
     say("C caller")
     exe = os.path.join(REPO, "tests", "c", "gli_caller")
     if not os.path.exists(exe):
         pytest.fail("tests/c/gli_caller is not built (python -c 'import __graft_entry__ as g; g.build()')")
     g = golden("golden_n3_K57s0.npz")
+    gn = golden("golden_nuclear_many.npz")
     c = np.ascontiguousarray(g["coeffs"], dtype="<f8")
-    rng = np.random.default_rng(70)
-    nuc = np.zeros((70, 4))
-    nuc[:, :3] = rng.uniform(-1.5, 1.5, size=(70, 3))
-    nuc[:, 3] = rng.integers(1, 3, size=70)
+    assert np.array_equal(c, gn["caller_coeffs"])
+    nuc = gn["caller_nuclei"]
     (tmp_path / "c.f64").write_bytes(c.tobytes())
     (tmp_path / "n.f64").write_bytes(np.ascontiguousarray(nuc, dtype="<f8").tobytes())
     out = tmp_path / "out"
@@ -344,6 +376,6 @@ def test_c_caller(L, orc, tmp_path):
     assert np.max(np.abs(rd("erida_values.f64", "<f8") - g["erida_values"])) <= 1e-13
     assert np.max(np.abs(rd("kinetic.f64", "<f8").reshape(27, 27) - g["kinetic"])) <= 1e-13
     nm = rd("nuclear.f64", "<f8").reshape(27, 27)
-    ref, bound = nuclear_reference(orc, c, sym(3), nuc[:, :3], nuc[:, 3].astype(np.int32), 1e-16)
-    assert np.max(np.abs(nm - ref)) <= bound
+    bound = nuclear_bound(gn["caller_spread70"])
+    assert np.max(np.abs(nm - gn["caller_m70"])) <= bound
     assert abs(rd("nuclear_single.f64", "<f8")[0] - nm[0, 26]) <= bound

@@ -413,3 +413,21 @@ def test_one_body_golden(orc):
     gs = orc.sym_grid(3)
     assert np.array_equal(orc.kinetic_matrix(g["coeffs"], gs), g["kinetic"])
     assert np.array_equal(orc.nuclear_matrix(g["coeffs"], gs, [[0, 0, 0]], [1], float(g["tol"])), g["nuclear"])
+
+
+def test_nuclear_many_nuclei_golden(orc):
+    """the oracle's single-entry nuclear integral (nuclear_integrals.c:31-126) for 100 seeded nuclei on a
+    3x4x5 grid at seeded entries, against the reference build's matrix within four times the reference's
+    own spread under re-association (stored with the fixture)"""
+    g = golden("golden_nuclear_many.npz")
+    grid = [tuple(int(v) for v in ax) for ax in g["many_grid"]]
+    n = [ax[1] for ax in grid]
+    bound = max(4.0 * float(g["many_spread100"]), 1e-13)
+    rng = np.random.default_rng(5)
+    for I, J in rng.integers(0, n[0] * n[1] * n[2], size=(6, 2)):
+        pts = []
+        for p in (I, J):
+            x, y, z = p // (n[1] * n[2]), (p // n[2]) % n[1], p % n[2]
+            pts.append([grid[0][0] + x, grid[1][0] + y, grid[2][0] + z])
+        v = orc.nuclear_single(g["many_coeffs"], pts, g["many_pos"], g["many_charges"], float(g["many_tol"]))
+        assert abs(v - g["many_m100"][I, J]) <= bound, (I, J)
