@@ -60,9 +60,12 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 // the identity image of the unshifted tuple is t itself and is skipped (element 0 is the
 // identity).  W[g] holds the signed weights of the original axes in the image's linear
 // index, lin(g q) = W[g] . q + H, so an image point costs three 24-bit multiply-adds.
+//
+// mc[q] packs minimg[q] (bits 48-63) and canon[q] (bits 0-47), so a placement's four points
+// cost four loads and no branch.  The placements are the subsets of the axes whose box sum c
+// is odd (c = +-1): one loop over those subsets, the empty one (no shift) first.
 __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], const int lp[4], int i, int IJ, int KL,
-                                 const int* __restrict__ minimg, const unsigned long long* __restrict__ canon,
-                                 const int4* __restrict__ W)
+                                 const unsigned long long* __restrict__ mc, const int4* __restrict__ W)
 {
 	const int n2 = g.n * g.n, H = g.h * (n2 + g.n + 1);
 	int c[3];
@@ -71,40 +74,39 @@ __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], const int
 		const int mx = max(max(P[0][a], P[1][a]), max(P[2][a], P[3][a]));
 		c[a] = mn + mx;
 	}
-	const int cx = __mul24(c[0], n2), cy = __mul24(c[1], g.n), cz = c[2];
-	for (int bx = 0; bx <= abs(c[0]); ++bx) {
-		for (int by = 0; by <= abs(c[1]); ++by) {
-			for (int bz = 0; bz <= abs(c[2]); ++bz) {
-				// placement P - b*c: every point's linear index moves by the same offset
-				const int off = (bx ? cx : 0) + (by ? cy : 0) + (bz ? cz : 0);
-				unsigned long long emask = 0;
-				for (int p = 0; p < 4; ++p) {
-					const int lq = lp[p] - off;
-					const int mi = minimg[lq];
-					if (mi < i) { return false; }
-					if (mi == i) { emask |= canon[lq]; }
-				}
-				if ((bx | by | bz) == 0) { emask &= ~1ull; }
-				if (emask == 0ull) { continue; }
-				int Q[4][3];
-				for (int p = 0; p < 4; ++p) {
-					Q[p][0] = P[p][0] - (bx ? c[0] : 0);
-					Q[p][1] = P[p][1] - (by ? c[1] : 0);
-					Q[p][2] = P[p][2] - (bz ? c[2] : 0);
-				}
-				for (; emask; emask &= emask - 1) {
-					const int4 w = W[__builtin_ctzll(emask)];
-					int L[4];
-					for (int r = 0; r < 4; ++r) { L[r] = __mul24(w.x, Q[r][0]) + __mul24(w.y, Q[r][1]) + __mul24(w.z, Q[r][2]) + H; }
-					const int a = min(L[0], L[1]), b = max(L[0], L[1]);
-					const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
-					int ab = __mul24(a, g.N) + b, cd = __mul24(cc, g.N) + d;
-					if (ab > cd) { const int t = ab; ab = cd; cd = t; }
-					if (ab < IJ || (ab == IJ && cd < KL)) { return false; }
-				}
+	const int dx = __mul24(c[0], n2), dy = __mul24(c[1], g.n), dz = c[2];
+	const unsigned act = (c[0] != 0 ? 1u : 0u) | (c[1] != 0 ? 2u : 0u) | (c[2] != 0 ? 4u : 0u);
+	unsigned b = 0;
+	do {
+		// placement P - b*c: every point's linear index moves by the same offset
+		const int off = ((b & 1u) ? dx : 0) + ((b & 2u) ? dy : 0) + ((b & 4u) ? dz : 0);
+		unsigned long long m[4];
+		for (int p = 0; p < 4; ++p) { m[p] = mc[lp[p] - off]; }
+		const int mi0 = (int)(m[0] >> 48), mi1 = (int)(m[1] >> 48), mi2 = (int)(m[2] >> 48), mi3 = (int)(m[3] >> 48);
+		if (min(min(mi0, mi1), min(mi2, mi3)) < i) { return false; }
+		unsigned long long emask = ((mi0 == i) ? m[0] : 0ull) | ((mi1 == i) ? m[1] : 0ull) | ((mi2 == i) ? m[2] : 0ull) |
+		                           ((mi3 == i) ? m[3] : 0ull);
+		emask &= (b == 0) ? 0xfffffffffffeull : 0xffffffffffffull;
+		if (emask != 0ull) {
+			int Q[4][3];
+			for (int p = 0; p < 4; ++p) {
+				Q[p][0] = P[p][0] - ((b & 1u) ? c[0] : 0);
+				Q[p][1] = P[p][1] - ((b & 2u) ? c[1] : 0);
+				Q[p][2] = P[p][2] - ((b & 4u) ? c[2] : 0);
+			}
+			for (; emask; emask &= emask - 1) {
+				const int4 w = W[__builtin_ctzll(emask)];
+				int L[4];
+				for (int r = 0; r < 4; ++r) { L[r] = __mul24(w.x, Q[r][0]) + __mul24(w.y, Q[r][1]) + __mul24(w.z, Q[r][2]) + H; }
+				const int a = min(L[0], L[1]), bb = max(L[0], L[1]);
+				const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
+				int ab = __mul24(a, g.N) + bb, cd = __mul24(cc, g.N) + d;
+				if (ab > cd) { const int t = ab; ab = cd; cd = t; }
+				if (ab < IJ || (ab == IJ && cd < KL)) { return false; }
 			}
 		}
-	}
+		b = (b - act) & act;   // next subset of the odd axes
+	} while (b != 0u);
 	return true;
 }
 
@@ -149,6 +151,10 @@ __device__ inline uint32_t udiv_small(uint32_t v, uint32_t d)
 	return q;
 }
 
+// row index of every 64th candidate, for items of up to 64 * ENUM_ROW_AT candidates (every item of a
+// 13^3 grid: 64 rows of at most 2197 candidates)
+constexpr int ENUM_ROW_AT = 4096;
+
 // the rows of one work item in LDS
 struct EnumRows {
 	uint32_t pre[ENUM_ROWS + 1];   // exclusive prefix of the rows' candidate counts
@@ -156,6 +162,7 @@ struct EnumRows {
 	uint32_t ks[ENUM_ROWS];        // k (bits 0-15) and the candidates skipped below lmin (bits 16-31)
 	uint32_t kc[ENUM_ROWS];        // k's coordinates + h, 5 bits per axis
 	uint32_t wsum[ENUM_T / 64];
+	uint8_t row_at[ENUM_ROW_AT];   // row of candidate 64 w (when the item has at most 64 * ENUM_ROW_AT candidates)
 };
 
 struct EnumItem {
@@ -238,7 +245,13 @@ __device__ uint32_t enum_load_rows(const EnumGrid& g, const EnumItem& it, EnumRo
 	}
 	uint32_t total;
 	const uint32_t before = enum_block_scan(c, s.wsum, total);
-	if (r < it.rows) { s.pre[r] = before; }
+	if (r < it.rows) {
+		s.pre[r] = before;
+		// the 64-candidate word starts inside this row
+		if (total <= 64u * ENUM_ROW_AT) {
+			for (uint32_t w = (before + 63) / 64; w * 64 < before + c; ++w) { s.row_at[w] = (uint8_t)r; }
+		}
+	}
 	__syncthreads();
 	return total;
 }
@@ -247,6 +260,21 @@ __device__ uint32_t enum_load_rows(const EnumGrid& g, const EnumItem& it, EnumRo
 __device__ inline int enum_row(const EnumRows& s, int rows, uint32_t q)
 {
 	int lo = 0, hi = rows;
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (s.pre[mid] <= q) { lo = mid; }
+		else { hi = mid; }
+	}
+	return lo;
+}
+
+// the same from the word index: the row lies between the rows of the word's first candidate and of
+// the next word's first one (usually equal or adjacent), for items of up to 64 * ENUM_ROW_AT candidates
+__device__ inline int enum_row_indexed(const EnumRows& s, int rows, uint32_t total, uint32_t q)
+{
+	const uint32_t w = q >> 6;
+	int lo = s.row_at[w];
+	int hi = ((w + 1) * 64 < total) ? (int)s.row_at[w + 1] + 1 : rows;
 	while (hi - lo > 1) {
 		const int mid = (lo + hi) >> 1;
 		if (s.pre[mid] <= q) { lo = mid; }
@@ -284,8 +312,8 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_item_words(EnumGrid g, const int
 // pass 2: test every candidate once; one bit per candidate, accepted count per item
 __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __restrict__ items, const int2* __restrict__ units,
 	const int* __restrict__ unit_icand, const int* __restrict__ klist, const int* __restrict__ kstart, const int* __restrict__ klen,
-	const int* __restrict__ minimg, const unsigned long long* __restrict__ canon, const int64_t* __restrict__ word_base,
-	unsigned long long* __restrict__ bits, uint32_t* __restrict__ accepted)
+	const unsigned long long* __restrict__ mc, const int64_t* __restrict__ word_base, unsigned long long* __restrict__ bits,
+	uint32_t* __restrict__ accepted)
 {
 	__shared__ EnumRows s;
 	__shared__ int4 W[48];
@@ -301,6 +329,7 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __r
 	const int lane = threadIdx.x & 63;
 	const int64_t word = word_base[blockIdx.x];
 	const uint32_t tot = enum_load_rows(g, it, s);   // its barriers also publish W
+	const bool indexed = tot <= 64u * ENUM_ROW_AT;
 	uint32_t acc = 0;
 	int P[4][3];
 	for (int a = 0; a < 3; ++a) { P[0][a] = it.pi[a]; P[1][a] = it.pj[a]; }
@@ -309,12 +338,13 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __r
 		bool keep = false;
 		if (q < tot) {
 			int k, L[3];
-			enum_decode(g, s, enum_row(s, it.rows, q), q, k, P[2], L);
+			const int r = indexed ? enum_row_indexed(s, it.rows, tot, q) : enum_row(s, it.rows, q);
+			enum_decode(g, s, r, q, k, P[2], L);
 			const int l = __mul24(__mul24(L[0] + g.h, g.n) + L[1] + g.h, g.n) + L[2] + g.h;
-			if (minimg[l] >= it.i) {
+			if ((int)(mc[l] >> 48) >= it.i) {
 				P[3][0] = L[0]; P[3][1] = L[1]; P[3][2] = L[2];
 				const int lp[4] = { it.i, it.j, k, l };
-				keep = eri_is_canonical(g, P, lp, it.i, it.IJ, __mul24(k, g.N) + l, minimg, canon, W);
+				keep = eri_is_canonical(g, P, lp, it.i, it.IJ, __mul24(k, g.N) + l, mc, W);
 			}
 		}
 		const unsigned long long m = __ballot(keep);
@@ -354,7 +384,8 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_expand(EnumGrid g, const int2* _
 		uint32_t total;
 		uint32_t rank = enum_block_scan((uint32_t)__popcll(m), s.wsum, total);
 		// the bits of one word are 64 consecutive candidates: one search, then forward steps
-		int r = (m != 0ull) ? enum_row(s, it.rows, w * 64 + (uint32_t)__builtin_ctzll(m)) : 0;
+		const uint32_t q1 = w * 64 + (uint32_t)__builtin_ctzll(m);
+		int r = (m == 0ull) ? 0 : (tot <= 64u * ENUM_ROW_AT) ? enum_row_indexed(s, it.rows, tot, q1) : enum_row(s, it.rows, q1);
 		for (; m; m &= m - 1) {
 			const uint32_t q = w * 64 + (uint32_t)__builtin_ctzll(m);
 			while (r + 1 < it.rows && s.pre[r + 1] <= q) { ++r; }
@@ -500,16 +531,21 @@ static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 	EnumPlan p;
 	plan_enumeration((int)n, p);
 	DevBuf<int2> units(p.units.size());
-	DevBuf<int> unit_ic(p.unit_icand.size()), klist(p.klist.size()), kstart(p.kstart.size()), klen(p.klen.size()), minimg(p.minimg.size());
+	DevBuf<int> unit_ic(p.unit_icand.size()), klist(p.klist.size()), kstart(p.kstart.size()), klen(p.klen.size());
 	hipStream_t st = c.stream;
 	GLI_HIP(hipMemcpyAsync(units.p, p.units.data(), p.units.size() * sizeof(int2), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(unit_ic.p, p.unit_icand.data(), p.unit_icand.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(klist.p, p.klist.data(), p.klist.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(kstart.p, p.kstart.data(), p.kstart.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(klen.p, p.klen.data(), p.klen.size() * sizeof(int), hipMemcpyHostToDevice, st));
-	GLI_HIP(hipMemcpyAsync(minimg.p, p.minimg.data(), p.minimg.size() * sizeof(int), hipMemcpyHostToDevice, st));
-	DevBuf<unsigned long long> canon(p.canon.size());
-	GLI_HIP(hipMemcpyAsync(canon.p, p.canon.data(), p.canon.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
+	// minimg (bits 48-63) and canon (bits 0-47) of every point in one table for the candidate test
+	std::vector<unsigned long long> hmc(p.canon.size());
+	for (size_t q = 0; q < hmc.size(); ++q) {
+		GLI_REQUIRE((p.canon[q] >> 48) == 0 && p.minimg[q] >= 0 && p.minimg[q] < 65536, "point tables do not fit the packed form");
+		hmc[q] = p.canon[q] | ((unsigned long long)p.minimg[q] << 48);
+	}
+	DevBuf<unsigned long long> mc(hmc.size());
+	GLI_HIP(hipMemcpyAsync(mc.p, hmc.data(), hmc.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
 	// work items: blocks of ENUM_ROWS rows of each unit, in (unit, row) order
 	std::vector<int2> hitems;
 	for (size_t u = 0; u < p.units.size(); ++u) {
@@ -529,8 +565,8 @@ static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 		GLI_HIP(hipGetLastError());
 		const int64_t nwords = exclusive_scan<int64_t>(words.p, ni, word_base.p);
 		DevBuf<unsigned long long> bits((size_t)std::max<int64_t>(nwords, 1));
-		k_eri_test<<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, minimg.p, canon.p, word_base.p,
-		                                  bits.p, accepted.p);
+		k_eri_test<<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, mc.p, word_base.p, bits.p,
+		                                  accepted.p);
 		GLI_HIP(hipGetLastError());
 		total = exclusive_scan<uint32_t>(accepted.p, ni, out_base.p);
 		GLI_HIP(hipMalloc(&out, (size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));
