@@ -42,9 +42,13 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // 1D tables
 
 // f[s][m] = (sqrt(pi)/3) sum_{a+b=m} c_a c_b exp(-(0.5 (3 s + a - b))^2), s = 0 .. R-1
-// (gausslet_factors.c:12-50; f_{-s} = f_s).  Same ascending-a summation order.
+// (gausslet_factors.c:12-50; f_{-s} = f_s).  Same ascending-a summation order.  The K coefficients
+// are staged in LDS (dynamic shared memory, K doubles) with one coalesced read per workgroup.
 __global__ void k_factors(const double* __restrict__ c, int K, double* __restrict__ f)
 {
+	extern __shared__ double cs[];
+	for (int q = threadIdx.x; q < K; q += blockDim.x) { cs[q] = c[q]; }
+	__syncthreads();
 	const int L = 2 * K - 1;
 	const int s = blockIdx.y;
 	const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,23 +59,30 @@ __global__ void k_factors(const double* __restrict__ c, int K, double* __restric
 	for (int i = ilo; i <= ihi; ++i) {
 		const int j = m - i;
 		const double x = 0.5 * (3 * s + (i - j));
-		acc += prefac * c[i] * c[j] * exp(-x * x);
+		acc += prefac * cs[i] * cs[j] * exp(-x * x);
 	}
 	f[(int64_t)s * L + m] = acc;
 }
 
 // P[a][b][t] = sum_m f_a[m] f_b[m - (t - (2K-2))], zeroed where |P| <= tol
-// (eri_integrals.c:59-103 with the tolerance skips of 629/637/658 folded in)
+// (eri_integrals.c:59-103 with the tolerance skips of 629/637/658 folded in).  The two factor rows
+// f_a and f_b are staged in LDS (dynamic shared memory, 2 (2K - 1) doubles).
 __global__ void k_products(const double* __restrict__ f, int K, int R, double tol, double* __restrict__ P)
 {
+	extern __shared__ double fs[];
 	const int L = 2 * K - 1, LP = 4 * K - 3;
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int ab = blockIdx.y;
-	if (t >= LP) { return; }
 	const int a = ab / R, b = ab % R;
+	double* fa = fs;
+	double* fb = fs + L;
+	for (int q = threadIdx.x; q < L; q += blockDim.x) {
+		fa[q] = f[(int64_t)a * L + q];
+		fb[q] = f[(int64_t)b * L + q];
+	}
+	__syncthreads();
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= LP) { return; }
 	const int d = t - (2 * K - 2);  // m - m'
-	const double* fa = f + (int64_t)a * L;
-	const double* fb = f + (int64_t)b * L;
 	double acc = 0;
 	const int mlo = max(0, d), mhi = min(L - 1, L - 1 + d);
 	for (int m = mlo; m <= mhi; ++m) { acc += fa[m] * fb[m - d]; }
@@ -1392,8 +1403,8 @@ static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol,
 	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
 	hipStream_t st = ctx().stream;
 	const int L = 2 * s.K - 1, LP = 4 * s.K - 3;
-	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, 0, st>>>(d_coeffs, s.K, f);
-	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, 0, st>>>(f, s.K, s.R, tol, P);
+	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, (size_t)s.K * sizeof(double), st>>>(d_coeffs, s.K, f);
+	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, (size_t)2 * L * sizeof(double), st>>>(f, s.K, s.R, tol, P);
 	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, s.Cp, B);
 	GLI_HIP(hipGetLastError());
 }

@@ -73,9 +73,12 @@ __global__ void k_nuclear_potential(const double4* __restrict__ nuc, int num, in
 }
 
 // f'[s][m] for s = 0..S-1, zeroed where |f| <= tol (gausslet_factors.c:12-77 with the skips of
-// nuclear_integrals.c:210-228 folded in)
+// nuclear_integrals.c:210-228 folded in); the K coefficients staged in LDS (dynamic shared memory)
 __global__ void k_factors_tol(const double* __restrict__ c, int K, double tol, double* __restrict__ f)
 {
+	extern __shared__ double cs[];
+	for (int q = threadIdx.x; q < K; q += blockDim.x) { cs[q] = c[q]; }
+	__syncthreads();
 	const int L = 2 * K - 1;
 	const int s = blockIdx.y;
 	const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,7 +89,7 @@ __global__ void k_factors_tol(const double* __restrict__ c, int K, double tol, d
 	for (int i = ilo; i <= ihi; ++i) {
 		const int j = m - i;
 		const double x = 0.5 * (3 * s + (i - j));
-		acc += prefac * c[i] * c[j] * exp(-x * x);
+		acc += prefac * cs[i] * cs[j] * exp(-x * x);
 	}
 	f[(int64_t)s * L + m] = (fabs(acc) > tol) ? acc : 0.0;
 }
@@ -139,6 +142,7 @@ static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	GLI_REQUIRE(K >= 3 && (K % 2) == 1, "the number of gausslet coefficients must be odd and at least 3");
+	GLI_REQUIRE(K <= 4095, "at most 4095 gausslet coefficients (the factor kernel stages them in LDS)");
 	GLI_REQUIRE(g[1] > 0 && g[3] > 0 && g[5] > 0, "grid axes must have a positive number of points");
 	GLI_REQUIRE(num_nuclei >= 0, "the number of nuclei must not be negative");
 	GLI_REQUIRE(num_nuclei == 0 || (pos != nullptr && charges != nullptr), "nuclei without positions or charges");
@@ -182,7 +186,7 @@ static void nuclear_matrix(const double* coeffs, int64_t K, const int64_t g[6], 
 	M.reserve((size_t)(N * N));
 	GLI_HIP(hipMemcpyAsync(dc.p, coeffs, K * sizeof(double), hipMemcpyHostToDevice, st));
 	upload_async(w.nuc.p, w.h_nuc.data(), (size_t)num_nuclei * sizeof(double4));
-	k_factors_tol<<<dim3((unsigned)ceil_div(L, 128), R), 128, 0, st>>>(dc.p, (int)K, tol, f.p);
+	k_factors_tol<<<dim3((unsigned)ceil_div(L, 128), R), 128, (size_t)K * sizeof(double), st>>>(dc.p, (int)K, tol, f.p);
 	k_nuclear_potential<<<dim3((unsigned)ceil_div(U[2], 64), U[1], U[0]), 64, 0, st>>>(w.nuc.p, num_nuclei, u0[0], u0[1], u0[2],
 	                                                                                    U[0], U[1], U[2], W.p);
 	double* Fs[3] = { F0.p, F1.p, F2.p };
