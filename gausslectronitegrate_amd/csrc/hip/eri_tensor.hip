@@ -557,12 +557,7 @@ __device__ inline int64_t gather_index(const GridDev& g, const DecodeConsts& dc,
 	return tet_index(x, y, z);
 }
 
-// values[e] = V[k1][k2][k3] from the tetrahedral key tensor.  Each wave takes GATHER_EPT groups
-// of 64 consecutive elements (coalesced index loads and value stores, several dependent load
-// chains index -> key table -> V in flight per lane).  In a sorted list the leading pair (i, j)
-// changes only every few thousand elements, so it is decoded once per wave; a group whose lanes
-// all share it decodes only (k, l) per lane with exact reciprocal multiplies.  Any other group
-// (a pair boundary, the tail, unsorted or out-of-range input) takes the general per-lane decode.
+// the value of lane 0 in every lane (scalar registers)
 __device__ inline int64_t uniform_i64(int64_t v)
 {
 	const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)v);
@@ -608,16 +603,24 @@ __host__ __device__ inline int gather_lds_bytes(int nlut, int C1)
 }
 constexpr int kGatherLdsMax = 48 * 1024;
 
-// lut and tet3 point to the descriptor's tables in global memory, or to their copies in LDS
-// kPos: write each element's key-tensor position to d.pos instead of reading V (the same decode)
-template <int GATHER_EPT, bool kPos = false>
-__device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const short* __restrict__ lut,
-                                   const uint32_t* __restrict__ tet3)
+// Key-tensor positions of one wave's GATHER_EPT groups of 64 consecutive elements (-1 past the end of
+// the list and for indices outside the grid).  lut and tet3 point to the descriptor's tables in global
+// memory, or to their copies in LDS.  In a sorted list the leading pair (i, j) changes only every few
+// thousand elements, so it is decoded once per wave; when every group of the wave shares it, each lane
+// decodes only (k, l), with exact reciprocal multiplies, and the loads are phased across the groups
+// (all key-table reads, then all tetrahedral offsets) so that the in-order load counter never makes
+// one group's reads wait for another's.  Any other wave (a pair boundary, the tail, unsorted or
+// out-of-range input, grids without 32-bit tetrahedral indices) takes the general per-lane decode.
+// The positions go to one of two consumers inside the decode's branch: fast(uint32_t[GATHER_EPT])
+// when the whole wave took the fast decode (every position valid and below 2^32), general(int64_t
+// [GATHER_EPT]) otherwise (-1 past the end of the list and for indices outside the grid).
+template <int GATHER_EPT, typename Fast, typename General>
+__device__ inline void wave_positions(const GatherDesc& d, int64_t wave_base, const short* __restrict__ lut,
+                                      const uint32_t* __restrict__ tet3, Fast&& fast_use, General&& general_use)
 {
 	const int lane = threadIdx.x & 63;
 	const GridDev& g = d.g;
 	const DecodeConsts& dc = d.dc;
-	const double* __restrict__ V = d.V;
 	int64_t t[GATHER_EPT];
 #pragma unroll
 	for (int u = 0; u < GATHER_EPT; ++u) {
@@ -646,10 +649,7 @@ __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const
 	bool fast = pair_base >= 0;
 #pragma unroll
 	for (int u = 0; u < GATHER_EPT; ++u) { fast = fast && (uint64_t)(t[u] - pair_base) < (uint64_t)g.N2; }
-	double v[GATHER_EPT];
 	if (__all(fast)) {
-		// all groups of the wave share the pair: phase the loads across groups so that the
-		// in-order load counter never makes one group's V read wait for another's key reads
 		const unsigned short* __restrict__ ulut = (const unsigned short*)lut;
 		unsigned kk[GATHER_EPT][3];
 #pragma unroll
@@ -662,12 +662,15 @@ __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const
 			point_xyz(l, g, dc, cl[0], cl[1], cl[2]);
 #pragma unroll
 			for (int a = 0; a < 3; ++a) {
-				// (|ci - cj| R + |ck - cl|) W + (ci + cj) - (ck + cl) + 2(R - 1), with the pair part uniform
-				const unsigned li = __umul24(__usad(ck[a], cl[a], 0u), (unsigned)W) + ((unsigned)base_key[a] - ck[a] - cl[a]);
-				kk[u][a] = ulut[li];
+				// (|ci - cj| R + |ck - cl|) W + (ci + cj) - (ck + cl) + 2(R - 1): the lane part, then the
+				// wave-uniform pair part as a scalar base of the table address.  The lane part is negative
+				// when ck = cl, so it stays a signed int: an unsigned one would be zero-extended into a
+				// 64-bit address when the table is in global memory
+				const int lane_part = (int)__umul24(__usad(ck[a], cl[a], 0u), (unsigned)W) - (int)(ck[a] + cl[a]);
+				kk[u][a] = (ulut + base_key[a])[lane_part];
 			}
 		}
-		uint32_t vi[GATHER_EPT];
+		uint32_t v32[GATHER_EPT];
 		unsigned k3[GATHER_EPT];
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) {
@@ -676,36 +679,39 @@ __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const
 			k3[u] = max(hi, kk[u][2]);
 			const unsigned k1 = min(lo, mid), k2 = max(lo, mid);
 			// tet_index without its k3 block offset (added from the table below); k1 (k3 + 1) < 2^32 for C1 < 2^15
-			vi[u] = __umul24(k1, k3[u] + 1u) - (__umul24(k1, k1 - 1u) >> 1) + (k2 - k1);
+			v32[u] = __umul24(k1, k3[u] + 1u) - (__umul24(k1, k1 - 1u) >> 1) + (k2 - k1);
 		}
 #pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] += tet3[k3[u]]; }
-		if (kPos) {
-#pragma unroll
-			for (int u = 0; u < GATHER_EPT; ++u) {
-				const int64_t e = wave_base + u * 64 + lane;
-				if (e < d.num) { d.pos[e] = vi[u]; }
-			}
-			return;
-		}
-#pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = V[vi[u]]; }
+		for (int u = 0; u < GATHER_EPT; ++u) { v32[u] += tet3[k3[u]]; }
+		fast_use(v32);
 	}
 	else {
 		int64_t vi[GATHER_EPT];
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { vi[u] = (t[u] >= 0 && t[u] < g.N4) ? gather_index(g, dc, lut, t[u]) : -1; }
-		if (kPos) {
-#pragma unroll
-			for (int u = 0; u < GATHER_EPT; ++u) {
-				const int64_t e = wave_base + u * 64 + lane;
-				if (e < d.num) { d.pos[e] = (vi[u] >= 0) ? (uint32_t)vi[u] : kNoPosition; }
-			}
-			return;
-		}
-#pragma unroll
-		for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
+		general_use(vi);
 	}
+}
+
+// values[e] = V[k1][k2][k3] from the tetrahedral key tensor for one wave's GATHER_EPT groups of 64
+// consecutive elements (coalesced index loads and value stores, several dependent load chains
+// index -> key table -> V in flight per lane); NaN for indices outside the grid
+template <int GATHER_EPT>
+__device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const short* __restrict__ lut,
+                                   const uint32_t* __restrict__ tet3)
+{
+	const int lane = threadIdx.x & 63;
+	const double* __restrict__ V = d.V;
+	double v[GATHER_EPT];
+	wave_positions<GATHER_EPT>(d, wave_base, lut, tet3,
+		[&](const uint32_t* vi) {
+#pragma unroll
+			for (int u = 0; u < GATHER_EPT; ++u) { v[u] = V[vi[u]]; }
+		},
+		[&](const int64_t* vi) {
+#pragma unroll
+			for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
+		});
 	if (d.scale != nullptr) {
 		const double sc = *d.scale;
 #pragma unroll
@@ -730,7 +736,7 @@ __device__ inline int64_t gather_piece(int64_t xcd_chunks)
 // With kLds the workgroup first copies the key table and the tetrahedral offsets into LDS: their
 // reads then no longer compete with the V reads for the vector L1's miss path (the bound of this
 // kernel); the copy pays off from 2048 elements per workgroup on.
-template <int GATHER_EPT, bool kLds, bool kPos = false>
+template <int GATHER_EPT, bool kLds>
 __device__ inline void gather_block(const GatherDesc& d, int64_t wave_base)
 {
 	extern __shared__ uint32_t gather_lds[];
@@ -746,7 +752,7 @@ __device__ inline void gather_block(const GatherDesc& d, int64_t wave_base)
 		tet3 = st;
 	}
 	if (wave_base >= d.num) { return; }
-	gather_wave<GATHER_EPT, kPos>(d, wave_base, lut, tet3);
+	gather_wave<GATHER_EPT>(d, wave_base, lut, tet3);
 }
 
 template <int GATHER_EPT, bool kLds>
@@ -759,13 +765,35 @@ __global__ __launch_bounds__(1024) void k_gather(GatherDesc d, int64_t xcd_chunk
 	gather_block<GATHER_EPT, kLds>(d, wave_base);
 }
 
-// all grids of a batch in one launch: descriptors in device memory, pieces concatenated
-// key-tensor positions of one list (the gather's decode, no V reads)
+// key-tensor positions of one list (the gather's decode, no V reads; kNoPosition for indices
+// outside the grid), with the key tables in LDS
 __global__ __launch_bounds__(256) void k_positions(GatherDesc d)
 {
+	extern __shared__ uint32_t gather_lds[];
 	const int64_t wave_base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * 8);
 	if ((int64_t)blockIdx.x * (blockDim.x >> 6) * (64 * 8) >= d.num) { return; }   // whole workgroup idle
-	gather_block<8, true, true>(d, wave_base);
+	uint32_t* st = gather_lds;
+	short* sl = (short*)(gather_lds + ((d.C1 + 3) & ~3));
+	for (int q = threadIdx.x; q < d.C1; q += blockDim.x) { st[q] = d.tet3[q]; }
+	for (int q = threadIdx.x; q < d.nlut; q += blockDim.x) { sl[q] = d.lut[q]; }
+	__syncthreads();
+	if (wave_base >= d.num) { return; }
+	const int lane = threadIdx.x & 63;
+	uint32_t pos[8];
+	wave_positions<8>(d, wave_base, sl, st,
+		[&](const uint32_t* vi) {
+#pragma unroll
+			for (int u = 0; u < 8; ++u) { pos[u] = vi[u]; }
+		},
+		[&](const int64_t* vi) {
+#pragma unroll
+			for (int u = 0; u < 8; ++u) { pos[u] = (vi[u] >= 0) ? (uint32_t)vi[u] : kNoPosition; }
+		});
+#pragma unroll
+	for (int u = 0; u < 8; ++u) {
+		const int64_t e = wave_base + u * 64 + lane;
+		if (e < d.num) { d.pos[e] = pos[u]; }
+	}
 }
 
 // values from cached positions: per element a 4-byte position read, the V read, the scale and the
@@ -893,12 +921,12 @@ __global__ __launch_bounds__(256) void k_gather_grouped(const GatherDesc* __rest
 // In list order, consecutive elements change their z key almost every time, so the 64 lanes of a
 // load read 64 scattered key-tensor entries and touch about 45 distinct 128-byte lines: the L1 miss
 // path, not HBM, bounds the plain gather.  Here a workgroup of 1024 lanes takes a window of 8192
-// consecutive elements, decodes every element's key-tensor position (the decode of gather_wave),
+// consecutive elements, decodes every element's key-tensor position (wave_positions, as k_gather),
 // and counting-sorts the window by bucket = position >> shift in LDS: one LDS atomic per element
 // gives its rank inside its bucket, a scan of the bucket counts gives the bucket starts.  Reading
 // the key tensor in bucket order puts the elements that share a line into the same 64-lane load.
 // The values return to their list places through LDS and leave in coalesced stores.  Each value is
-// the same key-tensor entry gather_wave reads, so the results are bit-identical.
+// the same key-tensor entry k_gather reads, so the results are bit-identical.
 constexpr int kBucketEpt = 8;           // elements per lane: each wave takes 8 groups of 64 consecutive elements
 constexpr int kBucketMaxThreads = 1024;
 
@@ -908,79 +936,6 @@ constexpr int kBucketMaxThreads = 1024;
 __host__ __device__ constexpr int bucket_lds_bytes(int threads)
 {
 	return threads * kBucketEpt * (int)sizeof(double);
-}
-
-// key-tensor positions of one wave's kBucketEpt groups of 64 consecutive elements, kNoPosition past
-// the end of the list and for indices outside the grid (needs dc.uniform_pairs: tetrahedral indices
-// below 2^32); the decode of gather_wave
-__device__ inline void wave_positions(const GatherDesc& d, int64_t wave_base, const short* __restrict__ lut,
-                                      const uint32_t* __restrict__ tet3, uint32_t vi[kBucketEpt])
-{
-	const int lane = threadIdx.x & 63;
-	const GridDev& g = d.g;
-	const DecodeConsts& dc = d.dc;
-	int64_t t[kBucketEpt];
-#pragma unroll
-	for (int u = 0; u < kBucketEpt; ++u) {
-		const int64_t e = wave_base + u * 64 + lane;
-		t[u] = (e < d.num) ? d.idx[e] : -1;
-	}
-	const int W = 4 * g.R - 3, off = 2 * (g.R - 1);
-	const int64_t t0 = uniform_i64(t[0]);
-	int64_t pair_base = -1;
-	int base_key[3] = { 0, 0, 0 };
-	if (t0 >= 0 && t0 < g.N4) {
-		const int64_t ij = (int64_t)((double)t0 * dc.inv_N2_up);
-		const unsigned i = (unsigned)((double)(unsigned)ij * dc.inv_N_up);
-		const unsigned j = (unsigned)ij - __umul24(i, (unsigned)g.N);
-		unsigned ci[3], cj[3];
-		point_xyz(i, g, dc, ci[0], ci[1], ci[2]);
-		point_xyz(j, g, dc, cj[0], cj[1], cj[2]);
-#pragma unroll
-		for (int a = 0; a < 3; ++a) {
-			const int dd = abs((int)ci[a] - (int)cj[a]);
-			base_key[a] = __builtin_amdgcn_readfirstlane(dd * g.R * W + (int)(ci[a] + cj[a]) + off);
-		}
-		pair_base = uniform_i64(ij * g.N2);
-	}
-	bool fast = pair_base >= 0;
-#pragma unroll
-	for (int u = 0; u < kBucketEpt; ++u) { fast = fast && (uint64_t)(t[u] - pair_base) < (uint64_t)g.N2; }
-	if (__all(fast)) {
-		const unsigned short* __restrict__ ulut = (const unsigned short*)lut;
-		unsigned kk[kBucketEpt][3];
-#pragma unroll
-		for (int u = 0; u < kBucketEpt; ++u) {
-			const unsigned kl = (unsigned)(t[u] - pair_base);
-			const unsigned k = (unsigned)((double)kl * dc.inv_N_up);
-			const unsigned l = kl - __umul24(k, (unsigned)g.N);
-			unsigned ck[3], cl[3];
-			point_xyz(k, g, dc, ck[0], ck[1], ck[2]);
-			point_xyz(l, g, dc, cl[0], cl[1], cl[2]);
-#pragma unroll
-			for (int a = 0; a < 3; ++a) {
-				const unsigned li = __umul24(__usad(ck[a], cl[a], 0u), (unsigned)W) + ((unsigned)base_key[a] - ck[a] - cl[a]);
-				kk[u][a] = ulut[li];
-			}
-		}
-		unsigned k3[kBucketEpt];
-#pragma unroll
-		for (int u = 0; u < kBucketEpt; ++u) {
-			const unsigned lo = min(kk[u][0], kk[u][1]), hi = max(kk[u][0], kk[u][1]);
-			const unsigned mid = min(hi, kk[u][2]);
-			k3[u] = max(hi, kk[u][2]);
-			const unsigned k1 = min(lo, mid), k2 = max(lo, mid);
-			vi[u] = __umul24(k1, k3[u] + 1u) - (__umul24(k1, k1 - 1u) >> 1) + (k2 - k1);
-		}
-#pragma unroll
-		for (int u = 0; u < kBucketEpt; ++u) { vi[u] += tet3[k3[u]]; }
-	}
-	else {
-#pragma unroll
-		for (int u = 0; u < kBucketEpt; ++u) {
-			vi[u] = (t[u] >= 0 && t[u] < g.N4) ? (uint32_t)gather_index(g, dc, lut, t[u]) : kNoPosition;
-		}
-	}
 }
 
 // One workgroup of kThreads lanes per window of kThreads * 8 elements; bins <= 4 * kThreads buckets.
@@ -1001,7 +956,15 @@ __global__ __launch_bounds__(kThreads) void k_gather_bucketed(GatherDesc d, int6
 	__syncthreads();
 	uint32_t pos[kBucketEpt];
 	const int first = wave * (64 * kBucketEpt);   // this wave's first place in the window
-	wave_positions(d, base + first, sl, st, pos);
+	wave_positions<kBucketEpt>(d, base + first, sl, st,
+		[&](const uint32_t* vi) {
+#pragma unroll
+			for (int u = 0; u < kBucketEpt; ++u) { pos[u] = vi[u]; }
+		},
+		[&](const int64_t* vi) {
+#pragma unroll
+			for (int u = 0; u < kBucketEpt; ++u) { pos[u] = (vi[u] >= 0) ? (uint32_t)vi[u] : kNoPosition; }
+		});
 	__syncthreads();   // the decode tables are no longer read: the region becomes the sort buffers
 	// phase 2: counting sort of the window's positions by bucket = position >> shift
 	uint32_t* spos = (uint32_t*)lds;                         // [kWin] positions in bucket order
