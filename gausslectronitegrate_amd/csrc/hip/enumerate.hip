@@ -63,11 +63,35 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 //
 // mc[q] packs minimg[q] (bits 48-63) and canon[q] (bits 0-47), so a placement's four points
 // cost four loads and no branch.  The placements are the subsets of the axes whose box sum c
-// is odd (c = +-1): one loop over those subsets, the empty one (no shift) first.
+// is odd (c = +-1).
+//
+// The unshifted placement needs no loads: i is orbit-minimal and j, k, l all have orbit minimum
+// >= i (the unit, row and candidate filters), so its minimum test always passes, and its element
+// set e0 is the union of canon over the points among (i, j, k, l) in i's orbit, identity removed,
+// which the caller assembles from a per-row mask of (i, j, k) and l's own table entry.
+
+// does an image of the placement Q under one of the elements in emask fall below t = (IJ, KL)?
+__device__ inline bool image_below(const EnumGrid& g, const int Q[4][3], unsigned long long emask, int H, int IJ, int KL,
+                                   const int4* __restrict__ W)
+{
+	for (; emask; emask &= emask - 1) {
+		const int4 w = W[__builtin_ctzll(emask)];
+		int L[4];
+		for (int r = 0; r < 4; ++r) { L[r] = __mul24(w.x, Q[r][0]) + __mul24(w.y, Q[r][1]) + __mul24(w.z, Q[r][2]) + H; }
+		const int a = min(L[0], L[1]), bb = max(L[0], L[1]);
+		const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
+		int ab = __mul24(a, g.N) + bb, cd = __mul24(cc, g.N) + d;
+		if (ab > cd) { const int t = ab; ab = cd; cd = t; }
+		if (ab < IJ || (ab == IJ && cd < KL)) { return true; }
+	}
+	return false;
+}
+
 __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], const int lp[4], int i, int IJ, int KL,
-                                 const unsigned long long* __restrict__ mc, const int4* __restrict__ W)
+                                 unsigned long long e0, const unsigned long long* __restrict__ mc, const int4* __restrict__ W)
 {
 	const int n2 = g.n * g.n, H = g.h * (n2 + g.n + 1);
+	if (image_below(g, P, e0, H, IJ, KL, W)) { return false; }
 	int c[3];
 	for (int a = 0; a < 3; ++a) {
 		const int mn = min(min(P[0][a], P[1][a]), min(P[2][a], P[3][a]));
@@ -76,17 +100,16 @@ __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], const int
 	}
 	const int dx = __mul24(c[0], n2), dy = __mul24(c[1], g.n), dz = c[2];
 	const unsigned act = (c[0] != 0 ? 1u : 0u) | (c[1] != 0 ? 2u : 0u) | (c[2] != 0 ? 4u : 0u);
-	unsigned b = 0;
-	do {
-		// placement P - b*c: every point's linear index moves by the same offset
+	// the shifted placements P - b*c, b a nonempty subset of the odd axes
+	for (unsigned b = act; b != 0u; b = (b - 1u) & act) {
+		// every point's linear index moves by the same offset
 		const int off = ((b & 1u) ? dx : 0) + ((b & 2u) ? dy : 0) + ((b & 4u) ? dz : 0);
 		unsigned long long m[4];
 		for (int p = 0; p < 4; ++p) { m[p] = mc[lp[p] - off]; }
 		const int mi0 = (int)(m[0] >> 48), mi1 = (int)(m[1] >> 48), mi2 = (int)(m[2] >> 48), mi3 = (int)(m[3] >> 48);
 		if (min(min(mi0, mi1), min(mi2, mi3)) < i) { return false; }
-		unsigned long long emask = ((mi0 == i) ? m[0] : 0ull) | ((mi1 == i) ? m[1] : 0ull) | ((mi2 == i) ? m[2] : 0ull) |
-		                           ((mi3 == i) ? m[3] : 0ull);
-		emask &= (b == 0) ? 0xfffffffffffeull : 0xffffffffffffull;
+		const unsigned long long emask = (((mi0 == i) ? m[0] : 0ull) | ((mi1 == i) ? m[1] : 0ull) | ((mi2 == i) ? m[2] : 0ull) |
+		                                  ((mi3 == i) ? m[3] : 0ull)) & 0xffffffffffffull;
 		if (emask != 0ull) {
 			int Q[4][3];
 			for (int p = 0; p < 4; ++p) {
@@ -94,19 +117,9 @@ __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], const int
 				Q[p][1] = P[p][1] - ((b & 2u) ? c[1] : 0);
 				Q[p][2] = P[p][2] - ((b & 4u) ? c[2] : 0);
 			}
-			for (; emask; emask &= emask - 1) {
-				const int4 w = W[__builtin_ctzll(emask)];
-				int L[4];
-				for (int r = 0; r < 4; ++r) { L[r] = __mul24(w.x, Q[r][0]) + __mul24(w.y, Q[r][1]) + __mul24(w.z, Q[r][2]) + H; }
-				const int a = min(L[0], L[1]), bb = max(L[0], L[1]);
-				const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
-				int ab = __mul24(a, g.N) + bb, cd = __mul24(cc, g.N) + d;
-				if (ab > cd) { const int t = ab; ab = cd; cd = t; }
-				if (ab < IJ || (ab == IJ && cd < KL)) { return false; }
-			}
+			if (image_below(g, Q, emask, H, IJ, KL, W)) { return false; }
 		}
-		b = (b - act) & act;   // next subset of the odd axes
-	} while (b != 0u);
+	}
 	return true;
 }
 
@@ -141,14 +154,27 @@ __device__ inline void axis_interval(int h, int pi, int pj, int pk, int& start, 
 	len = max(0, b - a + 1);
 }
 
-// v / d for v < 2^16, 0 < d < 2^8: float reciprocal and one correction step
-__device__ inline uint32_t udiv_small(uint32_t v, uint32_t d)
+// v / d for v < 2^15 and 1 <= d <= 31 (a candidate's number inside its row, an interval length):
+// the high word of (v 2^9) * m with m = ceil(2^23 / d), one 24-bit multiply.  Exact: the product
+// over 2^32 is v/d + e with 0 <= e < 2^15 2^9 / 2^32 = 2^-8 <= 1/d, and frac(v/d) <= 1 - 1/d.
+__host__ __device__ constexpr uint32_t enum_magic(uint32_t d)
 {
-	uint32_t q = (uint32_t)((float)v * __builtin_amdgcn_rcpf((float)d));
-	const int r = (int)v - (int)__umul24(q, d);
-	if (r < 0) { q -= 1; }
-	else if (r >= (int)d) { q += 1; }
-	return q;
+	return ((1u << 23) + d - 1u) / d;
+}
+
+__device__ inline uint32_t udiv_magic(uint32_t v, uint32_t m)
+{
+	const uint32_t a = (v & 0x7fffu) << 9;
+	return (uint32_t)(((uint64_t)a * (uint64_t)(m & 0xffffffu)) >> 32);
+}
+
+// x unchanged, with its value range hidden from the optimiser (an empty asm statement, no
+// instruction).  A quotient known to be small turns __umul24 of it into a plain multiply, which
+// is lowered to the quarter-rate v_mul_lo_u32; hidden, it stays a full-rate v_mul_u32_u24.
+__device__ inline uint32_t range_opaque(uint32_t x)
+{
+	asm("" : "+v"(x));
+	return x;
 }
 
 // row index of every 64th candidate, for items of up to 64 * ENUM_ROW_AT candidates (every item of a
@@ -161,6 +187,7 @@ struct EnumRows {
 	uint32_t iv[ENUM_ROWS];        // per axis 5 bits start + h and 5 bits length (x in bits 0-9, y 10-19, z 20-29)
 	uint32_t ks[ENUM_ROWS];        // k (bits 0-15) and the candidates skipped below lmin (bits 16-31)
 	uint32_t kc[ENUM_ROWS];        // k's coordinates + h, 5 bits per axis
+	uint32_t mz[ENUM_ROWS], my[ENUM_ROWS];   // enum_magic of the z and y interval lengths
 	uint32_t wsum[ENUM_T / 64];
 	uint8_t row_at[ENUM_ROW_AT];   // row of candidate 64 w (when the item has at most 64 * ENUM_ROW_AT candidates)
 };
@@ -185,8 +212,10 @@ __device__ inline EnumItem enum_item(const EnumGrid& g, const int2* __restrict__
 	it.r0 = ur.y;
 	it.rows = min(ENUM_ROWS, klen[ic] - ur.y);
 	it.ks = klist + kstart[ic];
-	it.pi[0] = it.i / (g.n * g.n) - g.h; it.pi[1] = (it.i / g.n) % g.n - g.h; it.pi[2] = it.i % g.n - g.h;
-	it.pj[0] = it.j / (g.n * g.n) - g.h; it.pj[1] = (it.j / g.n) % g.n - g.h; it.pj[2] = it.j % g.n - g.h;
+	// coordinates below n <= 31 (the masks tell the compiler so), minus h <= 15
+	const int h = g.h & 15;
+	it.pi[0] = ((it.i / (g.n * g.n)) & 31) - h; it.pi[1] = (((it.i / g.n) % g.n) & 31) - h; it.pi[2] = ((it.i % g.n) & 31) - h;
+	it.pj[0] = ((it.j / (g.n * g.n)) & 31) - h; it.pj[1] = (((it.j / g.n) % g.n) & 31) - h; it.pj[2] = ((it.j % g.n) & 31) - h;
 	return it;
 }
 
@@ -242,6 +271,8 @@ __device__ uint32_t enum_load_rows(const EnumGrid& g, const EnumItem& it, EnumRo
 		s.iv[r] = iv;
 		s.ks[r] = (uint32_t)k | (below << 16);
 		s.kc[r] = kc;
+		s.mz[r] = (len[2] > 0) ? enum_magic((uint32_t)len[2]) : 0u;
+		s.my[r] = (len[1] > 0) ? enum_magic((uint32_t)len[1]) : 0u;
 	}
 	uint32_t total;
 	const uint32_t before = enum_block_scan(c, s.wsum, total);
@@ -283,19 +314,29 @@ __device__ inline int enum_row_indexed(const EnumRows& s, int rows, uint32_t tot
 	return lo;
 }
 
-// candidate q of row r -> k, its coordinates K and the point L (relative to the centre)
-__device__ inline void enum_decode(const EnumGrid& g, const EnumRows& s, int r, uint32_t q, int& k, int K[3], int L[3])
+// candidate q of row r -> k, its coordinates K, the point L (relative to the centre) and l's linear
+// index.  Coordinates are at most 31 + h in magnitude (h <= 15), and the masks below let the compiler
+// see that they fit the 24-bit multiplies of the canonicality test.
+__device__ inline void enum_decode(const EnumGrid& g, const EnumRows& s, int r, uint32_t q, int& k, int K[3], int L[3], int& l)
 {
 	const uint32_t ksw = s.ks[r], iv = s.iv[r], kc = s.kc[r];
+	const int h = g.h & 15;
 	k = (int)(ksw & 0xffffu);
-	for (int a = 0; a < 3; ++a) { K[a] = (int)((kc >> (5 * a)) & 31u) - g.h; }
+	for (int a = 0; a < 3; ++a) { K[a] = (int)((kc >> (5 * a)) & 31u) - h; }
 	const uint32_t v = q - s.pre[r] + (ksw >> 16);
 	const uint32_t lz = (iv >> 25) & 31u, ly = (iv >> 15) & 31u;
-	const uint32_t vz = udiv_small(v, lz);
-	L[2] = (int)((iv >> 20) & 31u) - g.h + (int)(v - __umul24(vz, lz));
-	const uint32_t vy = udiv_small(vz, ly);
-	L[1] = (int)((iv >> 10) & 31u) - g.h + (int)(vz - __umul24(vy, ly));
-	L[0] = (int)(iv & 31u) - g.h + (int)vy;
+	const uint32_t vz = range_opaque(udiv_magic(v, s.mz[r]));
+	const uint32_t vy = range_opaque(udiv_magic(vz, s.my[r]));
+	// l's coordinates + h: interval start + offset, each below n <= 31 (the mask outside the
+	// barrier, so that it does not narrow the remainder's multiply)
+	const uint32_t cz = range_opaque(((iv >> 20) & 31u) + (v - __umul24(vz, lz))) & 31u;
+	const uint32_t cy = range_opaque(((iv >> 10) & 31u) + (vz - __umul24(vy, ly))) & 31u;
+	const uint32_t cx = ((iv & 31u) + vy) & 31u;
+	L[0] = (int)cx - h;
+	L[1] = (int)cy - h;
+	L[2] = (int)cz - h;
+	const uint32_t n = (uint32_t)g.n & 31u;
+	l = (int)(__umul24(__umul24(cx, n) + cy, n) + cz);
 }
 
 // pass 1: bitmap words of each item
@@ -309,14 +350,21 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_item_words(EnumGrid g, const int
 	if (threadIdx.x == 0) { words[blockIdx.x] = (tot + 63) / 64; }
 }
 
-// pass 2: test every candidate once; one bit per candidate, accepted count per item
+// point tables of grids with up to this many points are read from LDS (32 KB at most)
+constexpr int ENUM_LDS_POINTS = 4096;
+
+// pass 2: test every candidate once; one bit per candidate, accepted count per item.  With
+// kLdsMc the workgroup first copies the point table mc (g.N entries, dynamic LDS) and reads it there.
+template <bool kLdsMc>
 __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __restrict__ items, const int2* __restrict__ units,
 	const int* __restrict__ unit_icand, const int* __restrict__ klist, const int* __restrict__ kstart, const int* __restrict__ klen,
 	const unsigned long long* __restrict__ mc, const int64_t* __restrict__ word_base, unsigned long long* __restrict__ bits,
 	uint32_t* __restrict__ accepted)
 {
+	extern __shared__ unsigned long long smc[];
 	__shared__ EnumRows s;
 	__shared__ int4 W[48];
+	__shared__ unsigned long long rmask[ENUM_ROWS];   // per row: the unshifted placement's elements from (i, j, k)
 	if (threadIdx.x < 48) {
 		const int e = threadIdx.x;
 		int w[3];
@@ -325,10 +373,20 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __r
 		w[c_oh_col[e][2]] = c_oh_sgn[e][2];
 		W[e] = make_int4(w[0], w[1], w[2], 0);
 	}
+	if (kLdsMc) {
+		for (int q = threadIdx.x; q < g.N; q += ENUM_T) { smc[q] = mc[q]; }
+	}
+	const unsigned long long* __restrict__ tmc = kLdsMc ? smc : mc;
 	const EnumItem it = enum_item(g, items, units, unit_icand, klist, kstart, klen);
 	const int lane = threadIdx.x & 63;
 	const int64_t word = word_base[blockIdx.x];
-	const uint32_t tot = enum_load_rows(g, it, s);   // its barriers also publish W
+	const uint32_t tot = enum_load_rows(g, it, s);   // its barriers also publish W and the point table
+	if ((int)threadIdx.x < it.rows) {
+		const unsigned long long mi = tmc[it.i], mj = tmc[it.j], mk = tmc[s.ks[threadIdx.x] & 0xffffu];
+		const unsigned long long e = mi | (((int)(mj >> 48) == it.i) ? mj : 0ull) | (((int)(mk >> 48) == it.i) ? mk : 0ull);
+		rmask[threadIdx.x] = e & 0xfffffffffffeull;   // canon bits without the identity
+	}
+	__syncthreads();
 	const bool indexed = tot <= 64u * ENUM_ROW_AT;
 	uint32_t acc = 0;
 	int P[4][3];
@@ -339,12 +397,15 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __r
 		if (q < tot) {
 			int k, L[3];
 			const int r = indexed ? enum_row_indexed(s, it.rows, tot, q) : enum_row(s, it.rows, q);
-			enum_decode(g, s, r, q, k, P[2], L);
-			const int l = __mul24(__mul24(L[0] + g.h, g.n) + L[1] + g.h, g.n) + L[2] + g.h;
-			if ((int)(mc[l] >> 48) >= it.i) {
+			int l;
+			enum_decode(g, s, r, q, k, P[2], L, l);
+			const unsigned long long ml = tmc[l];
+			const int mil = (int)(ml >> 48);
+			if (mil >= it.i) {
 				P[3][0] = L[0]; P[3][1] = L[1]; P[3][2] = L[2];
 				const int lp[4] = { it.i, it.j, k, l };
-				keep = eri_is_canonical(g, P, lp, it.i, it.IJ, __mul24(k, g.N) + l, mc, W);
+				const unsigned long long e0 = rmask[r] | ((mil == it.i) ? (ml & 0xfffffffffffeull) : 0ull);
+				keep = eri_is_canonical(g, P, lp, it.i, it.IJ, __mul24(k, g.N) + l, e0, tmc, W);
 			}
 		}
 		const unsigned long long m = __ballot(keep);
@@ -390,8 +451,9 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_expand(EnumGrid g, const int2* _
 			const uint32_t q = w * 64 + (uint32_t)__builtin_ctzll(m);
 			while (r + 1 < it.rows && s.pre[r + 1] <= q) { ++r; }
 			int k, K[3], L[3];
-			enum_decode(g, s, r, q, k, K, L);
-			out[pos + rank++] = (ij * g.N + k) * g.N + lin_c(g, L[0], L[1], L[2]);
+			int l;
+			enum_decode(g, s, r, q, k, K, L, l);
+			out[pos + rank++] = (ij * g.N + k) * g.N + l;
 		}
 		pos += total;
 	}
@@ -565,8 +627,14 @@ static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 		GLI_HIP(hipGetLastError());
 		const int64_t nwords = exclusive_scan<int64_t>(words.p, ni, word_base.p);
 		DevBuf<unsigned long long> bits((size_t)std::max<int64_t>(nwords, 1));
-		k_eri_test<<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, mc.p, word_base.p, bits.p,
-		                                  accepted.p);
+		if (p.g.N <= ENUM_LDS_POINTS) {
+			k_eri_test<true><<<ni, ENUM_T, (size_t)p.g.N * sizeof(unsigned long long), st>>>(p.g, items.p, units.p, unit_ic.p, klist.p,
+				kstart.p, klen.p, mc.p, word_base.p, bits.p, accepted.p);
+		}
+		else {
+			k_eri_test<false><<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, mc.p, word_base.p,
+			                                         bits.p, accepted.p);
+		}
 		GLI_HIP(hipGetLastError());
 		total = exclusive_scan<uint32_t>(accepted.p, ni, out_base.p);
 		GLI_HIP(hipMalloc(&out, (size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));

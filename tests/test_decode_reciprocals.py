@@ -5,7 +5,11 @@ divisor N and a < 2^48 with divisor N^2 in double (the four-index split of grids
 
 float: every numerator and divisor.  double: every divisor, at the numerators where the quotient is
 most fragile (remainder 0 and remainder d - 1, the largest quotients below the bound, and a seeded
-random sample).  IEEE round-to-nearest products and truncating conversions, as on the GPU."""
+random sample).  IEEE round-to-nearest products and truncating conversions, as on the GPU.
+
+The enumeration's candidate decode (udiv_magic in csrc/hip/enumerate.hip) divides by an interval
+length d <= 31 with one 24-bit multiply-high: v // d = ((v << 9) * ceil(2^23 / d)) >> 32 for v < 2^15,
+checked for every numerator and divisor."""
 import numpy as np
 
 
@@ -53,3 +57,12 @@ def test_double_reciprocal_quotients_divisor_N2():
         a = np.concatenate([qs * d, qs * d + (d - np.uint64(1)), rng.integers(0, 1 << 48, 64, dtype=np.uint64)])
         a = a[a < np.uint64(1 << 48)]
         _check_double(a, np.full(a.shape, N * N, dtype=np.uint64))
+
+
+def test_enumeration_magic_quotients_exhaustive():
+    v = np.arange(1 << 15, dtype=np.uint64)
+    for d in range(1, 32):
+        m = ((1 << 23) + d - 1) // d
+        assert m < (1 << 24), d   # a 24-bit multiplier operand
+        q = ((v << np.uint64(9)) * np.uint64(m)) >> np.uint64(32)
+        assert np.array_equal(q, v // np.uint64(d)), d
