@@ -44,9 +44,9 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 	return ((x + g.h) * g.n + (y + g.h)) * g.n + (z + g.h);
 }
 
-// Is the centred 4-tuple P (coordinates relative to the grid centre) the minimum of its orbit?
-// i = t's first index (an orbit-minimal point), IJ = i*N + j, KL = k*N + l, lp = the linear
-// indices (i, j, k, l) of the four points.  A bias placement shifts every point by the same
+// Is the centred 4-tuple (i, j, k, l) (coordinates relative to the grid centre) the minimum of its
+// orbit?  i = t's first index (an orbit-minimal point), IJ = i*N + j, KL = k*N + l, lp = the
+// linear indices (i, j, k, l) of the four points.  A bias placement shifts every point by the same
 // vector, which stays inside the grid, so its linear indices are lp minus one offset.
 //
 // An image's four-index starts with the smallest of its four image points.  For a
@@ -59,7 +59,7 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 // The elements to test are the union of canon[q] over the placement's points in i's orbit;
 // the identity image of the unshifted tuple is t itself and is skipped (element 0 is the
 // identity).  W[g] holds the signed weights of the original axes in the image's linear
-// index, lin(g q) = W[g] . q + H, so an image point costs three 24-bit multiply-adds.
+// index, lin(g q) = W[g] . q + H.
 //
 // mc[q] packs minimg[q] (bits 48-63) and canon[q] (bits 0-47), so a placement's four points
 // cost four loads and no branch.  The placements are the subsets of the axes whose box sum c
@@ -70,34 +70,48 @@ __device__ inline int lin_c(const EnumGrid& g, int x, int y, int z)
 // set e0 is the union of canon over the points among (i, j, k, l) in i's orbit, identity removed,
 // which the caller assembles from a per-row mask of (i, j, k) and l's own table entry.
 
-// does an image of the placement Q under one of the elements in emask fall below t = (IJ, KL)?
-__device__ inline bool image_below(const EnumGrid& g, const int Q[4][3], unsigned long long emask, int H, int IJ, int KL,
-                                   const int4* __restrict__ W)
+// Per workgroup: i and j are fixed, so their images lin(g i) = W[g] . i + H and lin(g j) are
+// tabulated once per element (LP), as are the per-axis extents of (i, j) for the box sums.
+struct CanonUnit {
+	int i, IJ, H;
+	int mn01[3], mx01[3];   // per axis min and max of the coordinates of i and j
+};
+
+// Does an image of the placement (i, j, k, l) - s under one of the elements in emask fall below
+// t = (IJ, KL)?  lin(g (q - s)) = lin(g q) - W[g] . s, with s the bias shift (kShift false: the
+// unshifted placement); K and Lc are the coordinates of k and l.  An image point costs three
+// 24-bit multiply-adds, i's and j's come from the table.
+template <bool kShift>
+__device__ inline bool image_below(const EnumGrid& g, const CanonUnit& u, const int K[3], const int Lc[3], const int sh[3],
+                                   unsigned long long emask, int KL, const int4* __restrict__ W, const int2* __restrict__ LP)
 {
 	for (; emask; emask &= emask - 1) {
-		const int4 w = W[__builtin_ctzll(emask)];
-		int L[4];
-		for (int r = 0; r < 4; ++r) { L[r] = __mul24(w.x, Q[r][0]) + __mul24(w.y, Q[r][1]) + __mul24(w.z, Q[r][2]) + H; }
-		const int a = min(L[0], L[1]), bb = max(L[0], L[1]);
-		const int cc = min(L[2], L[3]), d = max(L[2], L[3]);
+		const int e = __builtin_ctzll(emask);
+		const int4 w = W[e];
+		const int2 lij = LP[e];
+		const int S = kShift ? __mul24(w.x, sh[0]) + __mul24(w.y, sh[1]) + __mul24(w.z, sh[2]) : 0;
+		const int HS = u.H - S;
+		const int L0 = lij.x - S, L1 = lij.y - S;
+		const int L2 = __mul24(w.x, K[0]) + __mul24(w.y, K[1]) + __mul24(w.z, K[2]) + HS;
+		const int L3 = __mul24(w.x, Lc[0]) + __mul24(w.y, Lc[1]) + __mul24(w.z, Lc[2]) + HS;
+		const int a = min(L0, L1), bb = max(L0, L1);
+		const int cc = min(L2, L3), d = max(L2, L3);
 		int ab = __mul24(a, g.N) + bb, cd = __mul24(cc, g.N) + d;
 		if (ab > cd) { const int t = ab; ab = cd; cd = t; }
-		if (ab < IJ || (ab == IJ && cd < KL)) { return true; }
+		if (ab < u.IJ || (ab == u.IJ && cd < KL)) { return true; }
 	}
 	return false;
 }
 
-__device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], const int lp[4], int i, int IJ, int KL,
-                                 unsigned long long e0, const unsigned long long* __restrict__ mc, const int4* __restrict__ W)
+__device__ bool eri_is_canonical(const EnumGrid& g, const CanonUnit& u, const int K[3], const int Lc[3], const int lp[4], int KL,
+                                 unsigned long long e0, const unsigned long long* __restrict__ mc, const int4* __restrict__ W,
+                                 const int2* __restrict__ LP)
 {
-	const int n2 = g.n * g.n, H = g.h * (n2 + g.n + 1);
-	if (image_below(g, P, e0, H, IJ, KL, W)) { return false; }
+	const int i = u.i;
+	if (image_below<false>(g, u, K, Lc, K, e0, KL, W, LP)) { return false; }
 	int c[3];
-	for (int a = 0; a < 3; ++a) {
-		const int mn = min(min(P[0][a], P[1][a]), min(P[2][a], P[3][a]));
-		const int mx = max(max(P[0][a], P[1][a]), max(P[2][a], P[3][a]));
-		c[a] = mn + mx;
-	}
+	for (int a = 0; a < 3; ++a) { c[a] = min(u.mn01[a], min(K[a], Lc[a])) + max(u.mx01[a], max(K[a], Lc[a])); }
+	const int n2 = g.n * g.n;
 	const int dx = __mul24(c[0], n2), dy = __mul24(c[1], g.n), dz = c[2];
 	const unsigned act = (c[0] != 0 ? 1u : 0u) | (c[1] != 0 ? 2u : 0u) | (c[2] != 0 ? 4u : 0u);
 	// the shifted placements P - b*c, b a nonempty subset of the odd axes
@@ -107,17 +121,14 @@ __device__ bool eri_is_canonical(const EnumGrid& g, const int P[4][3], const int
 		unsigned long long m[4];
 		for (int p = 0; p < 4; ++p) { m[p] = mc[lp[p] - off]; }
 		const int mi0 = (int)(m[0] >> 48), mi1 = (int)(m[1] >> 48), mi2 = (int)(m[2] >> 48), mi3 = (int)(m[3] >> 48);
-		if (min(min(mi0, mi1), min(mi2, mi3)) < i) { return false; }
-		const unsigned long long emask = (((mi0 == i) ? m[0] : 0ull) | ((mi1 == i) ? m[1] : 0ull) | ((mi2 == i) ? m[2] : 0ull) |
-		                                  ((mi3 == i) ? m[3] : 0ull)) & 0xffffffffffffull;
-		if (emask != 0ull) {
-			int Q[4][3];
-			for (int p = 0; p < 4; ++p) {
-				Q[p][0] = P[p][0] - ((b & 1u) ? c[0] : 0);
-				Q[p][1] = P[p][1] - ((b & 2u) ? c[1] : 0);
-				Q[p][2] = P[p][2] - ((b & 4u) ? c[2] : 0);
-			}
-			if (image_below(g, Q, emask, H, IJ, KL, W)) { return false; }
+		const int mn = min(min(mi0, mi1), min(mi2, mi3));
+		if (mn < i) { return false; }
+		// a point of the placement reaches i (and canon of such a point is never empty)
+		if (mn == i) {
+			const unsigned long long emask = (((mi0 == i) ? m[0] : 0ull) | ((mi1 == i) ? m[1] : 0ull) | ((mi2 == i) ? m[2] : 0ull) |
+			                                  ((mi3 == i) ? m[3] : 0ull)) & 0xffffffffffffull;
+			const int sh[3] = { (b & 1u) ? c[0] : 0, (b & 2u) ? c[1] : 0, (b & 4u) ? c[2] : 0 };
+			if (image_below<true>(g, u, K, Lc, sh, emask, KL, W, LP)) { return false; }
 		}
 	}
 	return true;
@@ -364,7 +375,17 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __r
 	extern __shared__ unsigned long long smc[];
 	__shared__ EnumRows s;
 	__shared__ int4 W[48];
+	__shared__ int2 LP[48];                            // per element: lin(g i), lin(g j)
 	__shared__ unsigned long long rmask[ENUM_ROWS];   // per row: the unshifted placement's elements from (i, j, k)
+	const EnumItem it = enum_item(g, items, units, unit_icand, klist, kstart, klen);
+	CanonUnit u;
+	u.i = it.i;
+	u.IJ = it.IJ;
+	u.H = g.h * (g.n * g.n + g.n + 1);
+	for (int a = 0; a < 3; ++a) {
+		u.mn01[a] = min(it.pi[a], it.pj[a]);
+		u.mx01[a] = max(it.pi[a], it.pj[a]);
+	}
 	if (threadIdx.x < 48) {
 		const int e = threadIdx.x;
 		int w[3];
@@ -372,15 +393,16 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __r
 		w[c_oh_col[e][1]] = c_oh_sgn[e][1] * g.n;
 		w[c_oh_col[e][2]] = c_oh_sgn[e][2];
 		W[e] = make_int4(w[0], w[1], w[2], 0);
+		LP[e] = make_int2(w[0] * it.pi[0] + w[1] * it.pi[1] + w[2] * it.pi[2] + u.H,
+		                  w[0] * it.pj[0] + w[1] * it.pj[1] + w[2] * it.pj[2] + u.H);
 	}
 	if (kLdsMc) {
 		for (int q = threadIdx.x; q < g.N; q += ENUM_T) { smc[q] = mc[q]; }
 	}
 	const unsigned long long* __restrict__ tmc = kLdsMc ? smc : mc;
-	const EnumItem it = enum_item(g, items, units, unit_icand, klist, kstart, klen);
 	const int lane = threadIdx.x & 63;
 	const int64_t word = word_base[blockIdx.x];
-	const uint32_t tot = enum_load_rows(g, it, s);   // its barriers also publish W and the point table
+	const uint32_t tot = enum_load_rows(g, it, s);   // its barriers also publish W, LP and the point table
 	if ((int)threadIdx.x < it.rows) {
 		const unsigned long long mi = tmc[it.i], mj = tmc[it.j], mk = tmc[s.ks[threadIdx.x] & 0xffffu];
 		const unsigned long long e = mi | (((int)(mj >> 48) == it.i) ? mj : 0ull) | (((int)(mk >> 48) == it.i) ? mk : 0ull);
@@ -389,23 +411,19 @@ __global__ __launch_bounds__(ENUM_T) void k_eri_test(EnumGrid g, const int2* __r
 	__syncthreads();
 	const bool indexed = tot <= 64u * ENUM_ROW_AT;
 	uint32_t acc = 0;
-	int P[4][3];
-	for (int a = 0; a < 3; ++a) { P[0][a] = it.pi[a]; P[1][a] = it.pj[a]; }
 	for (uint32_t q0 = 0; q0 < tot; q0 += ENUM_T) {
 		const uint32_t q = q0 + threadIdx.x;
 		bool keep = false;
 		if (q < tot) {
-			int k, L[3];
+			int k, K[3], L[3], l;
 			const int r = indexed ? enum_row_indexed(s, it.rows, tot, q) : enum_row(s, it.rows, q);
-			int l;
-			enum_decode(g, s, r, q, k, P[2], L, l);
+			enum_decode(g, s, r, q, k, K, L, l);
 			const unsigned long long ml = tmc[l];
 			const int mil = (int)(ml >> 48);
 			if (mil >= it.i) {
-				P[3][0] = L[0]; P[3][1] = L[1]; P[3][2] = L[2];
 				const int lp[4] = { it.i, it.j, k, l };
 				const unsigned long long e0 = rmask[r] | ((mil == it.i) ? (ml & 0xfffffffffffeull) : 0ull);
-				keep = eri_is_canonical(g, P, lp, it.i, it.IJ, __mul24(k, g.N) + l, e0, tmc, W);
+				keep = eri_is_canonical(g, u, K, L, lp, __mul24(k, g.N) + l, e0, tmc, W, LP);
 			}
 		}
 		const unsigned long long m = __ballot(keep);
