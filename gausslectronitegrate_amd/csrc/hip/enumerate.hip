@@ -602,30 +602,52 @@ static void plan_enumeration(int n, EnumPlan& p)
 	}
 }
 
-static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
+// The device form of one grid's enumeration plan: unit and row tables, the point table, the work
+// items, the items' bitmap word offsets and their per-call count and offset arrays.  It depends on
+// the grid size only, so a repeated enumeration of one grid skips the host planning, the uploads
+// and the word-count pass.  The last grid's plan is kept (context lock held) until the process ends
+// or another grid is enumerated: at most a few tens of MB (13^3: 1.9M items).
+struct EnumDevPlan {
+	int64_t n = 0;   // 0: no plan
+	EnumGrid g{};
+	unsigned ni = 0;
+	int64_t nwords = 0;
+	DevBuf<int2> units, items;
+	DevBuf<int> unit_ic, klist, kstart, klen;
+	DevBuf<unsigned long long> mc;
+	DevBuf<int64_t> word_base, out_base;
+	DevBuf<uint32_t> accepted;
+};
+
+static EnumDevPlan& enum_dev_plan(int64_t n)
 {
-	Context& c = ctx();
-	std::lock_guard<std::recursive_mutex> lk(c.mu);
-	GLI_REQUIRE(n >= 1 && (n % 2) == 1, "enumeration needs a symmetric grid with an odd number of points per axis");
-	GLI_REQUIRE(n <= 31, "grid too large for the enumeration (at most 31 points per axis)");
+	static EnumDevPlan* cached = nullptr;
+	if (cached == nullptr) { cached = new EnumDevPlan(); }
+	EnumDevPlan& dp = *cached;
+	if (dp.n == n) { return dp; }
+	dp.n = 0;   // stays empty if the build below fails
 	EnumPlan p;
 	plan_enumeration((int)n, p);
-	DevBuf<int2> units(p.units.size());
-	DevBuf<int> unit_ic(p.unit_icand.size()), klist(p.klist.size()), kstart(p.kstart.size()), klen(p.klen.size());
-	hipStream_t st = c.stream;
-	GLI_HIP(hipMemcpyAsync(units.p, p.units.data(), p.units.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-	GLI_HIP(hipMemcpyAsync(unit_ic.p, p.unit_icand.data(), p.unit_icand.size() * sizeof(int), hipMemcpyHostToDevice, st));
-	GLI_HIP(hipMemcpyAsync(klist.p, p.klist.data(), p.klist.size() * sizeof(int), hipMemcpyHostToDevice, st));
-	GLI_HIP(hipMemcpyAsync(kstart.p, p.kstart.data(), p.kstart.size() * sizeof(int), hipMemcpyHostToDevice, st));
-	GLI_HIP(hipMemcpyAsync(klen.p, p.klen.data(), p.klen.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	hipStream_t st = ctx().stream;
+	dp.g = p.g;
+	dp.units.alloc(p.units.size());
+	dp.unit_ic.alloc(p.unit_icand.size());
+	dp.klist.alloc(p.klist.size());
+	dp.kstart.alloc(p.kstart.size());
+	dp.klen.alloc(p.klen.size());
+	GLI_HIP(hipMemcpyAsync(dp.units.p, p.units.data(), p.units.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(dp.unit_ic.p, p.unit_icand.data(), p.unit_icand.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(dp.klist.p, p.klist.data(), p.klist.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(dp.kstart.p, p.kstart.data(), p.kstart.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(dp.klen.p, p.klen.data(), p.klen.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	// minimg (bits 48-63) and canon (bits 0-47) of every point in one table for the candidate test
 	std::vector<unsigned long long> hmc(p.canon.size());
 	for (size_t q = 0; q < hmc.size(); ++q) {
 		GLI_REQUIRE((p.canon[q] >> 48) == 0 && p.minimg[q] >= 0 && p.minimg[q] < 65536, "point tables do not fit the packed form");
 		hmc[q] = p.canon[q] | ((unsigned long long)p.minimg[q] << 48);
 	}
-	DevBuf<unsigned long long> mc(hmc.size());
-	GLI_HIP(hipMemcpyAsync(mc.p, hmc.data(), hmc.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
+	dp.mc.alloc(hmc.size());
+	GLI_HIP(hipMemcpyAsync(dp.mc.p, hmc.data(), hmc.size() * sizeof(unsigned long long), hipMemcpyHostToDevice, st));
 	// work items: blocks of ENUM_ROWS rows of each unit, in (unit, row) order
 	std::vector<int2> hitems;
 	for (size_t u = 0; u < p.units.size(); ++u) {
@@ -634,29 +656,52 @@ static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
 	}
 	GLI_REQUIRE(hitems.size() < (size_t)INT32_MAX, "enumeration has too many work items");
 	const unsigned ni = (unsigned)hitems.size();
-	DevBuf<int2> items(std::max<size_t>(hitems.size(), 1));
-	if (ni > 0) { GLI_HIP(hipMemcpyAsync(items.p, hitems.data(), hitems.size() * sizeof(int2), hipMemcpyHostToDevice, st)); }
-	DevBuf<int64_t> words(ni), word_base(ni), out_base(ni);
-	DevBuf<uint32_t> accepted(ni);
+	dp.items.alloc(std::max<size_t>(hitems.size(), 1));
+	if (ni > 0) { GLI_HIP(hipMemcpyAsync(dp.items.p, hitems.data(), hitems.size() * sizeof(int2), hipMemcpyHostToDevice, st)); }
+	dp.word_base.alloc(ni);
+	dp.out_base.alloc(ni);
+	dp.accepted.alloc(ni);
+	dp.nwords = 0;
+	if (ni > 0) {
+		DevBuf<int64_t> words(ni);
+		k_eri_item_words<<<ni, ENUM_T, 0, st>>>(dp.g, dp.items.p, dp.units.p, dp.unit_ic.p, dp.klist.p, dp.kstart.p, dp.klen.p,
+		                                        words.p);
+		GLI_HIP(hipGetLastError());
+		dp.nwords = exclusive_scan<int64_t>(words.p, ni, dp.word_base.p);
+	}
+	GLI_HIP(hipStreamSynchronize(st));   // the host tables above are read by the copies until here
+	dp.ni = ni;
+	dp.n = n;
+	return dp;
+}
+
+static void eri_enumerate_dev(int64_t n, int64_t** d_out, int64_t* num)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	GLI_REQUIRE(n >= 1 && (n % 2) == 1, "enumeration needs a symmetric grid with an odd number of points per axis");
+	GLI_REQUIRE(n <= 31, "grid too large for the enumeration (at most 31 points per axis)");
+	EnumDevPlan& dp = enum_dev_plan(n);
+	hipStream_t st = c.stream;
+	const unsigned ni = dp.ni;
 	int64_t total = 0;
 	int64_t* out = nullptr;
 	if (ni > 0) {
-		k_eri_item_words<<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, words.p);
-		GLI_HIP(hipGetLastError());
-		const int64_t nwords = exclusive_scan<int64_t>(words.p, ni, word_base.p);
-		DevBuf<unsigned long long> bits((size_t)std::max<int64_t>(nwords, 1));
-		if (p.g.N <= ENUM_LDS_POINTS) {
-			k_eri_test<true><<<ni, ENUM_T, (size_t)p.g.N * sizeof(unsigned long long), st>>>(p.g, items.p, units.p, unit_ic.p, klist.p,
-				kstart.p, klen.p, mc.p, word_base.p, bits.p, accepted.p);
+		// the bitmap is written whole by the test (every word of every item), so it needs no clearing
+		DevBuf<unsigned long long> bits((size_t)std::max<int64_t>(dp.nwords, 1));
+		if (dp.g.N <= ENUM_LDS_POINTS) {
+			k_eri_test<true><<<ni, ENUM_T, (size_t)dp.g.N * sizeof(unsigned long long), st>>>(dp.g, dp.items.p, dp.units.p,
+				dp.unit_ic.p, dp.klist.p, dp.kstart.p, dp.klen.p, dp.mc.p, dp.word_base.p, bits.p, dp.accepted.p);
 		}
 		else {
-			k_eri_test<false><<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, mc.p, word_base.p,
-			                                         bits.p, accepted.p);
+			k_eri_test<false><<<ni, ENUM_T, 0, st>>>(dp.g, dp.items.p, dp.units.p, dp.unit_ic.p, dp.klist.p, dp.kstart.p, dp.klen.p,
+			                                         dp.mc.p, dp.word_base.p, bits.p, dp.accepted.p);
 		}
 		GLI_HIP(hipGetLastError());
-		total = exclusive_scan<uint32_t>(accepted.p, ni, out_base.p);
+		total = exclusive_scan<uint32_t>(dp.accepted.p, ni, dp.out_base.p);
 		GLI_HIP(hipMalloc(&out, (size_t)std::max<int64_t>(total, 1) * sizeof(int64_t)));
-		k_eri_expand<<<ni, ENUM_T, 0, st>>>(p.g, items.p, units.p, unit_ic.p, klist.p, kstart.p, klen.p, word_base.p, bits.p, out_base.p, out);
+		k_eri_expand<<<ni, ENUM_T, 0, st>>>(dp.g, dp.items.p, dp.units.p, dp.unit_ic.p, dp.klist.p, dp.kstart.p, dp.klen.p,
+		                                    dp.word_base.p, bits.p, dp.out_base.p, out);
 		GLI_HIP(hipGetLastError());
 		GLI_HIP(hipStreamSynchronize(st));   // before the bitmap is released
 	}
