@@ -722,15 +722,17 @@ def main():
     Mp = (M + 1 + 15) // 16 * 16
     C1 = L.gli_eri_key_count(args.n, int(np.array_equal(c, c[::-1])))
     Cp = (C1 + 63) // 64 * 64
-    if name == "gather":
+    if name in ("gather", "gather_bucketed"):
+        # the library names the phase after the gather kernel it launched
+        kernel = "k_" + name
         achieved = 16.0 * num / (dom_ms / 1e3) / 1e9   # index in + value out, bytes per element
-        traffic = measured_traffic("k_gather", workload)
+        traffic = measured_traffic(kernel, workload)
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "traffic_source": ("profiles/k_gather_%s_traffic.md: rocprofv3 FETCH_SIZE + WRITE_SIZE of one dispatch, "
+                "traffic_source": ("profiles/%s_%s_traffic.md: rocprofv3 FETCH_SIZE + WRITE_SIZE of one dispatch, "
                                    "calibrated with a copy kernel of known size (a committed profile, not measured in "
-                                   "this run)" % workload) if traffic is not None else None,
-                "kernel": "k_gather", "algorithmic_bytes_per_launch": 16 * num}
+                                   "this run)" % (kernel, workload)) if traffic is not None else None,
+                "kernel": kernel, "algorithmic_bytes_per_launch": 16 * num}
     else:
         flops = {"mode1": 2.0 * Mp * Mp * Cp * Mp, "mode2": 2.0 * Mp * Cp * Cp * Mp / 2,
                  "mode3": 2.0 * Cp * Cp * Cp * Mp / 6}.get(name, 0.0)

@@ -1489,7 +1489,8 @@ static void launch_gather_bucketed(const GatherDesc& d, int xcd, int threads)
 	GLI_HIP(hipGetLastError());
 }
 
-static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
+// returns the phase name of the kernel it launched: "gather" (k_gather) or "gather_bucketed"
+static const char* launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 {
 	// window-bucketed reads: on with knob 1; with knob 2 (automatic) for key tensors above 128 MB,
 	// where they measured faster (13^3: 7.50 against 7.88 ms; 11^3: 1.52 against 1.46 ms)
@@ -1497,11 +1498,11 @@ static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 	const bool bucketed = bk == 1 || (bk == 2 && tet_index(0, 0, d.C1) > kLargeKeyTensorEntries);
 	if (bucketed && gather_bucketed_fits(d, ctx().gather_threads)) {
 		launch_gather_bucketed(d, xcd, ctx().gather_threads);
-		return;
+		return "gather_bucketed";
 	}
 	const int threads = ctx().gather_threads;
 	const int64_t pieces = ceil_div(d.num, (int64_t)threads * ept);
-	if (pieces == 0) { return; }
+	if (pieces == 0) { return "gather"; }
 	const int64_t chunks = xcd ? ceil_div(pieces, 8) : 0;
 	const int64_t blocks = xcd ? 8 * chunks : pieces;
 	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
@@ -1512,6 +1513,7 @@ static void launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 		k_gather<decltype(E)::value, decltype(Lds)::value><<<(unsigned)blocks, threads, shm, st>>>(d, chunks);
 	});
 	GLI_HIP(hipGetLastError());
+	return "gather";
 }
 
 static void launch_gather_grouped(const GatherDesc* d_descs, int n, int64_t total_pieces, int ept, int xcd, int lds,
@@ -1910,9 +1912,8 @@ static void eri_values_noalloc(const double* coeffs, int64_t K, const int64_t gr
 	launch_kernel_tensor(s, T3);
 	phase_mark("kernel_tensor");
 	key_tensor_fp64(s, B, T3, Y1, Y2, V, c.mode1_half != 0);
-	launch_gather(gather_desc(s, d_idx, num, V, d_vals, nullptr), c.gather_ept, resolve_gather_order(c.gather_xcd, tet_index(0, 0, s.C1)),
-	              c.gather_lds);
-	phase_mark("gather");
+	phase_mark(launch_gather(gather_desc(s, d_idx, num, V, d_vals, nullptr), c.gather_ept,
+	                         resolve_gather_order(c.gather_xcd, tet_index(0, 0, s.C1)), c.gather_lds));
 }
 
 // values of num indices (device arrays) from a prepared evaluation, queued on the library stream
@@ -1931,9 +1932,9 @@ static void eri_apply(const EriPrepared& ep, const int64_t* d_idx, int64_t num, 
 		if (mark) { phase_mark("direct"); }
 		return;
 	}
-	launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept,
-	              resolve_gather_order(c.gather_xcd, tet_index(0, 0, s.C1)), c.gather_lds);
-	if (mark) { phase_mark("gather"); }
+	const char* kernel = launch_gather(gather_desc(s, d_idx, num, w.V.p, d_vals, nullptr), c.gather_ept,
+	                                   resolve_gather_order(c.gather_xcd, tet_index(0, 0, s.C1)), c.gather_lds);
+	if (mark) { phase_mark(kernel); }
 }
 
 static void eri_values_dev(const double* coeffs, int64_t K, const int64_t grid[6], const int64_t* d_idx, int64_t num,

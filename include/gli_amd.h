@@ -76,7 +76,9 @@ int gli_get_tuning(const char* knob);
 /* Phase timing of ERI evaluations with HIP events on the library stream.  While enabled, every
  * evaluation records its phase boundaries (no synchronisation between evaluations);
  * gli_phase_times returns, per phase name, the average duration in ms over the evaluations since
- * timing was enabled or last queried, and starts a new average. */
+ * timing was enabled or last queried, and starts a new average.  The key-tensor method records
+ * "tables", "kernel_tensor", "mode1", "mode2", "mode3" and the gather as "gather" (k_gather) or
+ * "gather_bucketed" (the window-bucketed form); the per-element method records "direct". */
 int gli_enable_phase_timing(int on);
 int gli_phase_times(double* ms, const char** names, int max);
 
