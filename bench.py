@@ -523,7 +523,7 @@ def main():
         knob, value = kv.split("=", 1)
         _lib.check(L.gli_set_tuning(knob.encode(), int(value)), "gli_set_tuning " + kv)
     tuning = {k: L.gli_get_tuning(k.encode()) for k in ("tile_order", "gemm_shape", "gather_ept", "gather_order", "gather_lds",
-                                                      "mode1_half", "gather_threads", "gather_bucketed")}
+                                                      "mode1_half", "gather_threads", "gather_bucketed", "gather_classed")}
     method = METHODS[args.method]
     c = synthetic_coefficients(args.K, args.seed, symmetrise=not args.asymmetric)
     g = sym_grid6(args.n)
@@ -722,7 +722,7 @@ def main():
     Mp = (M + 1 + 15) // 16 * 16
     C1 = L.gli_eri_key_count(args.n, int(np.array_equal(c, c[::-1])))
     Cp = (C1 + 63) // 64 * 64
-    if name in ("gather", "gather_bucketed"):
+    if name in ("gather", "gather_bucketed", "gather_classed"):
         # the library names the phase after the gather kernel it launched
         kernel = "k_" + name
         achieved = 16.0 * num / (dom_ms / 1e3) / 1e9   # index in + value out, bytes per element

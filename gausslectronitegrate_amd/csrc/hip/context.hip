@@ -310,6 +310,10 @@ int gli_set_tuning(const char* knob, int value)
 			GLI_REQUIRE(value >= 0 && value <= 2, "gather_bucketed must be 0, 1 or 2");
 			c.gather_bucketed = value;
 		}
+		else if (k == "gather_classed") {
+			GLI_REQUIRE(value == 0 || value == 1, "gather_classed must be 0 or 1");
+			c.gather_classed = value;
+		}
 		else if (k == "project_chunk_kib") {
 			GLI_REQUIRE(value >= 1, "project_chunk_kib must be at least 1");
 			c.project_chunk_kib = value;
@@ -341,6 +345,7 @@ int gli_get_tuning(const char* knob)
 		if (k == "gemm_shape") { return c.gemm_shape; }
 		if (k == "project_chunk_kib") { return c.project_chunk_kib; }
 		if (k == "gather_bucketed") { return c.gather_bucketed; }
+		if (k == "gather_classed") { return c.gather_classed; }
 		GLI_REQUIRE(false, "unknown tuning knob");
 	}
 	catch (const Failure&) {

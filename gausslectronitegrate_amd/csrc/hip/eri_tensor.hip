@@ -1045,6 +1045,149 @@ __global__ __launch_bounds__(kThreads) void k_gather_bucketed(GatherDesc d, int6
 }
 
 // ---------------------------------------------------------------------------
+// Single-call gather with each wave's key-tensor reads grouped by line class.
+//
+// A wave takes 8 groups of 64 consecutive elements (512 elements) and decodes their key-tensor
+// positions as k_gather does.  Its 8 key-tensor loads then take the elements class by class, with
+// class = (position / 16) mod 8, the residue of the element's 128-byte line: elements that share a
+// line share a class, so they fall into one 64-lane load instead of being spread over eight.  Over
+// the 9^3 list the 8 loads of a wave touch 45 distinct lines each in list order and 29 grouped by
+// class (a full sort of the 512 would give 25).  The permutation is a counting sort inside the
+// wave, with no workgroup barrier: per-lane class counts in nibbles, a DPP prefix sum over the
+// lanes, and one 8-byte LDS slot per element that holds (position, place in the wave).  The values
+// return to their list places through the same slots and leave in coalesced stores.  Each value is
+// the same key-tensor entry k_gather reads, so the results are bit-identical.
+constexpr int kClassEpt = 8;
+
+// inclusive prefix sum over the 64 lanes (row shifts inside each row of 16, then the row
+// broadcasts of lanes 15 and 31); every lane must be active
+__device__ inline unsigned wave_inclusive_sum(unsigned v)
+{
+	v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);   // row_shr:1
+	v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);   // row_shr:2
+	v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);   // row_shr:4
+	v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);   // row_shr:8
+	v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+	v += (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+	return v;
+}
+
+// LDS bytes of the classed gather for workgroups of `threads` lanes: the key tables (when kept in
+// LDS), then 8 bytes per element of every wave
+__host__ __device__ inline int classed_table_bytes(int nlut, int C1)
+{
+	return (gather_lds_bytes(nlut, C1) + 15) & ~15;
+}
+__host__ __device__ inline int classed_lds_bytes(int threads, int nlut, int C1, bool lds)
+{
+	return (lds ? classed_table_bytes(nlut, C1) : 0) + threads * kClassEpt * 8;
+}
+
+// one wave's 512 elements; slots: the wave's 512 LDS slots
+__device__ inline void gather_wave_classed(const GatherDesc& d, int64_t wave_base, const short* __restrict__ lut,
+                                           const uint32_t* __restrict__ tet3, uint2* slots)
+{
+	constexpr int E = kClassEpt;
+	const int lane = threadIdx.x & 63;
+	const double* __restrict__ V = d.V;
+	double v[E];
+	wave_positions<E>(d, wave_base, lut, tet3,
+		[&](const uint32_t* p) {
+			// class of each element, its rank among the lane's earlier elements of that class, and the
+			// lane's count per class (nibbles: at most 8 per class)
+			unsigned cl[E], rk[E], cnt = 0;
+#pragma unroll
+			for (int u = 0; u < E; ++u) {
+				cl[u] = (p[u] >> 4) & 7u;
+				rk[u] = (cnt >> (4 * cl[u])) & 15u;
+				cnt += 1u << (4 * cl[u]);
+			}
+			// counts of classes 2r and 2r + 1 in the 16-bit halves of own[r]; the wave totals stay
+			// <= 512, so the halves never carry into each other
+			unsigned own[4], ex[4];
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				own[r] = ((cnt >> (8 * r)) & 15u) | (((cnt >> (8 * r + 4)) & 15u) << 16);
+				ex[r] = wave_inclusive_sum(own[r]);
+			}
+			// class starts: exclusive prefix of the wave's class totals (lane 63's inclusive sums)
+			unsigned run = 0;
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const unsigned tot = (unsigned)__builtin_amdgcn_readlane((int)ex[r], 63);
+				const unsigned lo = tot & 0xffffu;
+				ex[r] = ex[r] - own[r] + (run | ((run + lo) << 16));
+				run += lo + (tot >> 16);
+			}
+#pragma unroll
+			for (int u = 0; u < E; ++u) {
+				const unsigned c = cl[u];
+				const unsigned w01 = (c & 2u) ? ex[1] : ex[0];
+				const unsigned w23 = (c & 2u) ? ex[3] : ex[2];
+				const unsigned w = (c & 4u) ? w23 : w01;
+				const unsigned slot = (((w >> (16 * (c & 1u))) & 0xffffu) + rk[u]) & (unsigned)(E * 64 - 1);
+				slots[slot] = make_uint2(p[u], (unsigned)(u * 64 + lane));
+			}
+			// a wave's LDS operations complete in issue order; the barrier keeps the compiler from
+			// moving the reads above the writes of other lanes' slots
+			__builtin_amdgcn_wave_barrier();
+			uint2 q[E];
+#pragma unroll
+			for (int u = 0; u < E; ++u) { q[u] = slots[u * 64 + lane]; }
+#pragma unroll
+			for (int u = 0; u < E; ++u) { v[u] = V[q[u].x]; }
+			__builtin_amdgcn_wave_barrier();
+			double* vals = (double*)slots;
+#pragma unroll
+			for (int u = 0; u < E; ++u) { vals[q[u].y & (unsigned)(E * 64 - 1)] = v[u]; }
+			__builtin_amdgcn_wave_barrier();
+#pragma unroll
+			for (int u = 0; u < E; ++u) { v[u] = vals[u * 64 + lane]; }
+		},
+		[&](const int64_t* vi) {
+#pragma unroll
+			for (int u = 0; u < E; ++u) { v[u] = (vi[u] >= 0) ? V[vi[u]] : __builtin_nan(""); }
+		});
+	if (d.scale != nullptr) {
+		const double sc = *d.scale;
+#pragma unroll
+		for (int u = 0; u < E; ++u) { v[u] *= sc; }
+	}
+#pragma unroll
+	for (int u = 0; u < E; ++u) {
+		const int64_t e = wave_base + u * 64 + lane;
+		if (e < d.num) { d.out[e] = v[u]; }
+	}
+}
+
+// one workgroup of blockDim.x lanes per piece of 8 blockDim.x consecutive elements; the wave slots
+// follow the key tables in dynamic LDS
+template <bool kLds>
+__global__ __launch_bounds__(1024) void k_gather_classed(GatherDesc d, int64_t xcd_chunks)
+{
+	extern __shared__ __align__(16) uint32_t gather_lds[];
+	const int64_t piece = gather_piece(xcd_chunks);
+	if (piece * (blockDim.x >> 6) * (64 * kClassEpt) >= d.num) { return; }   // whole workgroup idle
+	const int64_t wave_base = (piece * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * kClassEpt);
+	const short* lut = d.lut;
+	const uint32_t* tet3 = d.tet3;
+	int table_bytes = 0;
+	if (kLds) {
+		uint32_t* st = gather_lds;
+		short* sl = (short*)(gather_lds + ((d.C1 + 3) & ~3));
+		for (int q = threadIdx.x; q < d.C1; q += blockDim.x) { st[q] = d.tet3[q]; }
+		for (int q = threadIdx.x; q < d.nlut; q += blockDim.x) { sl[q] = d.lut[q]; }
+		__syncthreads();
+		lut = sl;
+		tet3 = st;
+		table_bytes = classed_table_bytes(d.nlut, d.C1);
+	}
+	if (wave_base >= d.num) { return; }
+	uint2* slots = (uint2*)((unsigned char*)gather_lds + table_bytes) + (threadIdx.x >> 6) * (64 * kClassEpt);
+	gather_wave_classed(d, wave_base, lut, tet3, slots);
+}
+
+// ---------------------------------------------------------------------------
 // per-element FP64 path: the reference's binned sum (eri_integrals.c:608-669), one workgroup per
 // element, no reuse across elements.  On the folded columns B[.][kx], B[.][ky], B[.][kz]
 // (t -> |t| is an exact regrouping of the reference's terms):
@@ -1489,7 +1632,35 @@ static void launch_gather_bucketed(const GatherDesc& d, int xcd, int threads)
 	GLI_HIP(hipGetLastError());
 }
 
-// returns the phase name of the kernel it launched: "gather" (k_gather) or "gather_bucketed"
+// the classed gather's largest workgroup LDS (gfx950: 160 KB per workgroup)
+constexpr int kClassedLdsMax = 160 * 1024;
+
+static void launch_gather_classed(const GatherDesc& d, int xcd, int lds)
+{
+	const int threads = ctx().gather_threads;
+	const int64_t pieces = ceil_div(d.num, (int64_t)threads * kClassEpt);
+	if (pieces == 0) { return; }
+	const int64_t chunks = xcd ? ceil_div(pieces, 8) : 0;
+	const int64_t blocks = xcd ? 8 * chunks : pieces;
+	GLI_REQUIRE(blocks < 2147483647, "too many elements for one gather launch");
+	const bool use_lds = lds && gather_lds_fits(d);
+	const int shm = classed_lds_bytes(threads, d.nlut, d.C1, use_lds);
+	GLI_REQUIRE(shm <= kClassedLdsMax, "classed gather: workgroup LDS above 160 KB");
+	// dynamic LDS above 64 KB needs the kernel's limit raised once
+	static bool raised[2] = { false, false };
+	const void* fn = use_lds ? (const void*)k_gather_classed<true> : (const void*)k_gather_classed<false>;
+	if (shm > 64 * 1024 && !raised[use_lds ? 1 : 0]) {
+		GLI_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kClassedLdsMax));
+		raised[use_lds ? 1 : 0] = true;
+	}
+	hipStream_t st = ctx().stream;
+	if (use_lds) { k_gather_classed<true><<<(unsigned)blocks, threads, (size_t)shm, st>>>(d, chunks); }
+	else { k_gather_classed<false><<<(unsigned)blocks, threads, (size_t)shm, st>>>(d, chunks); }
+	GLI_HIP(hipGetLastError());
+}
+
+// returns the phase name of the kernel it launched: "gather" (k_gather), "gather_bucketed" or
+// "gather_classed"
 static const char* launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 {
 	// window-bucketed reads: on with knob 1; with knob 2 (automatic) for key tensors above 128 MB,
@@ -1499,6 +1670,10 @@ static const char* launch_gather(const GatherDesc& d, int ept, int xcd, int lds)
 	if (bucketed && gather_bucketed_fits(d, ctx().gather_threads)) {
 		launch_gather_bucketed(d, xcd, ctx().gather_threads);
 		return "gather_bucketed";
+	}
+	if (ctx().gather_classed && d.dc.uniform_pairs) {
+		launch_gather_classed(d, xcd, lds);
+		return "gather_classed";
 	}
 	const int threads = ctx().gather_threads;
 	const int64_t pieces = ceil_div(d.num, (int64_t)threads * ept);

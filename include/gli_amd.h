@@ -67,6 +67,9 @@ int gli_synchronize(void);
  *                 the window's key-tensor positions by bucket in LDS and reads the key tensor in that order
  *                 (the elements of one cache line share a load), storing the values back in list order:
  *                 0 never, 1 always, 2 (default) when the key tensor exceeds 128 MB (13^3: 299 MB)
+ *   "gather_classed" 1 (default): where the bucketed form is not chosen, each wave of the single-grid gather
+ *                 (8 groups of 64 elements) issues its 8 key-tensor loads class by class, class = the
+ *                 element's 128-byte line mod 8, so the elements of one line share a load; 0: list order
  *   "project_chunk_kib" host memory (KiB, default 262144 = 256 MB) of one chunk of product states in
  *                 project_sparse_eri_gausslet_integrals; the states and as many results are held per chunk
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
@@ -77,8 +80,9 @@ int gli_get_tuning(const char* knob);
  * evaluation records its phase boundaries (no synchronisation between evaluations);
  * gli_phase_times returns, per phase name, the average duration in ms over the evaluations since
  * timing was enabled or last queried, and starts a new average.  The key-tensor method records
- * "tables", "kernel_tensor", "mode1", "mode2", "mode3" and the gather as "gather" (k_gather) or
- * "gather_bucketed" (the window-bucketed form); the per-element method records "direct". */
+ * "tables", "kernel_tensor", "mode1", "mode2", "mode3" and the gather as "gather" (k_gather),
+ * "gather_classed" (line classes per wave) or "gather_bucketed" (the window-bucketed form); the
+ * per-element method records "direct". */
 int gli_enable_phase_timing(int on);
 int gli_phase_times(double* ms, const char** names, int max);
 

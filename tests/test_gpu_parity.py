@@ -503,7 +503,7 @@ def test_tuning_variants_bit_identical(L):
     shuffled = idx[rng.permutation(n)][: n - 77]
     knobs = [(b"tile_order", (0, 1)), (b"gather_order", (0, 1, 2)), (b"gather_ept", (2, 4, 8, 16)), (b"gather_lds", (0, 1)),
              (b"gemm_shape", (0, 3, 4, 5)), (b"mode1_half", (0, 1)), (b"gather_threads", (256, 512, 1024)),
-             (b"gather_bucketed", (0, 1))]
+             (b"gather_bucketed", (0, 1)), (b"gather_classed", (0, 1))]
     saved = {k: L.gli_get_tuning(k) for k, _ in knobs}
     try:
         ref, ref_s = {}, {}
@@ -513,8 +513,10 @@ def test_tuning_variants_bit_identical(L):
             half = (to + lds + shape + ept // 4) % 2   # the half-stored first mode: both values with every shape
             threads = (256, 512, 1024)[(to + go + shape + ept) % 3]
             bucketed = (go + lds + ept // 4) % 2   # window-bucketed key-tensor reads: both with every gather order
+            classed = (to + go + lds) % 2   # line-class grouped reads: both with every gather order and table place
             for k, v in zip((b"tile_order", b"gather_order", b"gather_ept", b"gather_lds", b"gemm_shape", b"mode1_half",
-                             b"gather_threads", b"gather_bucketed"), (to, go, ept, lds, shape, half, threads, bucketed)):
+                             b"gather_threads", b"gather_bucketed", b"gather_classed"),
+                            (to, go, ept, lds, shape, half, threads, bucketed, classed)):
                 assert L.gli_set_tuning(k, v) == 0
             for method in (0, 1):
                 got = eri_values_device(c, sym(7), idx, 1e-16, method)
@@ -531,6 +533,7 @@ def test_tuning_variants_bit_identical(L):
         assert L.gli_set_tuning(b"gather_threads", 128) == -1
         assert L.gli_set_tuning(b"gather_order", 3) == -1
         assert L.gli_set_tuning(b"gather_bucketed", 3) == -1
+        assert L.gli_set_tuning(b"gather_classed", 2) == -1
         assert L.gli_set_tuning(b"batch_positions", -1) == -1
         assert L.gli_get_tuning(b"batch_sorted") in (0, 1)
         assert L.gli_set_tuning(b"no_such_knob", 1) == -1
