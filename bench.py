@@ -32,9 +32,9 @@ from gausslectronitegrate_amd.coeffs import synthetic_coefficients  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
-METHODS = {"tensor": 0, "mixed": 1, "direct": 2, "bf16x2": 3, "fp32": 4, "fp32x2": 5}
+METHODS = {"tensor": 0, "mixed": 1, "direct": 2, "bf16x2": 3, "fp32": 4, "fp32x2": 5, "separable": 6}
 DTYPES = {"tensor": "f64", "mixed": "f64+split-bf16", "direct": "f64", "bf16x2": "bf16x2+f32acc", "fp32": "f32",
-          "fp32x2": "f32x2+f32acc"}
+          "fp32x2": "f32x2+f32acc", "separable": "f64"}
 
 
 def log(*a):
@@ -220,18 +220,32 @@ BATCH = [(5, 29, 0, 0.5), (7, 43, 0, 0.75), (9, 57, 1, 1.0), (11, 57, 0, 1.5)]
 
 
 def mixed_vs_fp64(compare, method, value):
-    """the split-precision (mixed) method's rate against the FP64 key-tensor method's on the same step,
-    when both were timed in this run"""
-    if method != "tensor":
+    """the fast method's rate against the exact FP64 key-tensor method's (tensor) on the same step, and the
+    split-precision (mixed) method's, when they were timed in this run"""
+    by = {c["method"]: c for c in compare}
+    if method == "tensor":
+        fp64, fast, fast_value = None, None, None
+        fp64_value = value
+    elif "tensor" in by:
+        fp64 = by["tensor"]
+        fp64_value = fp64["value"]
+        fast, fast_value = method, value
+    else:
         return None
-    for c in compare:
-        if c["method"] == "mixed":
-            return {"ratio": c["value"] / value, "mixed_elements_per_s": c["value"], "fp64_elements_per_s": value,
-                    "max_abs_err_vs_fp64": c["max_abs_err_vs_tensor_all_elements"],
-                    "note": "the step is bound by the per-element key-tensor gather (identical in both methods); the "
-                            "contraction is at most a fifth of it, and on MI355X FP64 MFMA (78.6 TF/s) is faster "
-                            "than the 15 split BF16 slice products with their digit passes"}
-    return None
+    out = {"fp64_exact_elements_per_s": fp64_value}
+    if fast is not None:
+        out.update({"fast_method": fast, "ratio": fast_value / fp64_value, "fast_elements_per_s": fast_value,
+                    "max_abs_err_vs_fp64": fp64["max_abs_err_vs_%s_all_elements" % method],
+                    "fast_note": "FP64 throughout: the Coulomb kernel T[i] = 6 erf(sqrt(i/8))/sqrt(i) written as a "
+                                 "Gauss-Legendre sum of separable Gaussians (within 2e-15 relative of the table), so the "
+                                 "key tensor is one rank-Q product instead of three mode products"})
+    if "mixed" in by:
+        m = by["mixed"]
+        out.update({"mixed_ratio": m["value"] / fp64_value, "mixed_elements_per_s": m["value"],
+                    "mixed_max_abs_err_vs_%s" % method: m["max_abs_err_vs_%s_all_elements" % method],
+                    "mixed_note": "the split BF16 products with their digit passes are slower than FP64 MFMA "
+                                  "(78.6 TF/s) on MI355X; the step is bound by the per-element gather"})
+    return out
 
 
 def batch_measure(L, args, stream):
@@ -239,7 +253,7 @@ def batch_measure(L, args, stream):
     scaled values in HBM, as one replay of the plan's captured graph.  The error is taken over every
     element against the single-grid FP64 path (times 1/a).  Returns the measurement as a dict."""
     import torch
-    method = METHODS[args.method] if args.method in ("tensor", "mixed") else 0
+    method = METHODS[args.method] if args.method in ("tensor", "mixed", "separable") else 0
     n_g = len(BATCH)
     coeffs, grids, idxs, nums, vals = [], [], [], [], []
     enum_ms = 0.0
@@ -307,7 +321,7 @@ def batch_measure(L, args, stream):
     L.gli_eri_batch_destroy(plan)
     for p in idxs:
         L.gli_device_free(p)
-    return {"workload": "eri_batch_5-7-9-11", "grids": BATCH, "method": "tensor" if method == 0 else "mixed",
+    return {"workload": "eri_batch_5-7-9-11", "grids": BATCH, "method": {0: "tensor", 1: "mixed", 6: "separable"}[method],
             "unique_elements": total, "ms_per_step": step_ms, "value": total / (step_ms / 1e3), "unit": "elements/s",
             "steps": args.steps, "plan_create_ms": create_ms, "enumeration_ms": enum_ms, "autotune": tuned,
             "errors": errs}
@@ -473,15 +487,18 @@ def main():
     ap.add_argument("--K", type=int, default=57)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-16)
-    ap.add_argument("--method", choices=sorted(METHODS), default="tensor")
+    ap.add_argument("--method", choices=sorted(METHODS), default="separable",
+                    help="method of the timed step: separable (default; FP64, the Coulomb kernel as a Gauss-Legendre "
+                         "sum of separable Gaussians, within ~1e-15 of tensor), tensor (the exact FP64 re-association), "
+                         "mixed (split BF16), direct (per element) or a comparison method")
     ap.add_argument("--direct-sample", type=int, default=65536)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--asymmetric", action="store_true",
                     help="coefficients without mirror symmetry (general axis-key set, GEMM-dominated)")
-    ap.add_argument("--compare", default="mixed",
+    ap.add_argument("--compare", default="tensor,mixed",
                     help="comma-separated methods timed beside the main one, each with its error against the main "
-                         "FP64 values over every element (e.g. mixed,bf16x2,fp32,fp32x2)")
+                         "method's values over every element (e.g. tensor,mixed,bf16x2,fp32,fp32x2)")
     ap.add_argument("--no-compare", action="store_true", help="time no other method")
     ap.add_argument("--no-host-api", action="store_true",
                     help="skip timing the reference entry points with host arrays (PCIe included)")
@@ -645,7 +662,7 @@ def main():
         vptr = (C.c_void_p * 1)(d_pl.ptr.value)
         t0 = time.perf_counter()
         _lib.check(L.gli_eri_batch_create(1, cptr, Ks.ctypes.data_as(_abi.P_glong), g.ctypes.data_as(_abi.P_glong), dptr,
-                                          nn.ctypes.data_as(_abi.P_glong), args.tol, method if method in (0, 1) else 0,
+                                          nn.ctypes.data_as(_abi.P_glong), args.tol, method if method in (0, 1, 6) else 0,
                                           vptr, 1, C.byref(plan)), "gli_eri_batch_create")
         create_ms = (time.perf_counter() - t0) * 1e3
         buf = C.create_string_buffer(1 << 16)

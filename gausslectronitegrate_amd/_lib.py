@@ -60,6 +60,7 @@ EXTRA = {
     "gli_eri_batch_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "gli_eri_batch_destroy": (None, [C.c_void_p]),
     "gli_eri_key_count": (C.c_int, [C.c_int64, C.c_int]),
+    "gli_eri_separable_rule": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "gli_eri_sparse_key": (C.c_int64, [C.POINTER(_abi.Grid), _abi.P_glong * 48, C.POINTER(_abi.GridPoint)]),
     "gli_erida_sparse_key": (C.c_int64, [C.POINTER(_abi.Grid), _abi.P_glong * 48, C.POINTER(_abi.GridPoint)]),
     "gli_release_workspace": (None, []),
@@ -72,6 +73,7 @@ EXTRA = {
 ERI_TENSOR_FP64 = 0
 ERI_TENSOR_MIXED = 1
 ERI_DIRECT_FP64 = 2
+ERI_SEPARABLE_FP64 = 6
 
 
 def lib():

@@ -240,6 +240,9 @@ struct ModeArgs {
 	// with sym_x = 1 reads X[m2][m3][k] for m2 > m3 from X[m3][m2][k] (X rows m3 * Cp + k)
 	int half_n = 0;
 	int sym_x = 0;
+	// rank-Q form of the last product (separable method): X[q][k1 Cp + k2] = cp_u[q Cp + k1] X[q ldx + k2],
+	// with ldx = Cp; the rows of a tile share k1
+	const double* cp_u = nullptr;
 };
 constexpr int kGemmShapes = 6;
 bool gemm_shape_fits(int shape, int Cp, int tri, int64_t p_base);

@@ -142,7 +142,7 @@ __host__ __device__ constexpr int shape_bm(int s) { return (s == 1 || s == 3 || 
 __host__ __device__ constexpr int shape_bn(int s) { return (s == 2 || s == 3) ? 128 : 64; }
 
 
-template <int BM, int BN, int GBK>
+template <int BM, int BN, int GBK, bool kCp = false>
 __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, double (*As)[BM], double (*Bs)[BN])
 {
 	constexpr int TM = BM / 32, TN = BN / 32;
@@ -192,6 +192,10 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 	// stay in flight during this slice's MFMAs.
 	const int ldx = (int)a.ldx, ldb = (int)a.ldb, p0i = (int)p0, Pi = (int)a.P;
 	const int m3i = (int)sym_m3, sym_base = (int)(sym_m3 * a.ldx + sym_k);
+	// rank-Q form: the tile's rows p = k1 Cp + k2 share k1; X[q][p] = cp_u[q Cp + k1] X[q Cp + k2]
+	const int cp_k1 = kCp ? (int)(p0 / a.Cp) : 0;
+	const int cp_off = kCp ? (int)(p0 - (int64_t)cp_k1 * a.Cp) : 0;
+	double su[kCp ? PA : 1];
 	bool oka[PA], okb[PB];
 	auto gload = [&](int mu0) {
 #pragma unroll
@@ -200,6 +204,11 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 			const int mu = id / BM, p = id % BM;
 			const int m = mu0 + mu;
 			oka[r] = m < a.Kd && p0i + p < Pi;
+			if (kCp) {
+				ra[r] = a.X[oka[r] ? (unsigned)(m * ldx + cp_off + p) : 0u];
+				su[kCp ? r : 0] = a.cp_u[oka[r] ? (unsigned)(m * a.Cp + cp_k1) : 0u];
+				continue;
+			}
 			const bool mirror = a.sym_x && m > m3i;
 			const int off = (mirror ? sym_base + m * a.Cp : m * ldx + p0i) + p;
 			ra[r] = a.X[oka[r] ? (unsigned)off : 0u];
@@ -217,7 +226,7 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 #pragma unroll
 		for (int r = 0; r < PA; ++r) {
 			const int id = tid + 256 * r;
-			As[id / BM][id % BM] = oka[r] ? ra[r] : 0.0;
+			As[id / BM][id % BM] = oka[r] ? (kCp ? ra[r] * su[kCp ? r : 0] : ra[r]) : 0.0;
 		}
 #pragma unroll
 		for (int r = 0; r < PB; ++r) {
@@ -291,14 +300,14 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 	}
 }
 
-template <int BM, int BN, int GBK>
+template <int BM, int BN, int GBK, bool kCp = false>
 __global__ __launch_bounds__(256) void k_mode_product(ModeArgs a, TileOrder order)
 {
 	__shared__ double As[GBK][BM];
 	__shared__ double Bs[GBK][BN];
 	int rt, ct;
 	if (!order.tile(blockIdx.x, rt, ct)) { return; }
-	mode_product_tile<BM, BN, GBK>(a, rt, ct, As, Bs);
+	mode_product_tile<BM, BN, GBK, kCp>(a, rt, ct, As, Bs);
 }
 
 // FP32 variants of the mode product (v_mfma_f32_16x16x4f32, FP32 accumulation over the whole mu
@@ -406,7 +415,7 @@ struct ModeGroup {
 	int n;
 };
 
-template <int BM, int BN, int GBK>
+template <int BM, int BN, int GBK, bool kCp = false>
 __global__ __launch_bounds__(256) void k_mode_product_grouped(ModeGroup grp)
 {
 	__shared__ double As[GBK][BM];
@@ -415,7 +424,7 @@ __global__ __launch_bounds__(256) void k_mode_product_grouped(ModeGroup grp)
 	while (g + 1 < grp.n && (int)blockIdx.x >= grp.block_begin[g + 1]) { ++g; }
 	int rt, ct;
 	if (!grp.order[g].tile((int)blockIdx.x - grp.block_begin[g], rt, ct)) { return; }
-	mode_product_tile<BM, BN, GBK>(grp.a[g], rt, ct, As, Bs);
+	mode_product_tile<BM, BN, GBK, kCp>(grp.a[g], rt, ct, As, Bs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1429,6 +1438,10 @@ static KeySet& key_set(int R, bool symmetric)
 
 struct EriWorkspace {
 	DevBuf<double> coeffs, f, P, B, Y1, Y2, V;
+	// separable method: the Gaussian factors [g | W g] of the folded range sep_M (Mp x 2 Qp, cached)
+	// and the rank-Q factors [G; U] of the current evaluation (2 Qp x Cp)
+	DevBuf<double> sep_gw, GU;
+	int sep_M = -1, sep_Qp = 0;
 	DevBuf<int> coords;
 	DevBuf<double> T3;
 	int T3_M = -1;
@@ -1817,7 +1830,12 @@ static void mode_product_f32(const ModeArgs& a, bool split)
 void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped)
 {
 	GLI_REQUIRE(n >= 1 && n <= kMaxGroup, "too many problems in one grouped mode product");
-	for (int g = 0; g < n; ++g) { check_mode_extent(a[g]); }
+	const bool cp = a[0].cp_u != nullptr;
+	for (int g = 0; g < n; ++g) {
+		check_mode_extent(a[g]);
+		GLI_REQUIRE((a[g].cp_u != nullptr) == cp && (!cp || (shape == 0 && a[g].ldx == a[g].Cp && a[g].p_base == 0)),
+		            "grouped rank-Q products: every problem rank-Q, 64x64 tiles, rows k1 Cp + k2");
+	}
 	ModeGroup grp;
 	grp.n = n;
 	int blocks = 0;
@@ -1835,6 +1853,11 @@ void mode_product_grouped(const ModeArgs* a, int n, int shape, int grouped)
 	grp.block_begin[n] = blocks;
 	if (blocks == 0) { return; }
 	hipStream_t st = ctx().stream;
+	if (cp) {
+		k_mode_product_grouped<64, 64, 16, true><<<(unsigned)blocks, 256, 0, st>>>(grp);
+		GLI_HIP(hipGetLastError());
+		return;
+	}
 	switch (shape) {
 		case 1: k_mode_product_grouped<128, 64, 16><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
 		case 2: k_mode_product_grouped<64, 128, 16><<<(unsigned)blocks, 256, 0, st>>>(grp); break;
@@ -1868,6 +1891,135 @@ static void direct_representations(EriWorkspace& w, int M)
 	GLI_HIP(hipMemcpy(w.rep_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
 	GLI_HIP(hipMemcpy(w.rep_list.p, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 	w.rep_M = M;
+}
+
+// ---------------------------------------------------------------------------
+// Separable method.  With d = sqrt(i) / 6 and w = 1/3 the Coulomb table is
+//     T[i] = erf(d / (sqrt(2) w)) / d = 6 erf(sqrt(i / 8)) / sqrt(i) = C int_0^1 exp(-i s^2 / 8) ds,
+// C = 6 / sqrt(2 pi) (also at i = 0), because erf(x) / x = (2 / sqrt(pi)) int_0^1 exp(-x^2 s^2) ds.
+// For i = m1^2 + m2^2 + m3^2 the integrand factorises, so a quadrature rule (s_q, w_q) gives
+//     T3[m1][m2][m3] ~ sum_q W_q g_q[m1] g_q[m2] g_q[m3],  g_q[m] = exp(-m^2 s_q^2 / 8),  W_q = C w_q,
+// and the key tensor becomes V[k1][k2][k3] = sum_q U[q][k1] G[q][k2] G[q][k3] with G = g^T B and
+// U = (W g)^T B: one rank-Q product in place of the three mode products.  The rule is composite
+// Gauss-Legendre on geometric panels [0, r^J], [r^J, r^(J-1)], ..., [r, 1] with r = 0.35, 12 nodes on
+// the first panel and 16 on each other, and J the smallest with r^J <= 1 / (1.5 sqrt(3 M^2)): every
+// integrand exp(-i s^2 / 8), i <= 3 M^2, then varies by a bounded amount per panel.  Q = 108 nodes for
+// the folded ranges of 3^3..13^3 grids with K = 57 (M = 124..184); against 6 erf(sqrt(i/8)) / sqrt(i)
+// the rule is within 2e-15 relative for every i <= 3 M^2 and every M from 4 to 263.
+constexpr long double kSepRatio = 0.35L;
+constexpr int kSepNodesFirst = 12, kSepNodesPanel = 16;
+
+// Gauss-Legendre nodes x and weights w on [-1, 1]: Newton iterations on P_p from the usual cosine
+// guesses, in extended precision
+static void gauss_legendre(int p, long double* x, long double* w)
+{
+	const long double pi = 3.141592653589793238462643383279502884L;
+	for (int i = 0; i < p; ++i) {
+		long double z = cosl(pi * (i + 0.75L) / (p + 0.5L)), dp = 1;
+		for (int it = 0; it < 100; ++it) {
+			long double p0 = 1, p1 = z;   // P_{k-1}(z), P_k(z)
+			for (int k = 2; k <= p; ++k) {
+				const long double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+				p0 = p1;
+				p1 = p2;
+			}
+			dp = p * (z * p1 - p0) / (z * z - 1);   // P_p'(z)
+			const long double dz = p1 / dp;
+			z -= dz;
+			if (fabsl(dz) <= 1e-19L) { break; }
+		}
+		x[i] = z;
+		w[i] = 2 / ((1 - z * z) * dp * dp);
+	}
+}
+
+// nodes s_q in (0, 1) and weights w_q of the composite rule for a folded range M
+void separable_rule(int M, std::vector<long double>& s, std::vector<long double>& w)
+{
+	const long double bound = 1.0L / (1.5L * sqrtl(3.0L * (long double)std::max(M, 1) * std::max(M, 1)));
+	int J = 0;
+	long double lo = 1.0L;
+	while (lo > bound) {
+		lo *= kSepRatio;
+		++J;
+	}
+	long double xf[kSepNodesFirst], wf[kSepNodesFirst], xp[kSepNodesPanel], wp[kSepNodesPanel];
+	gauss_legendre(kSepNodesFirst, xf, wf);
+	gauss_legendre(kSepNodesPanel, xp, wp);
+	s.clear();
+	w.clear();
+	long double a = 0, b = lo;
+	for (int panel = 0; panel <= J; ++panel) {
+		const int p = panel == 0 ? kSepNodesFirst : kSepNodesPanel;
+		const long double* x = panel == 0 ? xf : xp;
+		const long double* wx = panel == 0 ? wf : wp;
+		for (int i = 0; i < p; ++i) {
+			s.push_back((b - a) / 2 * x[i] + (a + b) / 2);
+			w.push_back((b - a) / 2 * wx[i]);
+		}
+		a = b;
+		b = (panel + 1 == J) ? 1.0L : b / kSepRatio;
+	}
+}
+
+// [g | W g] for a folded range M (Mp rows, 2 Qp columns, zero past m = M and q = Q), host memory
+static std::vector<double> separable_gw(int M, int Mp, int& Qp)
+{
+	std::vector<long double> sq, wq;
+	separable_rule(M, sq, wq);
+	const int Q = (int)sq.size();
+	Qp = (int)ceil_div(Q, 16) * 16;
+	const long double C = 6.0L / sqrtl(2.0L * 3.141592653589793238462643383279502884L);
+	std::vector<double> gw((size_t)Mp * 2 * Qp, 0.0);
+	for (int m = 0; m <= M && m < Mp; ++m) {
+		for (int q = 0; q < Q; ++q) {
+			const long double g = expl(-(long double)m * m * sq[(size_t)q] * sq[(size_t)q] / 8);
+			gw[(size_t)m * 2 * Qp + q] = (double)g;
+			gw[(size_t)m * 2 * Qp + Qp + q] = (double)(C * wq[(size_t)q] * g);
+		}
+	}
+	return gw;
+}
+
+// [g | W g] for the folded range of s, cached on the device per M
+static void separable_factors(EriWorkspace& w, const EriSetup& s)
+{
+	if (w.sep_M == s.M) { return; }
+	int Qp = 0;
+	const std::vector<double> gw = separable_gw(s.M, s.Mp, Qp);
+	GLI_HIP(hipStreamSynchronize(ctx().stream));   // an earlier evaluation may still read the old factors
+	w.sep_gw.alloc(gw.size());
+	GLI_HIP(hipMemcpy(w.sep_gw.p, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
+	w.sep_M = s.M;
+	w.sep_Qp = Qp;
+}
+
+// V[k1 <= k2 <= k3] = sum_q U[q][k1] G[q][k2] G[q][k3], tetrahedral store (the rank-Q tile kernel)
+static void mode_product_cp(const ModeArgs& a)
+{
+	if (a.P <= 0 || a.C <= 0) { return; }
+	check_mode_extent(a);
+	GLI_REQUIRE(a.cp_u != nullptr && a.ldx == a.Cp && a.p_base == 0 && gemm_shape_fits(0, a.Cp, 1, 0),
+	            "rank-Q product: rows must be k1 Cp + k2 with the key padding a multiple of the tile");
+	const TileOrder o = tile_order_for(0, a.P, a.C, tile_order_grouped());
+	k_mode_product<64, 64, 16, true><<<(unsigned)o.blocks(), 256, 0, ctx().stream>>>(a, o);
+	GLI_HIP(hipGetLastError());
+}
+
+// The key tensor of the separable method: the rank-Q factors G and U in one product of [g | W g]
+// with the folded columns B, then the rank-Q product into V.  Queued on the library stream.
+static void key_tensor_separable(const EriSetup& s, EriWorkspace& w)
+{
+	separable_factors(w, s);
+	const int64_t Qp = w.sep_Qp, Cp = s.Cp;
+	w.GU.reserve((size_t)(2 * Qp * Cp));
+	// [G; U] = [g | W g]^T B: an FP64 MFMA product with 2 Qp output rows (~30 MFLOP at 11^3)
+	mode_product(w.sep_gw.p, 2 * Qp, 2 * Qp, w.B.p, Cp, s.Cp, s.Mp, w.GU.p, Cp, s.Cp, 0, 0, 0, 0);
+	phase_mark("separable_factors");
+	ModeArgs a{w.GU.p, Cp, Cp * Cp, w.GU.p, Cp, s.Cp, (int)Qp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1};
+	a.cp_u = w.GU.p + Qp * Cp;
+	mode_product_cp(a);
+	phase_mark("separable_tensor");
 }
 
 // values of device-resident four-indices into device-resident values
@@ -1911,7 +2063,7 @@ static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t gr
 {
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
-	GLI_REQUIRE(method >= GLI_ERI_TENSOR_FP64 && method <= GLI_ERI_TENSOR_FP32X2, "unknown ERI method");
+	GLI_REQUIRE(method >= GLI_ERI_TENSOR_FP64 && method <= GLI_ERI_SEPARABLE_FP64, "unknown ERI method");
 	GLI_REQUIRE(coeffs != nullptr && K >= 3, "the number of gausslet coefficients must be odd and at least 3");
 	phase_begin();
 	EriPrepared ep;
@@ -1941,6 +2093,11 @@ static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t gr
 		}
 		ep.direct_shm = 3 * (size_t)s.Mp * sizeof(double) + DIRECT_CHUNK * (sizeof(double) + sizeof(int));
 		GLI_REQUIRE(ep.direct_shm <= 65536, "grid too long for the per-element path: 6(n-1) + 2(K-1) must stay below 1700");
+		return ep;
+	}
+	if (method == GLI_ERI_SEPARABLE_FP64) {
+		w.V.reserve((size_t)tet_index(0, 0, s.C1));
+		key_tensor_separable(s, w);
 		return ep;
 	}
 	const bool mixed = (method == GLI_ERI_TENSOR_MIXED || method == GLI_ERI_TENSOR_BF16X2);
@@ -2139,6 +2296,9 @@ struct BatchGrid {
 	int64_t num = 0;
 	double* d_vals = nullptr;
 	DevBuf<double> f, P, B, Y1, Y2, V;
+	DevBuf<double> GU;      // separable plans: the rank-Q factors [G; U] (2 Qp x Cp)
+	const double* sep_gw = nullptr;   // ... and the Gaussian factors [g | W g] of the grid's folded range
+	int sep_Qp = 0;
 	DevBuf<uint32_t> pos;   // key-tensor position of every element (plans with cached positions)
 	DevBuf<uint16_t> perm;  // place of each sorted position in its window (plans with sorted windows)
 	const double* T3 = nullptr;
@@ -2159,6 +2319,7 @@ struct BatchPlan {
 	double tol = 0;
 	std::vector<std::unique_ptr<BatchGrid>> grids;
 	std::vector<std::pair<int, std::unique_ptr<DevBuf<double>>>> t3;   // kernel tensor per folded range M
+	std::vector<std::pair<int, std::unique_ptr<DevBuf<double>>>> sep_gw;   // separable plans: [g | W g] per M
 	DevBuf<double> d_params;
 	double* h_params = nullptr;
 	size_t n_params = 0;
@@ -2216,6 +2377,33 @@ static ModeArgs batch_mode_args(const BatchGrid& bg, int mode, bool half)
 		return a;
 	}
 	return ModeArgs{bg.Y2.p, Cp * Cp, Cp * Cp, bg.B.p, Cp, s.Cp, s.Mp, bg.V.p, 0, s.Cp, 1, 1, 0, s.C1};
+}
+
+// separable plans: the rank-Q product of one grid (V from its factors [G; U])
+static ModeArgs batch_sep_args(const BatchGrid& bg)
+{
+	const EriSetup& s = bg.s;
+	const int64_t Cp = s.Cp, Qp = bg.sep_Qp;
+	ModeArgs a{bg.GU.p, Cp, Cp * Cp, bg.GU.p, Cp, s.Cp, (int)Qp, bg.V.p, 0, s.Cp, 1, 1, 0, s.C1};
+	a.cp_u = bg.GU.p + Qp * Cp;
+	return a;
+}
+
+// separable plans: every grid's factors [G; U] = [g | W g]^T B in one grouped product, then one grouped
+// rank-Q product over all grids (the single-grid path's tile arithmetic, so bit-identical to it)
+static void batch_separable(BatchPlan& plan, int order)
+{
+	ModeArgs f[kMaxGroup], a[kMaxGroup];
+	int n = 0;
+	for (auto& g : plan.grids) {
+		const EriSetup& s = g->s;
+		const int64_t Qp = g->sep_Qp;
+		f[n] = ModeArgs{g->sep_gw, 2 * Qp, 2 * Qp, g->B.p, s.Cp, s.Cp, s.Mp, g->GU.p, s.Cp, s.Cp, 0, 0, 0, 0};
+		a[n] = batch_sep_args(*g);
+		++n;
+	}
+	mode_product_grouped(f, n, 0, order);
+	mode_product_grouped(a, n, 0, order);
 }
 
 // the gather of a batch plan: from the cached positions when the plan holds them
@@ -2337,7 +2525,10 @@ static void batch_compute(BatchPlan& plan, const BatchChoice& ch)
 {
 	for (auto& g : plan.grids) { launch_tables(g->s, plan.d_params.p + g->coeff_off, plan.tol, g->f.p, g->P.p, g->B.p); }
 	const bool any_split = ch.split[0] || ch.split[1] || ch.split[2];
-	if (!any_split) {
+	if (plan.method == GLI_ERI_SEPARABLE_FP64) {
+		batch_separable(plan, ch.gemm_order[2]);
+	}
+	else if (!any_split) {
 		for (int m = 0; m < 3; ++m) { batch_mode(plan, m, ch); }
 	}
 	else {
@@ -2397,7 +2588,23 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 	// a mixed plan always runs the split-precision products (the autotuner picks among their
 	// variants only), so that the method a plan reports is the arithmetic it executes
 	const bool mixed = plan.method == GLI_ERI_TENSOR_MIXED;
-	for (int m = 0; m < 3; ++m) {
+	const bool separable = plan.method == GLI_ERI_SEPARABLE_FP64;
+	if (separable) {
+		// one rank-Q product per grid in one grouped launch: only its tile order is a choice
+		double best = 1e30;
+		rep += "{\"rank_q\": [";
+		for (int order = 0; order < 2; ++order) {
+			const double t = time_ms([&] { batch_separable(plan, order); });
+			rep += std::string(order ? ", " : "") + "{\"order\": " + std::to_string(order) + ", \"ms\": " + fmt("%.4f", t) + "}";
+			if (t < best) {
+				best = t;
+				ch.gemm_order[2] = order;
+			}
+		}
+		batch_separable(plan, ch.gemm_order[2]);
+		rep += "], \"chosen_order\": " + std::to_string(ch.gemm_order[2]) + "}";
+	}
+	for (int m = 0; m < 3 && !separable; ++m) {
 		double best = 1e30;
 		rep += std::string(m ? ", " : "") + "{\"mode\": " + std::to_string(m + 1) + ", \"candidates\": [";
 		bool first = true;
@@ -2499,7 +2706,8 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	GLI_REQUIRE(n >= 1 && n <= kMaxGroup, "a batch holds 1 to 16 grids");
-	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64 || method == GLI_ERI_TENSOR_MIXED, "a batch runs the key-tensor methods (FP64 or mixed)");
+	GLI_REQUIRE(method == GLI_ERI_TENSOR_FP64 || method == GLI_ERI_TENSOR_MIXED || method == GLI_ERI_SEPARABLE_FP64,
+	            "a batch runs the key-tensor methods (FP64, mixed or separable)");
 	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
 	std::unique_ptr<BatchPlan> plan(new BatchPlan());
 	plan->method = method;
@@ -2521,9 +2729,29 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 		bg->f.alloc((size_t)s.R * (2 * s.K - 1));
 		bg->P.alloc((size_t)s.R * s.R * (4 * s.K - 3));
 		bg->B.alloc((size_t)s.Mp * s.Cp);
+		bg->V.alloc((size_t)tet_index(0, 0, s.C1));
+		if (method == GLI_ERI_SEPARABLE_FP64) {
+			// the Gaussian factors per folded range, the rank-Q factors per grid; no T3, Y1 or Y2
+			for (auto& t : plan->sep_gw) {
+				if (t.first == s.M) { bg->sep_gw = t.second->p; }
+			}
+			int Qp = 0;
+			const std::vector<double> gw = separable_gw(s.M, s.Mp, Qp);
+			if (bg->sep_gw == nullptr) {
+				std::unique_ptr<DevBuf<double>> d(new DevBuf<double>(gw.size()));
+				GLI_HIP(hipMemcpy(d->p, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
+				bg->sep_gw = d->p;
+				plan->sep_gw.emplace_back(s.M, std::move(d));
+			}
+			bg->sep_Qp = Qp;
+			bg->GU.alloc((size_t)2 * Qp * s.Cp);
+			bg->coeff_off = params;
+			params += (size_t)K[g];
+			plan->grids.push_back(std::move(bg));
+			continue;
+		}
 		bg->Y1.alloc((size_t)s.Mp * s.Mp * s.Cp);
 		bg->Y2.alloc((size_t)s.Mp * s.Cp * s.Cp);
-		bg->V.alloc((size_t)tet_index(0, 0, s.C1));
 		bool found = false;
 		for (auto& t : plan->t3) {
 			if (t.first == s.M) {
@@ -2620,7 +2848,7 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 	capture(false, &plan->graph, &plan->exec);
 	// two-branch variant: FP64 plans of two or more grids, timed against the single branch when
 	// autotuning (knob batch_branches: 0 never, 1 autotuner decides, 2 always)
-	const bool eligible = n >= 2 && !(ch.split[0] || ch.split[1] || ch.split[2]);
+	const bool eligible = n >= 2 && !(ch.split[0] || ch.split[1] || ch.split[2]) && method != GLI_ERI_SEPARABLE_FP64;
 	if (eligible && c.batch_branches > 0 && (autotune || c.batch_branches == 2)) {
 		int64_t most = -1;
 		for (int g = 0; g < n; ++g) {
@@ -2910,6 +3138,22 @@ int gli_mode_product_device(const double* X, int64_t ldx, int64_t P, const doubl
 	catch (const Failure&) {
 		return -1;
 	}
+}
+
+// the separable method's quadrature rule for a folded range M (host arithmetic only)
+int gli_eri_separable_rule(int M, double* nodes, double* weights, int max)
+{
+	if (M < 0) { return -1; }
+	std::vector<long double> sq, wq;
+	separable_rule(M, sq, wq);
+	const int Q = (int)sq.size();
+	if (nodes != nullptr && weights != nullptr) {
+		for (int q = 0; q < Q && q < max; ++q) {
+			nodes[q] = (double)sq[(size_t)q];
+			weights[q] = (double)wq[(size_t)q];
+		}
+	}
+	return Q;
 }
 
 // number of distinct folded columns (axis keys) the contraction uses on an R-point axis

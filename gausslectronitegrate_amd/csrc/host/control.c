@@ -16,13 +16,14 @@ static int method_from_env(void)
 	if (strcmp(s, "bf16x2") == 0) { return GLI_ERI_TENSOR_BF16X2; }
 	if (strcmp(s, "fp32") == 0) { return GLI_ERI_TENSOR_FP32; }
 	if (strcmp(s, "fp32x2") == 0) { return GLI_ERI_TENSOR_FP32X2; }
-	fprintf(stderr, "gausslectronitegrate: unknown GLI_ERI_METHOD '%s' (tensor | mixed | direct | bf16x2 | fp32 | fp32x2)\n", s);
+	if (strcmp(s, "separable") == 0) { return GLI_ERI_SEPARABLE_FP64; }
+	fprintf(stderr, "gausslectronitegrate: unknown GLI_ERI_METHOD '%s' (tensor | mixed | direct | bf16x2 | fp32 | fp32x2 | separable)\n", s);
 	abort();
 }
 
 int gli_set_eri_method(int method)
 {
-	if (method < GLI_ERI_TENSOR_FP64 || method > GLI_ERI_TENSOR_FP32X2) { return -1; }
+	if (method < GLI_ERI_TENSOR_FP64 || method > GLI_ERI_SEPARABLE_FP64) { return -1; }
 	g_method = method;
 	return 0;
 }

@@ -24,9 +24,13 @@ extern "C" {
 #define GLI_ERI_TENSOR_BF16X2 3 /* comparison only: BF16 hi/lo two-term split, FP32 accumulation (~1e-6) */
 #define GLI_ERI_TENSOR_FP32  4  /* comparison only: FP32 MFMA products and accumulation (~1e-7) */
 #define GLI_ERI_TENSOR_FP32X2 5 /* comparison only: FP32 hi/lo split operands (three FP32 MFMA products), FP32 accumulation */
+#define GLI_ERI_SEPARABLE_FP64 6 /* key tensor from a Gauss-Legendre expansion of the Coulomb kernel in separable
+                                  * Gaussians: one rank-Q FP64 MFMA product instead of three mode products
+                                  * (kernel reproduced to ~1e-15 relative; values within ~1e-14 of the default) */
 
 /* method used by compute_sparse_eri_gausslet_integrals / compute_eri_gausslet_integral;
- * the environment variable GLI_ERI_METHOD (tensor | mixed | direct | bf16x2 | fp32 | fp32x2) sets the initial value */
+ * the environment variable GLI_ERI_METHOD (tensor | mixed | direct | bf16x2 | fp32 | fp32x2 | separable) sets the
+ * initial value */
 int gli_set_eri_method(int method);
 int gli_get_eri_method(void);
 
@@ -192,6 +196,12 @@ int gli_mode_product_split_device(const double* X, int64_t ldx, int64_t P, const
 
 /* number of distinct folded axis columns of an R-point axis (symmetric: mirror-symmetric coefficients) */
 int gli_eri_key_count(int64_t R, int symmetric);
+
+/* Quadrature rule of the separable method (GLI_ERI_SEPARABLE_FP64) for a folded range M = 6(R-1) + 2(K-1):
+ * nodes s_q in (0, 1) and weights w_q with 6 erf(sqrt(i/8)) / sqrt(i) ~ (6 / sqrt(2 pi)) sum_q w_q exp(-i s_q^2 / 8)
+ * for 0 <= i <= 3 M^2.  Writes at most max entries (nothing when an array is null) and returns the node
+ * count Q, or -1 for M < 0.  Host arithmetic only. */
+int gli_eri_separable_rule(int M, double* nodes, double* weights, int max);
 
 /* release cached device workspace */
 void gli_release_workspace(void);

@@ -214,7 +214,7 @@ def test_eri_fill_tensor_and_single(L):
 
 def test_eri_synthetic_3_full(L):
     g = golden("golden_n3_K57s0.npz")
-    for method in (0, 2):
+    for method in (0, 2, 6):
         got = eri_values_device(g["coeffs"], sym(3), g["eri_indices"], float(g["tol"]), method)
         assert np.max(np.abs(got - g["eri_values"])) <= 1e-13
 
@@ -222,9 +222,10 @@ def test_eri_synthetic_3_full(L):
 @pytest.mark.parametrize("name,n", [("golden_sample_n5_K29s0.npz", 5), ("golden_sample_n7_K43s0.npz", 7),
                                     ("golden_sample_n9_K57s1.npz", 9), ("golden_sample_n11_K57s0.npz", 11),
                                     ("golden_sample_n13_K57s0.npz", 13)])
-@pytest.mark.parametrize("method", [0, 2])
+@pytest.mark.parametrize("method", [0, 2, 6])
 def test_eri_golden_samples(L, name, n, method):
-    """seeded canonical samples with values of the reference build (tol 1e-16)."""
+    """seeded canonical samples with values of the reference build (tol 1e-16); method 6 is the separable
+    (Gauss-Legendre) expansion of the Coulomb kernel, held to the same bound as the exact methods"""
     say("eri golden", name, "method", method)
     g = golden(name)
     got = eri_values_device(g["coeffs"], sym(n), g["eri_indices"], float(g["tol"]), method)
@@ -237,7 +238,7 @@ def test_eri_nonsymmetric_coefficients(L, orc):
     c = rng.standard_normal(21) * np.exp(-(np.arange(21) - 10) ** 2 / 60.0)
     idx = golden("golden_enum_n5.npz")["eri_indices"][::97]
     ref = orc.eri_sparse(c, sym(5), idx, 1e-12)
-    for method in (0, 2):
+    for method in (0, 2, 6):
         got = eri_values_device(c, sym(5), idx, 1e-12, method)
         assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, np.abs(ref).max())
 
@@ -250,7 +251,7 @@ def test_eri_arbitrary_grid_and_indices(L, orc):
     rng = np.random.default_rng(4)
     idx = np.unique(rng.integers(0, N ** 4, size=300))
     ref = orc.eri_sparse(c, grid, idx, 1e-10)
-    for method in (0, 2):
+    for method in (0, 2, 6):
         got = eri_values_device(c, grid, idx, 1e-10, method)
         assert np.max(np.abs(got - ref)) <= 1e-13
         bad = eri_values_device(c, grid, np.array([-1, N ** 4], dtype=np.int64), 1e-10, method)
@@ -277,6 +278,22 @@ def test_direct_flop_count_matches_oracle(L, orc, n, K, tol):
     assert 0 < executed.value
     assert np.max(np.abs(got - orc.eri_sparse(c, sym(n), idx, tol))) <= 1e-13
     assert L.gli_direct_flops(C.byref(ref_form), C.byref(executed)) == -1   # counting is off again
+
+
+@pytest.mark.parametrize("n,K,seed", [(7, 57, 0), (9, 57, 1)])
+def test_separable_full_list_vs_tensor(L, n, K, seed):
+    """the separable method (Gauss-Legendre expansion of the Coulomb kernel, one rank-Q product) against
+    the exact FP64 key-tensor method over every unique element of the grid: within 1e-14 absolute"""
+    say("separable vs tensor", n)
+    c = golden("coeffs_K%d_s%d.npy" % (K, seed))
+    d, num = eri_enumerate_device(n)
+    idx = d.to_numpy(np.int64, num)
+    exact = eri_values_device(c, sym(n), idx, 1e-16, 0)
+    sep = eri_values_device(c, sym(n), idx, 1e-16, 6)
+    err = float(np.max(np.abs(sep - exact)))
+    say("  max abs diff %.3g over %d elements" % (err, num))
+    assert err <= 1e-14
+    assert not np.array_equal(sep, exact)   # a different evaluation, not a copy of the exact one
 
 
 def test_eri_7_full_list_tensor_vs_direct_sample(L, orc):
@@ -518,7 +535,7 @@ def test_tuning_variants_bit_identical(L):
                              b"gather_threads", b"gather_bucketed", b"gather_classed"),
                             (to, go, ept, lds, shape, half, threads, bucketed, classed)):
                 assert L.gli_set_tuning(k, v) == 0
-            for method in (0, 1):
+            for method in (0, 1, 6):
                 got = eri_values_device(c, sym(7), idx, 1e-16, method)
                 got_s = eri_values_device(c, sym(7), shuffled, 1e-16, method)
                 if method not in ref:
@@ -573,12 +590,12 @@ def _batch_arrays(entries, coeffs=None):
     return cptr, Ks, grids, dptr, nums, vptr
 
 
-@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("method", [0, 1, 6])
 @pytest.mark.parametrize("branches,positions,sorted_windows", [(1, 1, 1), (2, 0, 1), (1, 1, 0)])
 def test_batch_plan_matches_single_grids(L, method, branches, positions, sorted_windows):
     """a 4-grid plan (uneven K, one non-cubic grid with an unsorted sample): every replay equals the
-    single-grid path bit for bit (FP64; the same kernels in the same summation order) or within
-    1e-10 of FP64 (mixed, whose autotuner may keep FP64 products where they are faster); output
+    single-grid path bit for bit (FP64 and separable; the same kernels in the same summation order) or
+    within 1e-10 of FP64 (mixed); output
     scales, new coefficients and a symmetry change behave as documented.  branches = 2 forces the
     two-branch graph (largest grid beside the others on a second stream) wherever it applies;
     positions = 1 caches every element's key-tensor position at plan creation, 0 decodes per replay;
@@ -611,20 +628,30 @@ def test_batch_plan_matches_single_grids(L, method, branches, positions, sorted_
         buf = C.create_string_buffer(1 << 16)
         assert L.gli_eri_batch_describe(plan, buf, len(buf)) > 0
         report = json.loads(buf.value.decode())
-        assert len(report["modes"]) == 3 and report["gather"]
+        assert report["gather"]
         assert report["gather_chosen"]["cached_positions"] == bool(positions)
         assert report["gather_chosen"]["sorted_windows"] == bool(positions and sorted_windows)
-        split_any = any(m["chosen"] != "fp64" for m in report["modes"])
-        if not split_any:
-            assert report["branches"]["chosen"] == ("two" if branches == 2 else report["branches"]["chosen"])
-            assert report["branches"]["lead_grid"] == 2   # the full 7^3 list has the most elements
-        singles = [eri_values_device(e[1], e[0], e[2], 1e-16, 0) for e in entries]
+        if method == 6:
+            # one grouped rank-Q product for all grids; no two-branch graph
+            assert len(report["modes"]) == 1 and len(report["modes"][0]["rank_q"]) == 2 and "branches" not in report
+        else:
+            assert len(report["modes"]) == 3
+            split_any = any(m["chosen"] != "fp64" for m in report["modes"])
+            if not split_any:
+                assert report["branches"]["chosen"] == ("two" if branches == 2 else report["branches"]["chosen"])
+                assert report["branches"]["lead_grid"] == 2   # the full 7^3 list has the most elements
+        single_method = 6 if method == 6 else 0
+        singles = [eri_values_device(e[1], e[0], e[2], 1e-16, single_method) for e in entries]
+        if method == 6:
+            for e, ref in zip(entries, singles):
+                exact = eri_values_device(e[1], e[0], e[2], 1e-16, 0)
+                assert np.max(np.abs(ref - exact)) <= 1e-14
         for rep in range(3):
             _lib.check(L.gli_eri_batch_run(plan, None, None), "gli_eri_batch_run")
             assert L.gli_synchronize() == 0
             for e, ref in zip(entries, singles):
                 got = e[4].to_numpy(np.float64, len(e[2]))
-                if method == 0:
+                if method in (0, 6):
                     assert np.array_equal(got, ref), (rep, e[0])
                 else:
                     assert np.max(np.abs(got - ref)) <= 1e-10, (rep, e[0])
@@ -635,7 +662,7 @@ def test_batch_plan_matches_single_grids(L, method, branches, positions, sorted_
         assert L.gli_synchronize() == 0
         for e, ref, sc in zip(entries, singles, scales):
             got = e[4].to_numpy(np.float64, len(e[2]))
-            if method == 0:
+            if method in (0, 6):
                 assert np.array_equal(got, ref * sc)
             else:
                 assert np.max(np.abs(got - ref * sc)) <= 2e-10
@@ -647,9 +674,9 @@ def test_batch_plan_matches_single_grids(L, method, branches, positions, sorted_
         _lib.check(L.gli_eri_batch_run(plan, ncptr, None), "run new coefficients")
         assert L.gli_synchronize() == 0
         for e, c in zip(entries, new):
-            ref = eri_values_device(c, e[0], e[2], 1e-16, 0)
+            ref = eri_values_device(c, e[0], e[2], 1e-16, single_method)
             got = e[4].to_numpy(np.float64, len(e[2]))
-            if method == 0:
+            if method in (0, 6):
                 assert np.array_equal(got, ref)
             else:
                 assert np.max(np.abs(got - ref)) <= 1e-10

@@ -60,12 +60,12 @@ def g6(n):
 BATCH = [(5, 29, 0, 0.5), (7, 43, 0, 0.75), (9, 57, 1, 1.0), (11, 57, 0, 1.5)]
 
 
-@pytest.mark.parametrize("method,bound", [(0, 1e-13), (1, 1e-10)])
+@pytest.mark.parametrize("method,bound", [(0, 1e-13), (1, 1e-10), (6, 1e-13)])
 def test_batch_workload_vs_reference(L, method, bound):
     """One plan over the full enumerated lists of all four grids (304,231,528 elements), its graph
     replayed three times with eager work in between; every replay's values at the reference build's
-    seeded canonical samples equal the reference values times 1/a within bound/a (FP64 1e-13, mixed
-    1e-10).  Every sampled canonical index must also be present in the enumerated list."""
+    seeded canonical samples equal the reference values times 1/a within bound/a (FP64 and separable 1e-13,
+    mixed 1e-10).  Every sampled canonical index must also be present in the enumerated list."""
     say("batch workload, method", method)
     n_g = len(BATCH)
     coeffs, lists, vals, samples = [], [], [], []
@@ -198,14 +198,15 @@ def test_erida_13_all_pairs_vs_reference(L):
 # the precision study at 13^3
 
 def test_precision_ladder_13_vs_reference(L):
-    """FP64, the 5-digit split (mixed), BF16 hi/lo with FP32 accumulation (bf16x2), FP32 and FP32 hi/lo
-    on the 2048 seeded canonical 13^3 indices with the reference build's values: FP64 within 1e-13,
+    """FP64, the separable expansion, the 5-digit split (mixed), BF16 hi/lo with FP32 accumulation (bf16x2), FP32
+    and FP32 hi/lo on the 2048 seeded canonical 13^3 indices with the reference build's values: FP64 and
+    separable within 1e-13,
     mixed within 1e-10, the comparison methods near their unit roundoffs (bounds 1e-3 and 1e-4 catch
     a wrong accumulator layout, which gives O(1) errors), and the ladder ordered"""
     say("precision ladder 13")
     g = golden("golden_sample_n13_K57s0.npz")
     err = {}
-    for method, bound in ((0, 1e-13), (1, 1e-10), (3, 1e-3), (4, 1e-4), (5, 1e-4)):
+    for method, bound in ((0, 1e-13), (6, 1e-13), (1, 1e-10), (3, 1e-3), (4, 1e-4), (5, 1e-4)):
         got = eri_values_device(g["coeffs"], sym(13), g["eri_indices"], float(g["tol"]), method)
         err[method] = float(np.max(np.abs(got - g["eri_values"])))
         say("  method", method, "max abs err vs reference %.3g" % err[method])

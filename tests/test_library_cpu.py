@@ -106,3 +106,40 @@ def test_no_gpu_raises_cleanly():
             "try:\n    _lib.require_gpu()\nexcept RuntimeError as e:\n    print('raised', e)\n")
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=REPO, timeout=120)
     assert "raised" in out.stdout, out.stdout + out.stderr
+
+
+def _separable_rule(L, M):
+    Q = L.gli_eri_separable_rule(M, None, None, 0)
+    s = np.zeros(Q)
+    w = np.zeros(Q)
+    dp = ctypes.POINTER(ctypes.c_double)
+    assert L.gli_eri_separable_rule(M, s.ctypes.data_as(dp), w.ctypes.data_as(dp), Q) == Q
+    return s, w
+
+
+@pytest.mark.parametrize("n,K", [(3, 57), (5, 29), (7, 43), (7, 57), (9, 57), (11, 57), (13, 57), (15, 57)])
+def test_separable_rule_reproduces_coulomb_table(n, K):
+    """the separable method's quadrature rule against the closed form of the Coulomb table,
+    T[i] = erf(d / (sqrt(2) w)) / d with d = sqrt(i) / 6, w = 1/3 (eri_integrals.c:18-27, 569-574), i.e.
+    6 erf(sqrt(i / 8)) / sqrt(i), for every table index i <= 3 M^2 of the grid's folded range M"""
+    from scipy.special import erf
+    L = _lib.lib()
+    M = 6 * (n - 1) + 2 * (K - 1)
+    s, w = _separable_rule(L, M)
+    assert 0 < len(s) <= 128 and np.all((s > 0) & (s < 1)) and np.all(w > 0)
+    assert abs(w.sum() - 1.0) <= 1e-15   # the rule integrates 1 over [0, 1]: T at i = 0
+    i = np.arange(3 * M * M + 1, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        table = np.where(i > 0, 6.0 * erf(np.sqrt(i / 8.0)) / np.sqrt(i), 6.0 / np.sqrt(2.0 * np.pi))
+    approx = np.zeros_like(i)
+    for q in range(len(s)):   # one node at a time keeps the memory small
+        approx += w[q] * np.exp(-i * (s[q] * s[q] / 8.0))
+    approx *= 6.0 / np.sqrt(2.0 * np.pi)
+    assert np.max(np.abs(approx - table) / table) <= 3e-15
+
+
+def test_separable_rule_edges():
+    L = _lib.lib()
+    assert L.gli_eri_separable_rule(-1, None, None, 0) == -1
+    s, w = _separable_rule(L, 0)
+    assert len(s) > 0 and abs(w.sum() - 1.0) <= 1e-15
