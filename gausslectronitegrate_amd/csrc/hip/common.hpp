@@ -124,8 +124,8 @@ struct Context {
 	int gather_threads = 1024; // workgroup size of the single-grid gather (its waves share one CU's L1)
 	int gather_bucketed = 2; // single-grid gather reads the key tensor in bucket order inside windows: 0 never,
 	                         // 1 always, 2 for key tensors above 128 MB
-	int gather_classed = 1;  // otherwise each wave of the single-grid gather groups its key-tensor reads by line class
-	                         // (11^3: 1.425 against 1.458 ms)
+	int gather_classed = 0;  // otherwise each wave of the single-grid gather groups its key-tensor reads by line class
+	                         // (with the size-ordered key numbering 1.308 against 1.274 ms at 11^3: off)
 	int mode1_half = 1;     // FP64 paths store the first mode's (m2, m3)-symmetric output once
 	int batch_branches = 1; // batch plans: 0 one branch, 1 autotuner times a two-branch graph, 2 always two
 	int batch_positions = 1; // batch plans cache every element's key-tensor position at creation

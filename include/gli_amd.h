@@ -71,9 +71,10 @@ int gli_synchronize(void);
  *                 the window's key-tensor positions by bucket in LDS and reads the key tensor in that order
  *                 (the elements of one cache line share a load), storing the values back in list order:
  *                 0 never, 1 always, 2 (default) when the key tensor exceeds 128 MB (13^3: 299 MB)
- *   "gather_classed" 1 (default): where the bucketed form is not chosen, each wave of the single-grid gather
+ *   "gather_classed" 1: where the bucketed form is not chosen, each wave of the single-grid gather
  *                 (8 groups of 64 elements) issues its 8 key-tensor loads class by class, class = the
- *                 element's 128-byte line mod 8, so the elements of one line share a load; 0: list order
+ *                 element's 128-byte line mod 8, so the elements of one line share a load; 0 (default):
+ *                 list order
  *   "project_chunk_kib" host memory (KiB, default 262144 = 256 MB) of one chunk of product states in
  *                 project_sparse_eri_gausslet_integrals; the states and as many results are held per chunk
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
