@@ -122,8 +122,9 @@ struct Context {
 	int gather_ept = 8;     // gather elements per lane
 	int gather_lds = 1;     // gather keeps its key tables in LDS (when they fit)
 	int gather_threads = 1024; // workgroup size of the single-grid gather (its waves share one CU's L1)
-	int gather_bucketed = 2; // single-grid gather reads the key tensor in bucket order inside windows: 0 never,
-	                         // 1 always, 2 for key tensors above 128 MB
+	int gather_bucketed = 0; // single-grid gather reads the key tensor in bucket order inside windows: 0 never
+	                         // (default: with the k1-contiguous layout the list-order gather is faster, 13^3 5.75
+	                         // against 7.40 ms), 1 always, 2 for key tensors above 128 MB
 	int gather_classed = 0;  // otherwise each wave of the single-grid gather groups its key-tensor reads by line class
 	                         // (with the size-ordered key numbering 1.308 against 1.274 ms at 11^3: off)
 	int mode1_half = 1;     // FP64 paths store the first mode's (m2, m3)-symmetric output once
@@ -209,15 +210,17 @@ void phase_begin();
 void phase_mark(const char* name);
 
 // Y[p][k] = sum_{mu < Kd} X[mu * ldx + p] * B[mu * ldb + k] for p < P, k < C (FP64 MFMA);
-// with Cp > 0 and tri_cols / tri_rows, tiles outside k >= r / outer <= r (p = outer*Cp + r) are skipped;
-// with tet_C1 > 0 the output goes to the tetrahedral layout of the sorted triple (outer, r, k), all < tet_C1
-// Position of the sorted key triple k1 <= k2 <= k3 in the tetrahedral layout of a symmetric 3-tensor:
-// blocks of growing k3 (C(k3+2, 3) entries before block k3), inside a block the runs k2 = k1..k3 of
-// each k1 stored contiguously.  The middle key is the contiguous one because the mode-3 product writes
-// output tiles of fixed k1 whose rows run over k2 (full-line stores).
+// with Cp > 0 and tri_cols / tri_rows, tiles outside k <= r / r <= outer (p = outer*Cp + r) are skipped;
+// with tet_C1 > 0 the output goes to the tetrahedral layout of the sorted triple (k, r, outer), all < tet_C1
+// Position of the sorted key triple k1 <= k2 <= k3 in the tetrahedral layout of a symmetric 3-tensor
+// (the combinatorial number system): C(k3 + 2, 3) + C(k2 + 1, 2) + k1.  The smallest key is the
+// contiguous one: with the size-ordered key numbering, the entries that neighbouring elements of a
+// sorted list read share their (k2, k3) run more often (on the 11^3 list 145 instead of 186 distinct
+// 128-byte lines per 512 consecutive elements).  The last product writes output tiles of fixed k3 whose
+// rows run over k2 and columns over k1, so its stores are full lines.
 __host__ __device__ inline int64_t tet_index(int k1, int k2, int k3)
 {
-	return (int64_t)k3 * (k3 + 1) * (k3 + 2) / 6 + (int64_t)k1 * (k3 + 1) - (int64_t)k1 * (k1 - 1) / 2 + (k2 - k1);
+	return (int64_t)k3 * (k3 + 1) * (k3 + 2) / 6 + (int64_t)k2 * (k2 + 1) / 2 + k1;
 }
 
 // T3[m1][m2][m3] = T[m1^2 + m2^2 + m3^2] for m <= M, 0 in the padding up to Mp (library stream)
@@ -240,8 +243,8 @@ struct ModeArgs {
 	// with sym_x = 1 reads X[m2][m3][k] for m2 > m3 from X[m3][m2][k] (X rows m3 * Cp + k)
 	int half_n = 0;
 	int sym_x = 0;
-	// rank-Q form of the last product (separable method): X[q][k1 Cp + k2] = cp_u[q Cp + k1] X[q ldx + k2],
-	// with ldx = Cp; the rows of a tile share k1
+	// rank-Q form of the last product (separable method): X[q][outer Cp + r] = cp_u[q Cp + outer] X[q ldx + r],
+	// with ldx = Cp; the rows of a tile share outer
 	const double* cp_u = nullptr;
 };
 constexpr int kGemmShapes = 6;

@@ -129,7 +129,8 @@ __global__ void k_kernel_tensor(const double* __restrict__ T, int M, int Mp, dou
 // = (BM/32) x (BN/32) MFMA blocks; mu staged through LDS GBK at a time with the next slice
 // prefetched into registers.  Optional triangular skipping for the symmetric tensor: with
 // p = outer * Cp + r,
-//   tri_cols: only k >= r is needed;  tri_rows: only outer <= r is needed.
+//   tri_cols: only k <= r is needed;  tri_rows: only r <= outer is needed.
+// With both and tet_C1 > 0 the tile writes V[k <= r <= outer] in the tetrahedral layout (tet_index).
 // Tiles entirely outside the needed region are not computed (never read later).  The skip test
 // needs (p_base + p0) / Cp fixed per tile: Cp and p_base multiples of BM.
 
@@ -155,8 +156,8 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 		const int64_t gp = a.p_base + p0;
 		const int64_t outer = gp / a.Cp;
 		const int64_t rlo = gp % a.Cp, rhi = rlo + BM - 1;
-		if (a.tri_cols && k0 + BN - 1 < rlo) { return; }
-		if (a.tri_rows && rhi < outer) { return; }
+		if (a.tri_cols && k0 > rhi) { return; }
+		if (a.tri_rows && rlo > outer) { return; }
 	}
 	if (a.half_n > 0) {
 		// rows p = m2 n + m3; the tile is needed when one of its rows has m3 >= m2
@@ -192,7 +193,7 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 	// stay in flight during this slice's MFMAs.
 	const int ldx = (int)a.ldx, ldb = (int)a.ldb, p0i = (int)p0, Pi = (int)a.P;
 	const int m3i = (int)sym_m3, sym_base = (int)(sym_m3 * a.ldx + sym_k);
-	// rank-Q form: the tile's rows p = k1 Cp + k2 share k1; X[q][p] = cp_u[q Cp + k1] X[q Cp + k2]
+	// rank-Q form: the tile's rows p = outer Cp + r share outer; X[q][p] = cp_u[q Cp + outer] X[q Cp + r]
 	const int cp_k1 = kCp ? (int)(p0 / a.Cp) : 0;
 	const int cp_off = kCp ? (int)(p0 - (int64_t)cp_k1 * a.Cp) : 0;
 	double su[kCp ? PA : 1];
@@ -253,14 +254,14 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 		__syncthreads();
 	}
 
-	// per-tile constants, so that no output element needs a division: the rows of a tile share k1
-	// (tetrahedral store; Cp and p_base are multiples of BM) and start at (m2, m3) = (hm2, hm3)
-	// (half-stored first mode)
-	int64_t tk1 = 0, tk2 = 0, hm2 = 0, hm3 = 0;
+	// per-tile constants, so that no output element needs a division: the rows of a tile share the
+	// largest key tk3 (tetrahedral store; Cp and p_base are multiples of BM) and start at (m2, m3) =
+	// (hm2, hm3) (half-stored first mode)
+	int64_t tk3 = 0, tk2 = 0, hm2 = 0, hm3 = 0;
 	if (a.tet_C1 > 0) {
 		const int64_t gp0 = a.p_base + p0;
-		tk1 = gp0 / a.Cp;
-		tk2 = gp0 - tk1 * a.Cp;
+		tk3 = gp0 / a.Cp;
+		tk2 = gp0 - tk3 * a.Cp;
 	}
 	if (a.half_n > 0) {
 		hm2 = p0 / a.half_n;
@@ -273,8 +274,8 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 #pragma unroll
 		for (int j = 0; j < TN; ++j) {
 			const int col = k0 + wn * (BN / 2) + j * 16 + lane % 16;
-			// tet_index(k1, k2, col) = tet_index(k1, k1, col) + (k2 - k1)
-			const int64_t tbase = (a.tet_C1 > 0) ? tet_index((int)tk1, (int)tk1, col) - tk1 : 0;
+			// tet_index(col, k2, tk3) = tet_index(col, 0, tk3) + k2 (k2 + 1) / 2
+			const int64_t tbase = (a.tet_C1 > 0) ? tet_index(col, 0, (int)tk3) : 0;
 #pragma unroll
 			for (int v = 0; v < 4; ++v) {
 				const int d = wm * (BM / 2) + i * 16 + (lane / 16) + 4 * v;   // row - p0
@@ -290,7 +291,7 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 				}
 				if (a.tet_C1 > 0) {
 					const int64_t k2 = tk2 + d;
-					if (tk1 <= k2 && k2 <= col && col < a.tet_C1) { a.Y[tbase + k2] = acc[i][j][v]; }
+					if (col <= k2 && k2 <= tk3 && tk3 < a.tet_C1) { a.Y[tbase + k2 * (k2 + 1) / 2] = acc[i][j][v]; }
 				}
 				else {
 					a.Y[row * a.ldy + col] = acc[i][j][v];
@@ -335,8 +336,8 @@ __global__ __launch_bounds__(256) void k_mode_product_f32(ModeArgs a, TileOrder 
 		const int64_t gp = a.p_base + p0;
 		const int64_t outer = gp / a.Cp;
 		const int64_t rlo = gp % a.Cp, rhi = rlo + 63;
-		if (a.tri_cols && k0 + 63 < rlo) { return; }
-		if (a.tri_rows && rhi < outer) { return; }
+		if (a.tri_cols && k0 > rhi) { return; }
+		if (a.tri_rows && rlo > outer) { return; }
 	}
 	const int wm = wave >> 1, wn = wave & 1;
 	f4v acc[2][2];
@@ -379,15 +380,15 @@ __global__ __launch_bounds__(256) void k_mode_product_f32(ModeArgs a, TileOrder 
 		__syncthreads();
 	}
 	// FP32 16x16 accumulator layout: lane holds column lane%16 and rows 4*(lane/16) + v
-	int64_t tk1 = 0, tk2 = 0;   // the tile's rows share k1 (tetrahedral store)
+	int64_t tk3 = 0, tk2 = 0;   // the tile's rows share the largest key tk3 (tetrahedral store)
 	if (a.tet_C1 > 0) {
-		tk1 = (a.p_base + p0) / a.Cp;
-		tk2 = a.p_base + p0 - tk1 * a.Cp;
+		tk3 = (a.p_base + p0) / a.Cp;
+		tk2 = a.p_base + p0 - tk3 * a.Cp;
 	}
 	for (int i = 0; i < 2; ++i) {
 		for (int j = 0; j < 2; ++j) {
 			const int col = k0 + wn * 32 + j * 16 + lane % 16;
-			const int64_t tbase = (a.tet_C1 > 0) ? tet_index((int)tk1, (int)tk1, col) - tk1 : 0;
+			const int64_t tbase = (a.tet_C1 > 0) ? tet_index(col, 0, (int)tk3) : 0;
 			for (int v = 0; v < 4; ++v) {
 				const int d = wm * 32 + i * 16 + 4 * (lane / 16) + v;
 				const int64_t row = p0 + d;
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(256) void k_mode_product_f32(ModeArgs a, TileOrder 
 				const double val = (double)acc[i][j][v];
 				if (a.tet_C1 > 0) {
 					const int64_t k2 = tk2 + d;
-					if (tk1 <= k2 && k2 <= col && col < a.tet_C1) { a.Y[tbase + k2] = val; }
+					if (col <= k2 && k2 <= tk3 && tk3 < a.tet_C1) { a.Y[tbase + k2 * (k2 + 1) / 2] = val; }
 				}
 				else {
 					a.Y[row * a.ldy + col] = val;
@@ -687,8 +688,8 @@ __device__ inline void wave_positions(const GatherDesc& d, int64_t wave_base, co
 			const unsigned mid = min(hi, kk[u][2]);
 			k3[u] = max(hi, kk[u][2]);
 			const unsigned k1 = min(lo, mid), k2 = max(lo, mid);
-			// tet_index without its k3 block offset (added from the table below); k1 (k3 + 1) < 2^32 for C1 < 2^15
-			v32[u] = __umul24(k1, k3[u] + 1u) - (__umul24(k1, k1 - 1u) >> 1) + (k2 - k1);
+			// tet_index without its k3 block offset (added from the table below); k2 (k2 + 1) < 2^32 for C1 < 2^15
+			v32[u] = (__umul24(k2, k2 + 1u) >> 1) + k1;
 		}
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { v32[u] += tet3[k3[u]]; }
@@ -2011,7 +2012,8 @@ static void separable_factors(EriWorkspace& w, const EriSetup& s)
 	w.sep_Qp = Qp;
 }
 
-// V[k1 <= k2 <= k3] = sum_q U[q][k1] G[q][k2] G[q][k3], tetrahedral store (the rank-Q tile kernel)
+// V[k1 <= k2 <= k3] = sum_q U[q][k3] G[q][k2] G[q][k1], tetrahedral store (the rank-Q tile kernel; a tile's
+// rows p = k3 Cp + k2 share k3, its columns run over k1)
 static void mode_product_cp(const ModeArgs& a)
 {
 	if (a.P <= 0 || a.C <= 0) { return; }
@@ -2050,10 +2052,11 @@ struct EriPrepared {
 };
 
 // The FP64 key tensor V from the folded columns B and the kernel tensor T3: three MFMA mode products
-//   Y1[m2][m3][k1] = sum_m1 T3[m1][m2][m3] B[m1][k1]   (symmetric in (m2, m3): with half, only m3 >= m2
+//   Y1[m2][m3][k3] = sum_m1 T3[m1][m2][m3] B[m1][k3]   (symmetric in (m2, m3): with half, only m3 >= m2
 //                                                        is computed, and mode 2 reads the rest mirrored)
-//   Y2[m3][k1][k2] = sum_m2 Y1[m2][m3][k1] B[m2][k2]   (k2 >= k1 only)
-//   V[k1 <= k2 <= k3] = sum_m3 Y2[m3][k1][k2] B[m3][k3], written in tetrahedral layout
+//   Y2[m3][k3][k2] = sum_m2 Y1[m2][m3][k3] B[m2][k2]   (k2 <= k3 only)
+//   V[k1 <= k2 <= k3] = sum_m3 Y2[m3][k3][k2] B[m3][k1], written in tetrahedral layout
+// (V is symmetric, so the first product may carry the largest key)
 // queued on the library stream.  Every FP64 entry point runs exactly this sequence.
 static void key_tensor_fp64(const EriSetup& s, const double* B, const double* T3, double* Y1, double* Y2, double* V, bool half)
 {

@@ -112,8 +112,8 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 		const int64_t gp = p_base + p0;
 		const int64_t outer = gp / Cp;
 		const int64_t rlo = gp % Cp, rhi = rlo + SBM - 1;
-		if (tri_cols && k0 + SBN - 1 < rlo) { return; }
-		if (tri_rows && rhi < outer) { return; }
+		if (tri_cols && k0 > rhi) { return; }
+		if (tri_rows && rlo > outer) { return; }
 	}
 
 	const int wm = wave >> 1, wn = wave & 1;
@@ -206,17 +206,17 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 	}
 
 	// accumulator layout of the 16x16 FP32 block: lane holds column lane%16, rows 4*(lane/16)+v
-	int64_t tk1 = 0, tk2 = 0;   // the tile's rows share k1 (tetrahedral store)
+	int64_t tk3 = 0, tk2 = 0;   // the tile's rows share the largest key tk3 (tetrahedral store)
 	if (tet_C1 > 0) {
-		tk1 = (p_base + p0) / Cp;
-		tk2 = p_base + p0 - tk1 * Cp;
+		tk3 = (p_base + p0) / Cp;
+		tk2 = p_base + p0 - tk3 * Cp;
 	}
 #pragma unroll
 	for (int i = 0; i < 2; ++i) {
 #pragma unroll
 		for (int j = 0; j < 2; ++j) {
 			const int col = k0 + wn * 32 + j * 16 + lane % 16;
-			const int64_t tbase = (tet_C1 > 0) ? tet_index((int)tk1, (int)tk1, col) - tk1 : 0;
+			const int64_t tbase = (tet_C1 > 0) ? tet_index(col, 0, (int)tk3) : 0;
 #pragma unroll
 			for (int v = 0; v < 4; ++v) {
 				const int dr = wm * 32 + i * 16 + 4 * (lane / 16) + v;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_mode_product_split(
 				const double val = ldexp(acc, ex[row] + eb[col]);
 				if (tet_C1 > 0) {
 					const int64_t k2 = tk2 + dr;
-					if (tk1 <= k2 && k2 <= col && col < tet_C1) { Y[tbase + k2] = val; }
+					if (col <= k2 && k2 <= tk3 && tk3 < tet_C1) { Y[tbase + k2 * (k2 + 1) / 2] = val; }
 				}
 				else {
 					Y[row * ldy + col] = val;
