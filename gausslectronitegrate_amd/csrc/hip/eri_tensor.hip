@@ -43,14 +43,21 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 // f[s][m] = (sqrt(pi)/3) sum_{a+b=m} c_a c_b exp(-(0.5 (3 s + a - b))^2), s = 0 .. R-1
 // (gausslet_factors.c:12-50; f_{-s} = f_s).  Same ascending-a summation order.  The K coefficients
-// are staged in LDS (dynamic shared memory, K doubles) with one coalesced read per workgroup.
+// and the 2K - 1 Gaussians exp(-(0.5 (3 s + d))^2), d = a - b = -(K-1) .. K-1, of the workgroup's
+// shift are staged in LDS (dynamic shared memory, 3K - 1 doubles): each exp is evaluated once per
+// distinct argument, with the same argument and function as in the sum, so the values are unchanged.
 __global__ void k_factors(const double* __restrict__ c, int K, double* __restrict__ f)
 {
 	extern __shared__ double cs[];
+	double* ex = cs + K;   // ex[d + K - 1]
+	const int s = blockIdx.y;
 	for (int q = threadIdx.x; q < K; q += blockDim.x) { cs[q] = c[q]; }
+	for (int q = threadIdx.x; q < 2 * K - 1; q += blockDim.x) {
+		const double x = 0.5 * (3 * s + (q - (K - 1)));
+		ex[q] = exp(-x * x);
+	}
 	__syncthreads();
 	const int L = 2 * K - 1;
-	const int s = blockIdx.y;
 	const int m = blockIdx.x * blockDim.x + threadIdx.x;
 	if (m >= L) { return; }
 	const double prefac = 0.590817950301838675766;
@@ -58,8 +65,7 @@ __global__ void k_factors(const double* __restrict__ c, int K, double* __restric
 	const int ilo = max(0, m - (K - 1)), ihi = min(K - 1, m);
 	for (int i = ilo; i <= ihi; ++i) {
 		const int j = m - i;
-		const double x = 0.5 * (3 * s + (i - j));
-		acc += prefac * cs[i] * cs[j] * exp(-x * x);
+		acc += prefac * cs[i] * cs[j] * ex[i - j + K - 1];
 	}
 	f[(int64_t)s * L + m] = acc;
 }
@@ -1540,7 +1546,7 @@ static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol,
 	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
 	hipStream_t st = ctx().stream;
 	const int L = 2 * s.K - 1, LP = 4 * s.K - 3;
-	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, (size_t)s.K * sizeof(double), st>>>(d_coeffs, s.K, f);
+	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, (size_t)(3 * s.K - 1) * sizeof(double), st>>>(d_coeffs, s.K, f);
 	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, (size_t)2 * L * sizeof(double), st>>>(f, s.K, s.R, tol, P);
 	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, s.Cp, B);
 	GLI_HIP(hipGetLastError());
@@ -2018,10 +2024,20 @@ static void mode_product_cp(const ModeArgs& a)
 {
 	if (a.P <= 0 || a.C <= 0) { return; }
 	check_mode_extent(a);
-	GLI_REQUIRE(a.cp_u != nullptr && a.ldx == a.Cp && a.p_base == 0 && gemm_shape_fits(0, a.Cp, 1, 0),
-	            "rank-Q product: rows must be k1 Cp + k2 with the key padding a multiple of the tile");
-	const TileOrder o = tile_order_for(0, a.P, a.C, tile_order_grouped());
-	k_mode_product<64, 64, 16, true><<<(unsigned)o.blocks(), 256, 0, ctx().stream>>>(a, o);
+	int shape = ctx().gemm_shape;
+	if (!gemm_shape_fits(shape, a.Cp, 1, 0)) { shape = 0; }
+	GLI_REQUIRE(a.cp_u != nullptr && a.ldx == a.Cp && a.p_base == 0 && gemm_shape_fits(shape, a.Cp, 1, 0),
+	            "rank-Q product: rows must be k3 Cp + k2 with the key padding a multiple of the tile");
+	const TileOrder o = tile_order_for(shape, a.P, a.C, tile_order_grouped());
+	hipStream_t st = ctx().stream;
+	switch (shape) {
+		case 1: k_mode_product<128, 64, 16, true><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o); break;
+		case 2: k_mode_product<64, 128, 16, true><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o); break;
+		case 3: k_mode_product<128, 128, 16, true><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o); break;
+		case 4: k_mode_product<64, 64, 32, true><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o); break;
+		case 5: k_mode_product<128, 64, 32, true><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o); break;
+		default: k_mode_product<64, 64, 16, true><<<(unsigned)o.blocks(), 256, 0, st>>>(a, o); break;
+	}
 	GLI_HIP(hipGetLastError());
 }
 
