@@ -1925,13 +1925,15 @@ static void direct_representations(EriWorkspace& w, int M)
 //     T3[m1][m2][m3] ~ sum_q W_q g_q[m1] g_q[m2] g_q[m3],  g_q[m] = exp(-m^2 s_q^2 / 8),  W_q = C w_q,
 // and the key tensor becomes V[k1][k2][k3] = sum_q U[q][k1] G[q][k2] G[q][k3] with G = g^T B and
 // U = (W g)^T B: one rank-Q product in place of the three mode products.  The rule is composite
-// Gauss-Legendre on geometric panels [0, r^J], [r^J, r^(J-1)], ..., [r, 1] with r = 0.35, 12 nodes on
-// the first panel and 16 on each other, and J the smallest with r^J <= 1 / (1.5 sqrt(3 M^2)): every
-// integrand exp(-i s^2 / 8), i <= 3 M^2, then varies by a bounded amount per panel.  Q = 108 nodes for
-// the folded ranges of 3^3..13^3 grids with K = 57 (M = 124..184); against 6 erf(sqrt(i/8)) / sqrt(i)
-// the rule is within 2e-15 relative for every i <= 3 M^2 and every M from 4 to 263.
+// Gauss-Legendre on geometric panels [0, r^J], [r^J, r^(J-1)], ..., [r, 1] with r = 0.35 and J the
+// smallest with r^J <= 1 / (1.5 sqrt(3 M^2)): every integrand exp(-i s^2 / 8), i <= 3 M^2, then varies
+// by a bounded amount per panel.  Counted from s = 0 the panels take 5, 6, 9 and 14 nodes, every
+// further one 16 (the panels near 0 hold smooth, nearly flat parts of every integrand).  Q = 82 nodes
+// for the folded ranges of 3^3..13^3 grids with K = 57 (M = 124..184); against 6 erf(sqrt(i/8)) / sqrt(i)
+// the rule is within 2.2e-15 relative for every i <= 3 M^2 and every M from 0 to 318.
 constexpr long double kSepRatio = 0.35L;
-constexpr int kSepNodesFirst = 12, kSepNodesPanel = 16;
+constexpr int kSepNodesLow[4] = { 5, 6, 9, 14 };   // the first four panels from s = 0
+constexpr int kSepNodesPanel = 16;                 // every further panel
 
 // Gauss-Legendre nodes x and weights w on [-1, 1]: Newton iterations on P_p from the usual cosine
 // guesses, in extended precision
@@ -1967,16 +1969,13 @@ void separable_rule(int M, std::vector<long double>& s, std::vector<long double>
 		lo *= kSepRatio;
 		++J;
 	}
-	long double xf[kSepNodesFirst], wf[kSepNodesFirst], xp[kSepNodesPanel], wp[kSepNodesPanel];
-	gauss_legendre(kSepNodesFirst, xf, wf);
-	gauss_legendre(kSepNodesPanel, xp, wp);
+	long double x[kSepNodesPanel], wx[kSepNodesPanel];
 	s.clear();
 	w.clear();
 	long double a = 0, b = lo;
 	for (int panel = 0; panel <= J; ++panel) {
-		const int p = panel == 0 ? kSepNodesFirst : kSepNodesPanel;
-		const long double* x = panel == 0 ? xf : xp;
-		const long double* wx = panel == 0 ? wf : wp;
+		const int p = panel < 4 ? kSepNodesLow[panel] : kSepNodesPanel;
+		gauss_legendre(p, x, wx);
 		for (int i = 0; i < p; ++i) {
 			s.push_back((b - a) / 2 * x[i] + (a + b) / 2);
 			w.push_back((b - a) / 2 * wx[i]);
