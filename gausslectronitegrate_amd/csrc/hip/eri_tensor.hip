@@ -1548,7 +1548,9 @@ static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol,
 	const int L = 2 * s.K - 1, LP = 4 * s.K - 3;
 	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, (size_t)(3 * s.K - 1) * sizeof(double), st>>>(d_coeffs, s.K, f);
 	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, (size_t)2 * L * sizeof(double), st>>>(f, s.K, s.R, tol, P);
-	k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, s.Cp, B);
+	if (B != nullptr) {   // the separable method folds the Gaussian factors instead (k_separable_gu)
+		k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, s.Cp, B);
+	}
 	GLI_HIP(hipGetLastError());
 }
 
@@ -2017,6 +2019,45 @@ static void separable_factors(EriWorkspace& w, const EriSetup& s)
 	w.sep_Qp = Qp;
 }
 
+// [G; U][r][key] = sum_mu gw[mu][r] B[mu][key] straight from the pair products: B[mu][key] sums the P'
+// entries t of the key's shift pair (a, b) with |t - (2K-2) + 3 cd| = mu (k_fold), so
+//     [G; U][r][key] = sum_{t < 4K-3} P'_ab[t] gw[|t - (2K-2) + 3 cd|][r]   (terms with that index above M
+// vanish).  One workgroup per key column (Cp of them; padding columns are zero), one thread per row r of
+// [g | W g]; the key's P' row is staged in LDS.  About 8 MFLOP at 11^3: the launch is latency-bound, so
+// each step issues 16 independent loads before its FMAs (four accumulators, added at the end).
+__global__ __launch_bounds__(256) void k_separable_gu(const double* __restrict__ P, const int4* __restrict__ keys, int C1,
+                                                      int K, int R, int M, const double* __restrict__ gw, int rows,
+                                                      int Cp, double* __restrict__ GU)
+{
+	extern __shared__ double ps[];   // 4K - 3 doubles
+	const int key = blockIdx.x;
+	const int LP = 4 * K - 3, off = 2 * K - 2;
+	if (key >= C1) {
+		for (int r = threadIdx.x; r < rows; r += blockDim.x) { GU[(int64_t)r * Cp + key] = 0.0; }
+		return;
+	}
+	const int4 k = keys[key];
+	const double* p = P + ((int64_t)k.x * R + k.y) * LP;
+	for (int t = threadIdx.x; t < LP; t += blockDim.x) { ps[t] = p[t]; }
+	__syncthreads();
+	// the terms with |t + shift| <= M form one interval of t
+	const int shift = 3 * k.z - off;
+	const int tlo = max(0, -M - shift), thi = min(LP - 1, M - shift);
+	for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+		double acc[4] = { 0, 0, 0, 0 };
+		int t = tlo;
+		for (; t + 16 <= thi + 1; t += 16) {
+			double g[16];
+#pragma unroll
+			for (int u = 0; u < 16; ++u) { g[u] = gw[(int64_t)abs(t + u + shift) * rows + r]; }
+#pragma unroll
+			for (int u = 0; u < 16; ++u) { acc[u & 3] = fma(ps[t + u], g[u], acc[u & 3]); }
+		}
+		for (; t <= thi; ++t) { acc[0] = fma(ps[t], gw[(int64_t)abs(t + shift) * rows + r], acc[0]); }
+		GU[(int64_t)r * Cp + key] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+	}
+}
+
 // V[k1 <= k2 <= k3] = sum_q U[q][k3] G[q][k2] G[q][k1], tetrahedral store (the rank-Q tile kernel; a tile's
 // rows p = k3 Cp + k2 share k3, its columns run over k1)
 static void mode_product_cp(const ModeArgs& a)
@@ -2040,15 +2081,23 @@ static void mode_product_cp(const ModeArgs& a)
 	GLI_HIP(hipGetLastError());
 }
 
-// The key tensor of the separable method: the rank-Q factors G and U in one product of [g | W g]
-// with the folded columns B, then the rank-Q product into V.  Queued on the library stream.
+// [G; U] of one grid from its pair products P' (library stream)
+static void separable_gu(const EriSetup& s, const double* P, const double* gw, int Qp, double* GU)
+{
+	const int rows = 2 * Qp;
+	k_separable_gu<<<(unsigned)s.Cp, (unsigned)std::min(256, (int)round_up(rows, 64)), (size_t)(4 * s.K - 3) * sizeof(double),
+	                 ctx().stream>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, gw, rows, s.Cp, GU);
+	GLI_HIP(hipGetLastError());
+}
+
+// The key tensor of the separable method: the rank-Q factors G and U from the pair products, then the
+// rank-Q product into V.  Queued on the library stream.
 static void key_tensor_separable(const EriSetup& s, EriWorkspace& w)
 {
 	separable_factors(w, s);
 	const int64_t Qp = w.sep_Qp, Cp = s.Cp;
 	w.GU.reserve((size_t)(2 * Qp * Cp));
-	// [G; U] = [g | W g]^T B: an FP64 MFMA product with 2 Qp output rows (~30 MFLOP at 11^3)
-	mode_product(w.sep_gw.p, 2 * Qp, 2 * Qp, w.B.p, Cp, s.Cp, s.Mp, w.GU.p, Cp, s.Cp, 0, 0, 0, 0);
+	separable_gu(s, w.P.p, w.sep_gw.p, (int)Qp, w.GU.p);
 	phase_mark("separable_factors");
 	ModeArgs a{w.GU.p, Cp, Cp * Cp, w.GU.p, Cp, s.Cp, (int)Qp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1};
 	a.cp_u = w.GU.p + Qp * Cp;
@@ -2111,7 +2160,7 @@ static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t gr
 	w.P.reserve((size_t)s.R * s.R * (4 * s.K - 3));
 	w.B.reserve((size_t)s.Mp * s.Cp);
 	upload_async(w.coeffs.p, coeffs, K * sizeof(double));
-	launch_tables(s, w.coeffs.p, tol, w.f.p, w.P.p, w.B.p);
+	launch_tables(s, w.coeffs.p, tol, w.f.p, w.P.p, method == GLI_ERI_SEPARABLE_FP64 ? nullptr : w.B.p);
 	phase_mark("tables");
 	if (!any) { return ep; }
 	const int64_t Mp = s.Mp, Cp = s.Cp;
@@ -2414,6 +2463,13 @@ static ModeArgs batch_mode_args(const BatchGrid& bg, int mode, bool half)
 	return ModeArgs{bg.Y2.p, Cp * Cp, Cp * Cp, bg.B.p, Cp, s.Cp, s.Mp, bg.V.p, 0, s.Cp, 1, 1, 0, s.C1};
 }
 
+// the tables of one grid of a plan (separable plans need no folded columns)
+static void batch_tables(const BatchPlan& plan, BatchGrid& g)
+{
+	launch_tables(g.s, plan.d_params.p + g.coeff_off, plan.tol, g.f.p, g.P.p,
+	              plan.method == GLI_ERI_SEPARABLE_FP64 ? nullptr : g.B.p);
+}
+
 // separable plans: the rank-Q product of one grid (V from its factors [G; U])
 static ModeArgs batch_sep_args(const BatchGrid& bg)
 {
@@ -2424,20 +2480,19 @@ static ModeArgs batch_sep_args(const BatchGrid& bg)
 	return a;
 }
 
-// separable plans: every grid's factors [G; U] = [g | W g]^T B in one grouped product, then one grouped
-// rank-Q product over all grids (the single-grid path's tile arithmetic, so bit-identical to it)
+// separable plans: every grid's factors [G; U] from its pair products, then one grouped rank-Q product
+// over all grids (the single-grid path's kernels, so bit-identical to it)
 static void batch_separable(BatchPlan& plan, int order)
 {
-	ModeArgs f[kMaxGroup], a[kMaxGroup];
+	ModeArgs a[kMaxGroup];
 	int n = 0;
 	for (auto& g : plan.grids) {
 		const EriSetup& s = g->s;
 		const int64_t Qp = g->sep_Qp;
-		f[n] = ModeArgs{g->sep_gw, 2 * Qp, 2 * Qp, g->B.p, s.Cp, s.Cp, s.Mp, g->GU.p, s.Cp, s.Cp, 0, 0, 0, 0};
+		separable_gu(s, g->P.p, g->sep_gw, (int)Qp, g->GU.p);
 		a[n] = batch_sep_args(*g);
 		++n;
 	}
-	mode_product_grouped(f, n, 0, order);
 	mode_product_grouped(a, n, 0, order);
 }
 
@@ -2539,7 +2594,7 @@ struct StreamScope {
 static void batch_compute_branches(BatchPlan& plan, const BatchChoice& ch)
 {
 	hipStream_t lib_stream = ctx().stream;
-	for (auto& g : plan.grids) { launch_tables(g->s, plan.d_params.p + g->coeff_off, plan.tol, g->f.p, g->P.p, g->B.p); }
+	for (auto& g : plan.grids) { batch_tables(plan, *g); }
 	batch_gather_descs_branches(plan, ch.gather_ept);
 	GLI_HIP(hipEventRecord(plan.fork, lib_stream));
 	GLI_HIP(hipStreamWaitEvent(plan.side, plan.fork, 0));
@@ -2558,7 +2613,7 @@ static void batch_compute_branches(BatchPlan& plan, const BatchChoice& ch)
 // everything after the parameter upload
 static void batch_compute(BatchPlan& plan, const BatchChoice& ch)
 {
-	for (auto& g : plan.grids) { launch_tables(g->s, plan.d_params.p + g->coeff_off, plan.tol, g->f.p, g->P.p, g->B.p); }
+	for (auto& g : plan.grids) { batch_tables(plan, *g); }
 	const bool any_split = ch.split[0] || ch.split[1] || ch.split[2];
 	if (plan.method == GLI_ERI_SEPARABLE_FP64) {
 		batch_separable(plan, ch.gemm_order[2]);
@@ -2619,7 +2674,7 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 {
 	BatchChoice ch;
 	std::string rep = "{\"modes\": [";
-	for (auto& g : plan.grids) { launch_tables(g->s, plan.d_params.p + g->coeff_off, plan.tol, g->f.p, g->P.p, g->B.p); }
+	for (auto& g : plan.grids) { batch_tables(plan, *g); }
 	// a mixed plan always runs the split-precision products (the autotuner picks among their
 	// variants only), so that the method a plan reports is the arithmetic it executes
 	const bool mixed = plan.method == GLI_ERI_TENSOR_MIXED;
