@@ -42,57 +42,57 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // 1D tables
 
 // f[s][m] = (sqrt(pi)/3) sum_{a+b=m} c_a c_b exp(-(0.5 (3 s + a - b))^2), s = 0 .. R-1
-// (gausslet_factors.c:12-50; f_{-s} = f_s).  Same ascending-a summation order.  The K coefficients
-// and the 2K - 1 Gaussians exp(-(0.5 (3 s + d))^2), d = a - b = -(K-1) .. K-1, of the workgroup's
-// shift are staged in LDS (dynamic shared memory, 3K - 1 doubles): each exp is evaluated once per
-// distinct argument, with the same argument and function as in the sum, so the values are unchanged.
-__global__ void k_factors(const double* __restrict__ c, int K, double* __restrict__ f)
+// (gausslet_factors.c:12-50; f_{-s} = f_s), same ascending-a summation order, into fs[m].  ex holds the
+// 2K - 1 Gaussians exp(-(0.5 (3 s + d))^2), d = a - b = -(K-1) .. K-1, of the shift: each exp is evaluated
+// once per distinct argument, with the same argument and function as in the sum.  Called by a whole
+// workgroup; cs holds the K coefficients.
+__device__ inline void factor_row(const double* cs, int K, int s, double* ex, double* fs)
 {
-	extern __shared__ double cs[];
-	double* ex = cs + K;   // ex[d + K - 1]
-	const int s = blockIdx.y;
-	for (int q = threadIdx.x; q < K; q += blockDim.x) { cs[q] = c[q]; }
 	for (int q = threadIdx.x; q < 2 * K - 1; q += blockDim.x) {
 		const double x = 0.5 * (3 * s + (q - (K - 1)));
 		ex[q] = exp(-x * x);
 	}
 	__syncthreads();
-	const int L = 2 * K - 1;
-	const int m = blockIdx.x * blockDim.x + threadIdx.x;
-	if (m >= L) { return; }
 	const double prefac = 0.590817950301838675766;
-	double acc = 0;
-	const int ilo = max(0, m - (K - 1)), ihi = min(K - 1, m);
-	for (int i = ilo; i <= ihi; ++i) {
-		const int j = m - i;
-		acc += prefac * cs[i] * cs[j] * ex[i - j + K - 1];
+	for (int m = threadIdx.x; m < 2 * K - 1; m += blockDim.x) {
+		double acc = 0;
+		const int ilo = max(0, m - (K - 1)), ihi = min(K - 1, m);
+		for (int i = ilo; i <= ihi; ++i) {
+			const int j = m - i;
+			acc += prefac * cs[i] * cs[j] * ex[i - j + K - 1];
+		}
+		fs[m] = acc;
 	}
-	f[(int64_t)s * L + m] = acc;
 }
 
 // P[a][b][t] = sum_m f_a[m] f_b[m - (t - (2K-2))], zeroed where |P| <= tol
-// (eri_integrals.c:59-103 with the tolerance skips of 629/637/658 folded in).  The two factor rows
-// f_a and f_b are staged in LDS (dynamic shared memory, 2 (2K - 1) doubles).
-__global__ void k_products(const double* __restrict__ f, int K, int R, double tol, double* __restrict__ P)
+// (eri_integrals.c:59-103 with the tolerance skips of 629/637/658 folded in).  One workgroup per pair
+// (a, b): it computes the two factor rows f_a and f_b itself (factor_row; 2 (2K - 1) K terms, cheaper
+// than a separate launch) into LDS, then every t.  Dynamic LDS: K coefficients, 2K - 1 Gaussians and the
+// two factor rows.
+__global__ void k_products(const double* __restrict__ c, int K, int R, double tol, double* __restrict__ P)
 {
-	extern __shared__ double fs[];
+	extern __shared__ double lds[];
 	const int L = 2 * K - 1, LP = 4 * K - 3;
-	const int ab = blockIdx.y;
+	double* cs = lds;
+	double* ex = cs + K;
+	double* fa = ex + L;
+	double* fb = fa + L;
+	const int ab = blockIdx.x;
 	const int a = ab / R, b = ab % R;
-	double* fa = fs;
-	double* fb = fs + L;
-	for (int q = threadIdx.x; q < L; q += blockDim.x) {
-		fa[q] = f[(int64_t)a * L + q];
-		fb[q] = f[(int64_t)b * L + q];
-	}
+	for (int q = threadIdx.x; q < K; q += blockDim.x) { cs[q] = c[q]; }
 	__syncthreads();
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= LP) { return; }
-	const int d = t - (2 * K - 2);  // m - m'
-	double acc = 0;
-	const int mlo = max(0, d), mhi = min(L - 1, L - 1 + d);
-	for (int m = mlo; m <= mhi; ++m) { acc += fa[m] * fb[m - d]; }
-	P[(int64_t)ab * LP + t] = (fabs(acc) > tol) ? acc : 0.0;
+	factor_row(cs, K, a, ex, fa);
+	__syncthreads();
+	factor_row(cs, K, b, ex, fb);
+	__syncthreads();
+	for (int t = threadIdx.x; t < LP; t += blockDim.x) {
+		const int d = t - (2 * K - 2);  // m - m'
+		double acc = 0;
+		const int mlo = max(0, d), mhi = min(L - 1, L - 1 + d);
+		for (int m = mlo; m <= mhi; ++m) { acc += fa[m] * fb[m - d]; }
+		P[(int64_t)ab * LP + t] = (fabs(acc) > tol) ? acc : 0.0;
+	}
 }
 
 // B[mu][q] = P'[a][b][mu - 3cd] + (mu > 0 ? P'[a][b][-mu - 3cd] : 0)
@@ -1461,7 +1461,7 @@ static KeySet& key_set(int R, bool symmetric)
 }
 
 struct EriWorkspace {
-	DevBuf<double> coeffs, f, P, B, Y1, Y2, V;
+	DevBuf<double> coeffs, P, B, Y1, Y2, V;
 	// separable method: the Gaussian factors [g | W g] of the folded range sep_M (Mp x 2 Qp, cached)
 	// and the rank-Q factors [G; U] of the current evaluation (2 Qp x Cp)
 	DevBuf<double> sep_gw, GU;
@@ -1541,13 +1541,12 @@ static EriSetup eri_shape(int64_t K, const int64_t grid[6], bool sym, bool devic
 }
 
 // f, P (zeroed where |P| <= tol) and the folded key columns B from device-resident coefficients
-static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol, double* f, double* P, double* B)
+static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol, double* P, double* B)
 {
 	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
 	hipStream_t st = ctx().stream;
-	const int L = 2 * s.K - 1, LP = 4 * s.K - 3;
-	k_factors<<<dim3((unsigned)ceil_div(L, 128), s.R), 128, (size_t)(3 * s.K - 1) * sizeof(double), st>>>(d_coeffs, s.K, f);
-	k_products<<<dim3((unsigned)ceil_div(LP, 128), s.R * s.R), 128, (size_t)2 * L * sizeof(double), st>>>(f, s.K, s.R, tol, P);
+	const int L = 2 * s.K - 1;
+	k_products<<<(unsigned)(s.R * s.R), 256, (size_t)(s.K + 3 * L) * sizeof(double), st>>>(d_coeffs, s.K, s.R, tol, P);
 	if (B != nullptr) {   // the separable method folds the Gaussian factors instead (k_separable_gu)
 		k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, s.Cp, B);
 	}
@@ -2156,11 +2155,10 @@ static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t gr
 	const EriSetup& s = ep.s;
 	EriWorkspace& w = eri_ws();
 	w.coeffs.reserve(K);
-	w.f.reserve((size_t)s.R * (2 * s.K - 1));
 	w.P.reserve((size_t)s.R * s.R * (4 * s.K - 3));
 	w.B.reserve((size_t)s.Mp * s.Cp);
 	upload_async(w.coeffs.p, coeffs, K * sizeof(double));
-	launch_tables(s, w.coeffs.p, tol, w.f.p, w.P.p, method == GLI_ERI_SEPARABLE_FP64 ? nullptr : w.B.p);
+	launch_tables(s, w.coeffs.p, tol, w.P.p, method == GLI_ERI_SEPARABLE_FP64 ? nullptr : w.B.p);
 	phase_mark("tables");
 	if (!any) { return ep; }
 	const int64_t Mp = s.Mp, Cp = s.Cp;
@@ -2290,7 +2288,6 @@ static void eri_values_noalloc(const double* coeffs, int64_t K, const int64_t gr
 	const int64_t nlut = (int64_t)s.R * s.R * (4 * s.R - 3);
 	ScratchCarve sc{base, kEriScratchBytes};
 	double* d_coeffs = sc.take<double>(K);
-	double* f = sc.take<double>((int64_t)s.R * (2 * s.K - 1));
 	double* P = sc.take<double>((int64_t)s.R * s.R * (4 * s.K - 3));
 	double* B = sc.take<double>(Mp * Cp);
 	double* T = sc.take<double>(nT);
@@ -2322,7 +2319,7 @@ static void eri_values_noalloc(const double* coeffs, int64_t K, const int64_t gr
 	s.d_tet3 = tet3;
 	// the same kernels in the same order as gli_eri_values_device with the FP64 method
 	phase_begin();
-	launch_tables(s, d_coeffs, tol, f, P, B);
+	launch_tables(s, d_coeffs, tol, P, B);
 	phase_mark("tables");
 	if (num == 0) { return; }
 	launch_kernel_tensor(s, T3);
@@ -2379,7 +2376,7 @@ struct BatchGrid {
 	const int64_t* d_idx = nullptr;
 	int64_t num = 0;
 	double* d_vals = nullptr;
-	DevBuf<double> f, P, B, Y1, Y2, V;
+	DevBuf<double> P, B, Y1, Y2, V;
 	DevBuf<double> GU;      // separable plans: the rank-Q factors [G; U] (2 Qp x Cp)
 	const double* sep_gw = nullptr;   // ... and the Gaussian factors [g | W g] of the grid's folded range
 	int sep_Qp = 0;
@@ -2466,7 +2463,7 @@ static ModeArgs batch_mode_args(const BatchGrid& bg, int mode, bool half)
 // the tables of one grid of a plan (separable plans need no folded columns)
 static void batch_tables(const BatchPlan& plan, BatchGrid& g)
 {
-	launch_tables(g.s, plan.d_params.p + g.coeff_off, plan.tol, g.f.p, g.P.p,
+	launch_tables(g.s, plan.d_params.p + g.coeff_off, plan.tol, g.P.p,
 	              plan.method == GLI_ERI_SEPARABLE_FP64 ? nullptr : g.B.p);
 }
 
@@ -2816,7 +2813,6 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 		bg->num = num[g];
 		bg->d_vals = d_values[g];
 		const EriSetup& s = bg->s;
-		bg->f.alloc((size_t)s.R * (2 * s.K - 1));
 		bg->P.alloc((size_t)s.R * s.R * (4 * s.K - 3));
 		bg->B.alloc((size_t)s.Mp * s.Cp);
 		bg->V.alloc((size_t)tet_index(0, 0, s.C1));
@@ -3261,9 +3257,11 @@ void gli_release_workspace(void)
 {
 	std::lock_guard<std::recursive_mutex> lk(ctx().mu);
 	EriWorkspace& w = eri_ws();
-	w.coeffs.release(); w.f.release(); w.P.release(); w.B.release();
+	w.coeffs.release(); w.P.release(); w.B.release();
 	w.Y1.release(); w.Y2.release(); w.V.release(); w.T3.release();
 	w.T3_M = -1;
+	w.GU.release(); w.sep_gw.release();
+	w.sep_M = -1;
 	w.rep_off.release(); w.rep_list.release();
 	w.rep_M = -1;
 }
