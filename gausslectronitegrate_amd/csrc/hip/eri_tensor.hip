@@ -70,7 +70,14 @@ __device__ inline void factor_row(const double* cs, int K, int s, double* ex, do
 // (a, b): it computes the two factor rows f_a and f_b itself (factor_row; 2 (2K - 1) K terms, cheaper
 // than a separate launch) into LDS, then every t.  Dynamic LDS: K coefficients, 2K - 1 Gaussians and the
 // two factor rows.
-__global__ void k_products(const double* __restrict__ c, int K, int R, double tol, double* __restrict__ P)
+// Coefficient sets of up to kArgCoeffs entries travel as a kernel argument, so a single evaluation needs no
+// host-to-device copy before its first kernel; longer sets are read from device memory.
+constexpr int kArgCoeffs = 64;
+struct CoeffArg {
+	double c[kArgCoeffs];
+};
+
+__global__ void k_products(const double* __restrict__ c, CoeffArg ca, int K, int R, double tol, double* __restrict__ P)
 {
 	extern __shared__ double lds[];
 	const int L = 2 * K - 1, LP = 4 * K - 3;
@@ -80,7 +87,7 @@ __global__ void k_products(const double* __restrict__ c, int K, int R, double to
 	double* fb = fa + L;
 	const int ab = blockIdx.x;
 	const int a = ab / R, b = ab % R;
-	for (int q = threadIdx.x; q < K; q += blockDim.x) { cs[q] = c[q]; }
+	for (int q = threadIdx.x; q < K; q += blockDim.x) { cs[q] = (c != nullptr) ? c[q] : ca.c[q]; }
 	__syncthreads();
 	factor_row(cs, K, a, ex, fa);
 	__syncthreads();
@@ -1541,12 +1548,20 @@ static EriSetup eri_shape(int64_t K, const int64_t grid[6], bool sym, bool devic
 }
 
 // f, P (zeroed where |P| <= tol) and the folded key columns B from device-resident coefficients
-static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol, double* P, double* B)
+// tables of one evaluation from device-resident coefficients (d_coeffs) or, with d_coeffs null, from
+// host coefficients h_coeffs of at most kArgCoeffs entries passed as a kernel argument
+static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol, double* P, double* B,
+                          const double* h_coeffs = nullptr)
 {
 	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
 	hipStream_t st = ctx().stream;
 	const int L = 2 * s.K - 1;
-	k_products<<<(unsigned)(s.R * s.R), 256, (size_t)(s.K + 3 * L) * sizeof(double), st>>>(d_coeffs, s.K, s.R, tol, P);
+	CoeffArg ca{};
+	if (d_coeffs == nullptr) {
+		GLI_REQUIRE(h_coeffs != nullptr && s.K <= kArgCoeffs, "coefficients by argument: at most 64 of them");
+		for (int q = 0; q < s.K; ++q) { ca.c[q] = h_coeffs[q]; }
+	}
+	k_products<<<(unsigned)(s.R * s.R), 256, (size_t)(s.K + 3 * L) * sizeof(double), st>>>(d_coeffs, ca, s.K, s.R, tol, P);
 	if (B != nullptr) {   // the separable method folds the Gaussian factors instead (k_separable_gu)
 		k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, s.Cp, B);
 	}
@@ -2154,11 +2169,15 @@ static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t gr
 	ep.method = method;
 	const EriSetup& s = ep.s;
 	EriWorkspace& w = eri_ws();
-	w.coeffs.reserve(K);
 	w.P.reserve((size_t)s.R * s.R * (4 * s.K - 3));
 	w.B.reserve((size_t)s.Mp * s.Cp);
-	upload_async(w.coeffs.p, coeffs, K * sizeof(double));
-	launch_tables(s, w.coeffs.p, tol, w.P.p, method == GLI_ERI_SEPARABLE_FP64 ? nullptr : w.B.p);
+	const double* d_coeffs = nullptr;   // up to 64 coefficients go as a kernel argument
+	if (K > kArgCoeffs) {
+		w.coeffs.reserve(K);
+		upload_async(w.coeffs.p, coeffs, K * sizeof(double));
+		d_coeffs = w.coeffs.p;
+	}
+	launch_tables(s, d_coeffs, tol, w.P.p, method == GLI_ERI_SEPARABLE_FP64 ? nullptr : w.B.p, coeffs);
 	phase_mark("tables");
 	if (!any) { return ep; }
 	const int64_t Mp = s.Mp, Cp = s.Cp;
