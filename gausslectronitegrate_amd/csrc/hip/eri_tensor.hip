@@ -697,10 +697,10 @@ __device__ inline void wave_positions(const GatherDesc& d, int64_t wave_base, co
 		unsigned k3[GATHER_EPT];
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) {
-			const unsigned lo = min(kk[u][0], kk[u][1]), hi = max(kk[u][0], kk[u][1]);
-			const unsigned mid = min(hi, kk[u][2]);
-			k3[u] = max(hi, kk[u][2]);
-			const unsigned k1 = min(lo, mid), k2 = max(lo, mid);
+			// the sorted keys k1 <= k2 <= k3 (minimum, median of three, maximum)
+			const unsigned a0 = kk[u][0], a1 = kk[u][1], a2 = kk[u][2];
+			const unsigned k1 = min(min(a0, a1), a2), k2 = max(min(a0, a1), min(max(a0, a1), a2));
+			k3[u] = max(max(a0, a1), a2);
 			// tet_index without its k3 block offset (added from the table below); k2 (k2 + 1) < 2^32 for C1 < 2^15
 			v32[u] = (__umul24(k2, k2 + 1u) >> 1) + k1;
 		}
