@@ -327,6 +327,63 @@ def batch_measure(L, args, stream):
             "errors": errs}
 
 
+def large_measure(L, args, stream):
+    """The largest single-GPU case: every unique four-index of the n^3 grid (n = --large-n, K = 57,
+    seed 0), timed with the bench's method, the exact FP64 key-tensor method and the BF16 / FP32 key-tensor
+    variants, each with its error against the exact FP64 method over every element (computed on the
+    device), and the ERI-DA set of the same grid.  Returns the measurement as a dict."""
+    import torch
+    import gausslectronitegrate_amd as gli
+    n, K, seed = args.large_n, 57, 0
+    c = synthetic_coefficients(K, seed)
+    g = sym_grid6(n)
+    p = C.c_void_p()
+    num = C.c_int64()
+    t0 = time.perf_counter()
+    _lib.check(L.gli_eri_enumerate_device(n, C.byref(p), C.byref(num)), "enumeration")
+    L.gli_synchronize()
+    enum_ms = (time.perf_counter() - t0) * 1e3
+    num = num.value
+    log("large: %d^3, %d unique four-indices (%.1f ms)" % (n, num, enum_ms))
+    steps = max(1, min(args.steps, 5))
+    names = [args.method] + [m for m in ("tensor", "bf16x2", "fp32") if m != args.method]
+    out = {}
+    ref = torch.empty(num, dtype=torch.float64, device="cuda")
+    cur = torch.empty(num, dtype=torch.float64, device="cuda")
+    for name in ["tensor"] + [m for m in names if m != "tensor"]:
+        dst = ref if name == "tensor" else cur
+        eri_step(L, c, g, p, num, args.tol, METHODS[name], C.c_void_p(dst.data_ptr()))
+        L.gli_synchronize()
+        torch.cuda.synchronize()
+        m0 = torch.cuda.Event(enable_timing=True)
+        m1 = torch.cuda.Event(enable_timing=True)
+        m0.record(stream)
+        for _ in range(steps):
+            eri_step(L, c, g, p, num, args.tol, METHODS[name], C.c_void_p(dst.data_ptr()))
+        m1.record(stream)
+        L.gli_synchronize()
+        torch.cuda.synchronize()
+        ms = m0.elapsed_time(m1) / steps
+        rec = {"dtype": DTYPES[name], "ms_per_step": ms, "value": num / (ms / 1e3)}
+        if name != "tensor":
+            d = (cur - ref).abs_()
+            big = ref.abs() >= 1e-10
+            rec["max_abs_err_vs_fp64_all_elements"] = float(d.max())
+            rec["max_rel_err_vs_fp64_where_abs_ge_1e-10"] = float((d[big] / ref[big].abs()).max()) if bool(big.any()) else None
+            del d, big
+        out[name] = rec
+        log("large %s: %.3f ms/step" % (name, ms))
+    del ref, cur
+    L.gli_device_free(p)
+    h = (n - 1) // 2
+    t0 = time.perf_counter()
+    erida = gli.compute_sparse_erida_gausslet_integrals(c, ((-h, n), (-h, n), (-h, n)), args.tol)
+    erida_ms = (time.perf_counter() - t0) * 1e3
+    return {"workload": "eri_%dx%dx%d_K%d_seed%d" % (n, n, n, K, seed), "unique_elements": num, "steps": steps,
+            "enumeration_ms": enum_ms, "methods": out,
+            "erida": {"unique_elements": int(erida.num_entries), "ms_incl_host_copies": erida_ms}}
+
+
 def batch_bench(args):
     """the mixed-size batch alone, as its own JSON line"""
     import torch
@@ -506,6 +563,9 @@ def main():
                     help="skip the one-grid batch-plan replay reported beside the single-call rate")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the mixed-size batch (5^3/7^3/9^3/11^3 plan replay) reported beside the single-call rate")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the largest single-GPU case (--large-n grid, its methods and ERI-DA) in the default line")
+    ap.add_argument("--large-n", type=int, default=13)
     ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
                     help="set a kernel tuning knob (gli_set_tuning) before timing; repeatable")
     ap.add_argument("--batch", action="store_true",
@@ -731,6 +791,13 @@ def main():
                             "host arrays (2nd call), wall clock"}
         log("host API: enumerate %.1f ms, values %.1f ms (diff vs device path %.3g)" % (t_enum[1], t_val[1], diff))
 
+    # the largest single-GPU case (13^3): methods and their errors against exact FP64, and ERI-DA; after
+    # the host-array calls, whose wall clock it could otherwise disturb
+    large_line = None
+    if not args.no_large:
+        large_line = large_measure(L, args, stream)
+        torch.cuda.empty_cache()
+
     # roofline of the dominant kernel
     workload = "eri_%dx%dx%d_K%d_seed%d%s" % (args.n, args.n, args.n, args.K, args.seed, "_asym" if args.asymmetric else "")
     dom = max(phases.items(), key=lambda kv: kv[1]) if phases else ("step", step_ms)
@@ -805,6 +872,7 @@ def main():
         "mixed_vs_fp64": mixed_vs_fp64(compare, args.method, value),
         "plan_replay": plan_line,
         "batch": batch_line,
+        "large": large_line,
     }
     L.gli_device_free(d_idx)
     print(json.dumps(line), flush=True)
