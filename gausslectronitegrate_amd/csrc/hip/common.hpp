@@ -127,6 +127,7 @@ struct Context {
 	                         // against 7.40 ms), 1 always, 2 for key tensors above 128 MB
 	int gather_classed = 0;  // otherwise each wave of the single-grid gather groups its key-tensor reads by line class
 	                         // (with the size-ordered key numbering 1.308 against 1.274 ms at 11^3: off)
+	int rankq_tet = 1;       // separable key tensor: the tetrahedral rank-Q kernel (0: the 64x64 tile kernel)
 	int mode1_half = 1;     // FP64 paths store the first mode's (m2, m3)-symmetric output once
 	int batch_branches = 1; // batch plans: 0 one branch, 1 autotuner times a two-branch graph, 2 always two
 	int batch_positions = 1; // batch plans cache every element's key-tensor position at creation

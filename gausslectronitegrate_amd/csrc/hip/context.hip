@@ -314,6 +314,10 @@ int gli_set_tuning(const char* knob, int value)
 			GLI_REQUIRE(value == 0 || value == 1, "gather_classed must be 0 or 1");
 			c.gather_classed = value;
 		}
+		else if (k == "rankq_tet") {
+			GLI_REQUIRE(value == 0 || value == 1, "rankq_tet must be 0 or 1");
+			c.rankq_tet = value;
+		}
 		else if (k == "project_chunk_kib") {
 			GLI_REQUIRE(value >= 1, "project_chunk_kib must be at least 1");
 			c.project_chunk_kib = value;
@@ -346,6 +350,7 @@ int gli_get_tuning(const char* knob)
 		if (k == "project_chunk_kib") { return c.project_chunk_kib; }
 		if (k == "gather_bucketed") { return c.gather_bucketed; }
 		if (k == "gather_classed") { return c.gather_classed; }
+		if (k == "rankq_tet") { return c.rankq_tet; }
 		GLI_REQUIRE(false, "unknown tuning knob");
 	}
 	catch (const Failure&) {

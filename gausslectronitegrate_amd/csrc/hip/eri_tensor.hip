@@ -442,6 +442,97 @@ __global__ __launch_bounds__(256) void k_mode_product_grouped(ModeGroup grp)
 }
 
 // ---------------------------------------------------------------------------
+// Tetrahedral rank-Q product of the separable method, V[k1 <= k2 <= k3] = sum_q G[q][k1] G[q][k2] U[q][k3],
+// on the tetrahedron only.
+//
+// Work unit: one wave per (k2 block, k1 block) pair of 16 x 16 keys with k1 block <= k2 block, for a chunk
+// of 16 consecutive k3 (chunk c needs the pairs with k2 block <= c: (c + 1)(c + 2) / 2 of them).  The
+// wave keeps its pair's two G fragments in registers for the whole chunk; per k3 only the A operand's
+// scaling by U[q][k3] changes (one multiply per MFMA), and two k3 run as independent accumulator chains.
+// A workgroup of 4 waves takes 4 pairs of one chunk and stages that chunk's U columns in LDS.  The
+// MFMAs run over S = ceil(Q / 4) steps of 4 nodes (rows q >= Q of G and U are zero).
+//
+// v_mfma_f64_16x16x4f64 operands: lane l holds A[l % 16][l / 16] (row k2, node 4s + l / 16) and
+// B[l / 16][l % 16] (node, column k1); the accumulator holds rows l / 16 + 4 v and column l % 16.
+// Each value is the sum of the same products in the same order as the 64 x 64 tile kernel's (A = G U
+// rounded once, MFMA steps over ascending q, the tile kernel's extra steps add exact zeros), so V is
+// bit-identical to it.
+struct RankQProblem {
+	const double* G;     // [q][key], row stride Cp
+	const double* U;     // [q][key], row stride Cp
+	double* V;           // tetrahedral key tensor (tet_index)
+	int Cp, C1;          // Cp >= 16 ceil(C1 / 16): the chunk's U columns past C1 are zero padding
+	int wg_begin;        // first workgroup of this problem in the launch
+};
+struct RankQGroup {
+	RankQProblem p[kMaxGroup];
+	int n;
+};
+
+__host__ __device__ inline int rankq_pairs(int c) { return (c + 1) * (c + 2) / 2; }
+__host__ __device__ inline int rankq_chunk_wgs(int c) { return (rankq_pairs(c) + 3) / 4; }
+
+template <int S>
+__global__ __launch_bounds__(256) void k_rankq_tet(RankQGroup grp)
+{
+	__shared__ double us[4 * S][16];
+	int gi = 0;
+	while (gi + 1 < grp.n && (int)blockIdx.x >= grp.p[gi + 1].wg_begin) { ++gi; }
+	const RankQProblem pr = grp.p[gi];
+	int b = (int)blockIdx.x - pr.wg_begin;
+	const int nch = (pr.C1 + 15) / 16;
+	int c = 0;
+	for (; c < nch; ++c) {
+		const int nw = rankq_chunk_wgs(c);
+		if (b < nw) { break; }
+		b -= nw;
+	}
+	if (c >= nch) { return; }   // whole workgroup past its problem's chunks
+	const int k3lo = 16 * c, nk3 = min(16, pr.C1 - k3lo);
+	for (int i = threadIdx.x; i < 4 * S * 16; i += 256) {
+		const int q = i >> 4, j = i & 15;
+		us[q][j] = pr.U[(int64_t)q * pr.Cp + k3lo + j];
+	}
+	__syncthreads();
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int p = 4 * b + wave;
+	if (p >= rankq_pairs(c)) { return; }
+	int b2 = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+	while (rankq_pairs(b2) <= p) { ++b2; }
+	while (b2 > 0 && rankq_pairs(b2 - 1) > p) { --b2; }
+	const int b1 = p - b2 * (b2 + 1) / 2;
+	const int r = lane & 15, g = lane >> 4;
+	double ga[S], gb[S];
+#pragma unroll
+	for (int s = 0; s < S; ++s) {
+		const int64_t row = (int64_t)(4 * s + g) * pr.Cp;
+		ga[s] = pr.G[row + 16 * b2 + r];
+		gb[s] = pr.G[row + 16 * b1 + r];
+	}
+	const int k1 = 16 * b1 + r;
+	for (int j = 0; j < nk3; j += 2) {
+		d4 acc0 = d4{0, 0, 0, 0}, acc1 = d4{0, 0, 0, 0};
+#pragma unroll
+		for (int s = 0; s < S; ++s) {
+			const double u0 = us[4 * s + g][j], u1 = us[4 * s + g][j + 1];
+			acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(ga[s] * u0, gb[s], acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ga[s] * u1, gb[s], acc1, 0, 0, 0);
+		}
+#pragma unroll
+		for (int h = 0; h < 2; ++h) {
+			const int k3 = k3lo + j + h;
+			if (j + h >= nk3) { break; }
+			const int64_t t3 = (int64_t)k3 * (k3 + 1) * (k3 + 2) / 6;
+#pragma unroll
+			for (int v = 0; v < 4; ++v) {
+				const int k2 = 16 * b2 + g + 4 * v;
+				if (k1 <= k2 && k2 <= k3) { pr.V[t3 + (int64_t)k2 * (k2 + 1) / 2 + k1] = h ? acc1[v] : acc0[v]; }
+			}
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
 // per-element decoding
 
 struct GridDev {
@@ -1472,7 +1563,7 @@ struct EriWorkspace {
 	// separable method: the Gaussian factors [g | W g] of the folded range sep_M (Mp x 2 Qp, cached)
 	// and the rank-Q factors [G; U] of the current evaluation (2 Qp x Cp)
 	DevBuf<double> sep_gw, GU;
-	int sep_M = -1, sep_Qp = 0;
+	int sep_M = -1, sep_Qp = 0, sep_Q = 0;
 	DevBuf<int> coords;
 	DevBuf<double> T3;
 	int T3_M = -1;
@@ -2002,11 +2093,11 @@ void separable_rule(int M, std::vector<long double>& s, std::vector<long double>
 }
 
 // [g | W g] for a folded range M (Mp rows, 2 Qp columns, zero past m = M and q = Q), host memory
-static std::vector<double> separable_gw(int M, int Mp, int& Qp)
+static std::vector<double> separable_gw(int M, int Mp, int& Qp, int& Q)
 {
 	std::vector<long double> sq, wq;
 	separable_rule(M, sq, wq);
-	const int Q = (int)sq.size();
+	Q = (int)sq.size();
 	Qp = (int)ceil_div(Q, 16) * 16;
 	const long double C = 6.0L / sqrtl(2.0L * 3.141592653589793238462643383279502884L);
 	std::vector<double> gw((size_t)Mp * 2 * Qp, 0.0);
@@ -2024,13 +2115,14 @@ static std::vector<double> separable_gw(int M, int Mp, int& Qp)
 static void separable_factors(EriWorkspace& w, const EriSetup& s)
 {
 	if (w.sep_M == s.M) { return; }
-	int Qp = 0;
-	const std::vector<double> gw = separable_gw(s.M, s.Mp, Qp);
+	int Qp = 0, Q = 0;
+	const std::vector<double> gw = separable_gw(s.M, s.Mp, Qp, Q);
 	GLI_HIP(hipStreamSynchronize(ctx().stream));   // an earlier evaluation may still read the old factors
 	w.sep_gw.alloc(gw.size());
 	GLI_HIP(hipMemcpy(w.sep_gw.p, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
 	w.sep_M = s.M;
 	w.sep_Qp = Qp;
+	w.sep_Q = Q;
 }
 
 // [G; U][r][key] = sum_mu gw[mu][r] B[mu][key] straight from the pair products: B[mu][key] sums the P'
@@ -2095,6 +2187,42 @@ static void mode_product_cp(const ModeArgs& a)
 	GLI_HIP(hipGetLastError());
 }
 
+// The tetrahedral rank-Q kernel over n problems with Q nodes (G and U hold Qp >= 4 ceil(Q / 4) rows).
+// Returns false, having launched nothing, when the knob selects the tile kernel or no instance covers
+// S = ceil(Q / 4) steps; the caller then runs the tile kernel.
+static bool rankq_tet_launch(const RankQProblem* probs, int n, int Q, int Qp)
+{
+	if (ctx().rankq_tet == 0 || n < 1 || n > kMaxGroup) { return false; }
+	const int S = (Q + 3) / 4;
+	GLI_REQUIRE(Q >= 1 && 4 * S <= Qp, "rank-Q product: the factors need 4 ceil(Q / 4) rows");
+	RankQGroup grp;
+	grp.n = n;
+	int wgs = 0;
+	for (int g = 0; g < n; ++g) {
+		grp.p[g] = probs[g];
+		grp.p[g].wg_begin = wgs;
+		const int nch = (probs[g].C1 + 15) / 16;
+		GLI_REQUIRE(16 * nch <= probs[g].Cp, "rank-Q product: the key padding must cover whole chunks of 16");
+		for (int c = 0; c < nch; ++c) { wgs += rankq_chunk_wgs(c); }
+	}
+	if (wgs == 0) { return true; }
+	hipStream_t st = ctx().stream;
+	switch (S) {
+		case 2: k_rankq_tet<2><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		case 3: k_rankq_tet<3><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		case 5: k_rankq_tet<5><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		case 9: k_rankq_tet<9><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		case 13: k_rankq_tet<13><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		case 17: k_rankq_tet<17><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		case 21: k_rankq_tet<21><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		case 25: k_rankq_tet<25><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		case 29: k_rankq_tet<29><<<(unsigned)wgs, 256, 0, st>>>(grp); break;
+		default: return false;   // the rule's node counts are 5, 11, 20, 34, 50, 66, 82, 98, 114, 130, ...
+	}
+	GLI_HIP(hipGetLastError());
+	return true;
+}
+
 // [G; U] of one grid from its pair products P' (library stream)
 static void separable_gu(const EriSetup& s, const double* P, const double* gw, int Qp, double* GU)
 {
@@ -2113,9 +2241,12 @@ static void key_tensor_separable(const EriSetup& s, EriWorkspace& w)
 	w.GU.reserve((size_t)(2 * Qp * Cp));
 	separable_gu(s, w.P.p, w.sep_gw.p, (int)Qp, w.GU.p);
 	phase_mark("separable_factors");
-	ModeArgs a{w.GU.p, Cp, Cp * Cp, w.GU.p, Cp, s.Cp, (int)Qp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1};
-	a.cp_u = w.GU.p + Qp * Cp;
-	mode_product_cp(a);
+	const RankQProblem pr{w.GU.p, w.GU.p + Qp * Cp, w.V.p, (int)Cp, s.C1, 0};
+	if (!rankq_tet_launch(&pr, 1, w.sep_Q, (int)Qp)) {
+		ModeArgs a{w.GU.p, Cp, Cp * Cp, w.GU.p, Cp, s.Cp, (int)Qp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1};
+		a.cp_u = w.GU.p + Qp * Cp;
+		mode_product_cp(a);
+	}
 	phase_mark("separable_tensor");
 }
 
@@ -2398,7 +2529,7 @@ struct BatchGrid {
 	DevBuf<double> P, B, Y1, Y2, V;
 	DevBuf<double> GU;      // separable plans: the rank-Q factors [G; U] (2 Qp x Cp)
 	const double* sep_gw = nullptr;   // ... and the Gaussian factors [g | W g] of the grid's folded range
-	int sep_Qp = 0;
+	int sep_Qp = 0, sep_Q = 0;
 	DevBuf<uint32_t> pos;   // key-tensor position of every element (plans with cached positions)
 	DevBuf<uint16_t> perm;  // place of each sorted position in its window (plans with sorted windows)
 	const double* T3 = nullptr;
@@ -2497,19 +2628,27 @@ static ModeArgs batch_sep_args(const BatchGrid& bg)
 }
 
 // separable plans: every grid's factors [G; U] from its pair products, then one grouped rank-Q product
-// over all grids (the single-grid path's kernels, so bit-identical to it)
+// over all grids (the single-grid path's kernels, so bit-identical to it; the tile order applies to the
+// tile kernel only)
 static void batch_separable(BatchPlan& plan, int order)
 {
 	ModeArgs a[kMaxGroup];
-	int n = 0;
+	RankQProblem pr[kMaxGroup];
+	int n = 0, Q = -1, Qp = 0;
+	bool same_q = true;
 	for (auto& g : plan.grids) {
 		const EriSetup& s = g->s;
-		const int64_t Qp = g->sep_Qp;
-		separable_gu(s, g->P.p, g->sep_gw, (int)Qp, g->GU.p);
+		const int64_t Qg = g->sep_Qp;
+		separable_gu(s, g->P.p, g->sep_gw, (int)Qg, g->GU.p);
 		a[n] = batch_sep_args(*g);
+		pr[n] = RankQProblem{g->GU.p, g->GU.p + Qg * s.Cp, g->V.p, s.Cp, s.C1, 0};
+		same_q = same_q && (Q < 0 || g->sep_Q == Q);
+		Q = g->sep_Q;
+		Qp = (n == 0) ? (int)Qg : std::min(Qp, (int)Qg);
 		++n;
 	}
-	mode_product_grouped(a, n, 0, order);
+	// the tetrahedral kernel takes one node count per launch; the tile kernel any mix
+	if (!(same_q && rankq_tet_launch(pr, n, Q, Qp))) { mode_product_grouped(a, n, 0, order); }
 }
 
 // the gather of a batch plan: from the cached positions when the plan holds them
@@ -2840,8 +2979,8 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 			for (auto& t : plan->sep_gw) {
 				if (t.first == s.M) { bg->sep_gw = t.second->p; }
 			}
-			int Qp = 0;
-			const std::vector<double> gw = separable_gw(s.M, s.Mp, Qp);
+			int Qp = 0, Q = 0;
+			const std::vector<double> gw = separable_gw(s.M, s.Mp, Qp, Q);
 			if (bg->sep_gw == nullptr) {
 				std::unique_ptr<DevBuf<double>> d(new DevBuf<double>(gw.size()));
 				GLI_HIP(hipMemcpy(d->p, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -2849,6 +2988,7 @@ static BatchPlan* batch_create(int n, const double* const* coeffs, const int64_t
 				plan->sep_gw.emplace_back(s.M, std::move(d));
 			}
 			bg->sep_Qp = Qp;
+			bg->sep_Q = Q;
 			bg->GU.alloc((size_t)2 * Qp * s.Cp);
 			bg->coeff_off = params;
 			params += (size_t)K[g];

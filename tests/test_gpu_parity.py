@@ -561,6 +561,38 @@ def test_tuning_variants_bit_identical(L):
             L.gli_set_tuning(k, v)
 
 
+def _coeffs_sym(K, seed, mirror=True):
+    """seeded coefficients of K entries (K odd), mirror-symmetric unless asked otherwise"""
+    z = np.random.default_rng(seed).standard_normal(K) * np.exp(-(4 * np.arange(-(K // 2), K // 2 + 1) / 57.0) ** 2)
+    return 0.5 * (z + z[::-1]) if mirror else z
+
+
+@pytest.mark.parametrize("n,K,mirror", [(3, 3, True), (5, 15, True), (7, 57, True), (7, 89, True), (5, 57, False)])
+def test_rankq_tet_bit_identical_to_tile_kernel(L, n, K, mirror):
+    """the separable key tensor from the tetrahedral rank-Q kernel (16 x 16 key blocks per wave over chunks of
+    16 k3) equals the 64 x 64 tile kernel's bit for bit on every element of the full n^3 list: both add the
+    same products in the same order.  The cases take S = ceil(Q / 4) = 13, 17, 21, 25 MFMA steps
+    (folded ranges M = 16, 52, 148, 212) and a coefficient set without mirror symmetry (n^3 keys per axis)"""
+    say("rankq tet", n, K, mirror)
+    c = _coeffs_sym(K, 11 + n + K, mirror)
+    d, num = eri_enumerate_device(n)
+    idx = d.to_numpy(np.int64, num)
+    saved = L.gli_get_tuning(b"rankq_tet")
+    try:
+        assert L.gli_set_tuning(b"rankq_tet", 0) == 0
+        tile = eri_values_device(c, sym(n), idx, 1e-16, 6)
+        assert L.gli_set_tuning(b"rankq_tet", 1) == 0
+        tet = eri_values_device(c, sym(n), idx, 1e-16, 6)
+        assert np.all(np.isfinite(tet))
+        assert np.array_equal(tet.view(np.uint64), tile.view(np.uint64)), float(np.max(np.abs(tet - tile)))
+        exact = eri_values_device(c, sym(n), idx, 1e-16, 0)
+        assert np.max(np.abs(tet - exact)) <= 4e-15 * max(1.0, float(np.max(np.abs(exact))))
+        assert L.gli_set_tuning(b"rankq_tet", 2) == -1
+    finally:
+        L.gli_clear_error()
+        L.gli_set_tuning(b"rankq_tet", saved)
+
+
 # ---------------------------------------------------------------------------
 # batch plans: several grids, grouped launches, autotuner, one replayed hipGraph
 
