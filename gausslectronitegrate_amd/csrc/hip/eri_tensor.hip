@@ -710,12 +710,24 @@ struct GatherDesc {
 };
 constexpr uint32_t kNoPosition = 0xffffffffu;
 
-// LDS bytes a workgroup needs to hold the key table and the tetrahedral offsets
+// Bytes of the key-table image: the tetrahedral offsets (C1 uint32, padded to a multiple of 4), then
+// the 16-bit key table (nlut entries), zero-padded to a multiple of 16 bytes.  The image has this
+// layout in device memory (16-byte aligned, starting at GatherDesc::tet3) and in LDS, so a workgroup
+// copies it with one 16-byte load per lane and step instead of one 2-byte load per table entry.
 __host__ __device__ inline int gather_lds_bytes(int nlut, int C1)
 {
-	return 4 * ((C1 + 3) & ~3) + 2 * nlut;
+	return (4 * ((C1 + 3) & ~3) + 2 * nlut + 15) & ~15;
 }
 constexpr int kGatherLdsMax = 48 * 1024;
+
+// the key-table image into LDS (11^3: 717 pieces of 16 bytes, one step of a 1024-lane workgroup)
+__device__ inline void copy_table_image(const GatherDesc& d, void* lds)
+{
+	const uint4* __restrict__ src = (const uint4*)d.tet3;
+	uint4* dst = (uint4*)lds;
+	const int nvec = gather_lds_bytes(d.nlut, d.C1) >> 4;
+	for (int q = threadIdx.x; q < nvec; q += blockDim.x) { dst[q] = src[q]; }
+}
 
 // Key-tensor positions of one wave's GATHER_EPT groups of 64 consecutive elements (-1 past the end of
 // the list and for indices outside the grid).  lut and tet3 point to the descriptor's tables in global
@@ -853,14 +865,13 @@ __device__ inline int64_t gather_piece(int64_t xcd_chunks)
 template <int GATHER_EPT, bool kLds>
 __device__ inline void gather_block(const GatherDesc& d, int64_t wave_base)
 {
-	extern __shared__ uint32_t gather_lds[];
+	extern __shared__ __align__(16) uint32_t gather_lds[];
 	const short* lut = d.lut;
 	const uint32_t* tet3 = d.tet3;
 	if (kLds) {
 		uint32_t* st = gather_lds;
 		short* sl = (short*)(gather_lds + ((d.C1 + 3) & ~3));
-		for (int q = threadIdx.x; q < d.C1; q += blockDim.x) { st[q] = d.tet3[q]; }
-		for (int q = threadIdx.x; q < d.nlut; q += blockDim.x) { sl[q] = d.lut[q]; }
+		copy_table_image(d, gather_lds);
 		__syncthreads();
 		lut = sl;
 		tet3 = st;
@@ -883,13 +894,12 @@ __global__ __launch_bounds__(1024) void k_gather(GatherDesc d, int64_t xcd_chunk
 // outside the grid), with the key tables in LDS
 __global__ __launch_bounds__(256) void k_positions(GatherDesc d)
 {
-	extern __shared__ uint32_t gather_lds[];
+	extern __shared__ __align__(16) uint32_t gather_lds[];
 	const int64_t wave_base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 * 8);
 	if ((int64_t)blockIdx.x * (blockDim.x >> 6) * (64 * 8) >= d.num) { return; }   // whole workgroup idle
 	uint32_t* st = gather_lds;
 	short* sl = (short*)(gather_lds + ((d.C1 + 3) & ~3));
-	for (int q = threadIdx.x; q < d.C1; q += blockDim.x) { st[q] = d.tet3[q]; }
-	for (int q = threadIdx.x; q < d.nlut; q += blockDim.x) { sl[q] = d.lut[q]; }
+	copy_table_image(d, gather_lds);
 	__syncthreads();
 	if (wave_base >= d.num) { return; }
 	const int lane = threadIdx.x & 63;
@@ -1065,8 +1075,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_bucketed(GatherDesc d, int6
 	// phase 1: key tables into LDS, then every element's position (registers)
 	uint32_t* st = (uint32_t*)lds;
 	short* sl = (short*)(lds + 4 * ((d.C1 + 3) & ~3));
-	for (int q = tid; q < d.C1; q += kThreads) { st[q] = d.tet3[q]; }
-	for (int q = tid; q < d.nlut; q += kThreads) { sl[q] = d.lut[q]; }
+	copy_table_image(d, lds);
 	__syncthreads();
 	uint32_t pos[kBucketEpt];
 	const int first = wave * (64 * kBucketEpt);   // this wave's first place in the window
@@ -1289,8 +1298,7 @@ __global__ __launch_bounds__(1024) void k_gather_classed(GatherDesc d, int64_t x
 	if (kLds) {
 		uint32_t* st = gather_lds;
 		short* sl = (short*)(gather_lds + ((d.C1 + 3) & ~3));
-		for (int q = threadIdx.x; q < d.C1; q += blockDim.x) { st[q] = d.tet3[q]; }
-		for (int q = threadIdx.x; q < d.nlut; q += blockDim.x) { sl[q] = d.lut[q]; }
+		copy_table_image(d, gather_lds);
 		__syncthreads();
 		lut = sl;
 		tet3 = st;
@@ -1469,12 +1477,12 @@ struct KeySet {
 	std::vector<int> lut;    // (a * R + b) * (4R - 3) + cd + 2(R - 1) -> key id
 	std::vector<short> lut16;    // the same table as int16 (key ids stay below 32768)
 	std::vector<uint32_t> tet3;  // tet_index(0, 0, k) per key
+	std::vector<uint32_t> image; // the gather's key-table image: tet3, then lut16 (gather_lds_bytes long)
 	bool tet32 = false;         // every tetrahedral index fits 32 bits
 	bool on_device = false;     // the DevBuf copies below exist
 	DevBuf<int4> keys_dev;
 	DevBuf<int> lut_dev;
-	DevBuf<short> lut16_dev;
-	DevBuf<uint32_t> tet3_dev;
+	DevBuf<uint32_t> image_dev;  // tet3 and lut16 on the device, as one image
 };
 
 // the key set of per-axis length R in host memory, built once per (R, symmetric) and kept
@@ -1536,6 +1544,9 @@ static KeySet& key_tables(int R, bool symmetric)
 	ks->tet32 = tet_index(0, 0, ks->C1) <= (int64_t)UINT32_MAX;
 	ks->tet3.resize((size_t)ks->C1);
 	for (int k = 0; k < ks->C1; ++k) { ks->tet3[(size_t)k] = ks->tet32 ? (uint32_t)tet_index(0, 0, k) : 0u; }
+	ks->image.assign((size_t)gather_lds_bytes((int)ks->lut16.size(), ks->C1) / 4, 0u);
+	std::memcpy(ks->image.data(), ks->tet3.data(), ks->tet3.size() * sizeof(uint32_t));
+	std::memcpy(ks->image.data() + ((ks->C1 + 3) & ~3), ks->lut16.data(), ks->lut16.size() * sizeof(short));
 	cache[{R, symmetric}] = ks;
 	return *ks;
 }
@@ -1547,12 +1558,10 @@ static KeySet& key_set(int R, bool symmetric)
 	if (!ks.on_device) {
 		ks.keys_dev.alloc(ks.keys.size());
 		ks.lut_dev.alloc(ks.lut.size());
-		ks.lut16_dev.alloc(ks.lut16.size());
-		ks.tet3_dev.alloc(ks.tet3.size());
+		ks.image_dev.alloc(ks.image.size());
 		GLI_HIP(hipMemcpy(ks.keys_dev.p, ks.keys.data(), ks.keys.size() * sizeof(int4), hipMemcpyHostToDevice));
 		GLI_HIP(hipMemcpy(ks.lut_dev.p, ks.lut.data(), ks.lut.size() * sizeof(int), hipMemcpyHostToDevice));
-		GLI_HIP(hipMemcpy(ks.lut16_dev.p, ks.lut16.data(), ks.lut16.size() * sizeof(short), hipMemcpyHostToDevice));
-		GLI_HIP(hipMemcpy(ks.tet3_dev.p, ks.tet3.data(), ks.tet3.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+		GLI_HIP(hipMemcpy(ks.image_dev.p, ks.image.data(), ks.image.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
 		ks.on_device = true;
 	}
 	return ks;
@@ -1633,8 +1642,8 @@ static EriSetup eri_shape(int64_t K, const int64_t grid[6], bool sym, bool devic
 	s.g.N4 = s.g.N2 * s.g.N2;
 	s.T = device ? eri_table(3 * (int64_t)s.M * s.M + 1) : nullptr;
 	s.d_keys = device ? ks.keys_dev.p : nullptr;
-	s.d_lut16 = device ? ks.lut16_dev.p : nullptr;
-	s.d_tet3 = device ? ks.tet3_dev.p : nullptr;
+	s.d_tet3 = device ? ks.image_dev.p : nullptr;   // the start of the key-table image
+	s.d_lut16 = device ? (const short*)(ks.image_dev.p + ((ks.C1 + 3) & ~3)) : nullptr;
 	return s;
 }
 
@@ -1732,12 +1741,13 @@ static void gather_dispatch(int ept, bool lds, F&& f)
 	}
 }
 
-// Gather order for a key tensor of nV entries.  One chunk of the list per XCD keeps each XCD's
-// key-tensor lines in its own L2.  Past a threshold of 128 MB of key tensor (half of the 256 MB
-// shared last-level cache) consecutive pieces win instead: all XCDs then read neighbouring parts
-// of the list and share the cached lines (13^3, 299 MB: 7.93 against 8.22 ms; 11^3, 71 MB: 1.444
-// against 1.451 ms, for twice the L2-to-fabric bytes).  Enumerated grids have odd sides, so no
-// tensor between these two sizes has been measured.
+// Gather order for a key tensor of nV entries (knob value 2; the default is 0, consecutive pieces).
+// One chunk of the list per XCD keeps each XCD's key-tensor lines in its own L2.  Past a threshold of
+// 128 MB of key tensor (half of the 256 MB shared last-level cache) consecutive pieces win instead: all
+// XCDs then read neighbouring parts of the list and share the cached lines (first-encounter key
+// numbering: 13^3, 299 MB: 7.93 against 8.22 ms; 11^3, 71 MB: 1.444 against 1.451 ms, for twice the
+// L2-to-fabric bytes).  With the size-ordered numbering, the smallest key contiguous and the 16-byte
+// table copy, consecutive pieces also win at 11^3 (1.113 against 1.121 ms).
 constexpr int64_t kLargeKeyTensorEntries = ((int64_t)128 << 20) / (int64_t)sizeof(double);
 
 static int resolve_gather_order(int knob, int64_t nV)
@@ -2435,7 +2445,6 @@ static void eri_values_noalloc(const double* coeffs, int64_t K, const int64_t gr
 	EriSetup s = eri_shape(K, grid, mirror_symmetric(coeffs, K), false);
 	GLI_REQUIRE(s.ks->tet32, "key tensor too large for 32-bit tetrahedral offsets");
 	const int64_t Mp = s.Mp, Cp = s.Cp, nT = 3 * (int64_t)s.M * s.M + 1;
-	const int64_t nlut = (int64_t)s.R * s.R * (4 * s.R - 3);
 	ScratchCarve sc{base, kEriScratchBytes};
 	double* d_coeffs = sc.take<double>(K);
 	double* P = sc.take<double>((int64_t)s.R * s.R * (4 * s.K - 3));
@@ -2446,8 +2455,7 @@ static void eri_values_noalloc(const double* coeffs, int64_t K, const int64_t gr
 	double* Y2 = sc.take<double>(Mp * Cp * Cp);
 	double* V = sc.take<double>(tet_index(0, 0, s.C1));
 	int4* keys = sc.take<int4>(s.C1);
-	short* lut16 = sc.take<short>(nlut);
-	uint32_t* tet3 = sc.take<uint32_t>(s.C1);
+	uint32_t* image = sc.take<uint32_t>((int64_t)s.ks->image.size());   // tet3 and lut16, as the gather reads them
 	if (sc.used > sc.cap) {
 		char msg[256];
 		snprintf(msg, sizeof(msg), "the allocation-free entry point needs %.1f MB of scratch for this grid and K, it holds %zu MB "
@@ -2460,13 +2468,12 @@ static void eri_values_noalloc(const double* coeffs, int64_t K, const int64_t gr
 	GLI_HIP(hipMemcpyAsync(d_coeffs, coeffs, (size_t)K * sizeof(double), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(T, hT, (size_t)nT * sizeof(double), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipMemcpyAsync(keys, s.ks->keys.data(), (size_t)s.C1 * sizeof(int4), hipMemcpyHostToDevice, st));
-	GLI_HIP(hipMemcpyAsync(lut16, s.ks->lut16.data(), (size_t)nlut * sizeof(short), hipMemcpyHostToDevice, st));
-	GLI_HIP(hipMemcpyAsync(tet3, s.ks->tet3.data(), (size_t)s.C1 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+	GLI_HIP(hipMemcpyAsync(image, s.ks->image.data(), s.ks->image.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
 	GLI_HIP(hipStreamSynchronize(st));
 	s.T = T;
 	s.d_keys = keys;
-	s.d_lut16 = lut16;
-	s.d_tet3 = tet3;
+	s.d_tet3 = image;
+	s.d_lut16 = (const short*)(image + ((s.C1 + 3) & ~3));
 	// the same kernels in the same order as gli_eri_values_device with the FP64 method
 	phase_begin();
 	launch_tables(s, d_coeffs, tol, P, B);

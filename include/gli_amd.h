@@ -64,17 +64,20 @@ int gli_synchronize(void);
  *                 8192 consecutive elements (keeping each one's place), so every replay reads the key tensor
  *                 in ascending order within a window and stores the values back in list order through LDS;
  *                 0: positions in list order
- *   "gather_order" 0: workgroups take consecutive pieces of the list; 1: each XCD takes one contiguous
- *                 chunk of the list (its L2 then holds the key-tensor entries it reuses); 2 (default): 1
+ *   "gather_order" 0 (default): workgroups take consecutive pieces of the list; 1: each XCD takes one
+ *                 contiguous chunk of the list (its L2 then holds the key-tensor entries it reuses); 2: 1
  *                 while the key tensor is at most 128 MB (11^3: 71 MB), else 0 (13^3: 299 MB)
  *   "gather_bucketed" the single-grid gather decodes a window of 8 x gather_threads elements, counting-sorts
  *                 the window's key-tensor positions by bucket in LDS and reads the key tensor in that order
  *                 (the elements of one cache line share a load), storing the values back in list order:
- *                 0 never, 1 always, 2 (default) when the key tensor exceeds 128 MB (13^3: 299 MB)
+ *                 0 (default) never, 1 always, 2 when the key tensor exceeds 128 MB (13^3: 299 MB)
  *   "gather_classed" 1: where the bucketed form is not chosen, each wave of the single-grid gather
  *                 (8 groups of 64 elements) issues its 8 key-tensor loads class by class, class = the
  *                 element's 128-byte line mod 8, so the elements of one line share a load; 0 (default):
  *                 list order
+ *   "rankq_tet" 1 (default): the separable method's key tensor comes from the tetrahedral rank-Q kernel
+ *                 (16 x 16 key blocks per wave, only blocks that meet k1 <= k2 <= k3); 0: from the 64 x 64
+ *                 tile kernel; both give the same bits
  *   "project_chunk_kib" host memory (KiB, default 262144 = 256 MB) of one chunk of product states in
  *                 project_sparse_eri_gausslet_integrals; the states and as many results are held per chunk
  * gli_set_tuning returns 0, or -1 for an unknown knob or value; gli_get_tuning returns the value or -1. */
