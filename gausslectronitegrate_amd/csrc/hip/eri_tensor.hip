@@ -2174,6 +2174,60 @@ __global__ __launch_bounds__(256) void k_separable_gu(const double* __restrict__
 	}
 }
 
+// The separable method's tables in one launch: one workgroup per key column (a, b, cd) computes the two
+// factor rows f_a and f_b (factor_row), its shift pair's products P'_ab (the k_products sums, zeroed where
+// |P| <= tol) and the column's [G; U] (the k_separable_gu sums), all in LDS, with the same terms in the same
+// order as those two kernels.  The 2R - 1 or so keys of one shift pair recompute the same P', which costs
+// less than a second launch.  Dynamic LDS: K coefficients, 2K - 1 Gaussians, two factor rows, 4K - 3 products.
+__global__ __launch_bounds__(256) void k_separable_tables(const double* __restrict__ c, CoeffArg ca, int K, int R,
+                                                          double tol, const int4* __restrict__ keys, int C1, int M,
+                                                          const double* __restrict__ gw, int rows, int Cp,
+                                                          double* __restrict__ GU)
+{
+	extern __shared__ double lds[];
+	const int key = blockIdx.x;
+	if (key >= C1) {
+		for (int r = threadIdx.x; r < rows; r += blockDim.x) { GU[(int64_t)r * Cp + key] = 0.0; }
+		return;
+	}
+	const int L = 2 * K - 1, LP = 4 * K - 3, off = 2 * K - 2;
+	double* cs = lds;
+	double* ex = cs + K;
+	double* fa = ex + L;
+	double* fb = fa + L;
+	double* ps = fb + L;
+	const int4 k = keys[key];
+	for (int q = threadIdx.x; q < K; q += blockDim.x) { cs[q] = (c != nullptr) ? c[q] : ca.c[q]; }
+	__syncthreads();
+	factor_row(cs, K, k.x, ex, fa);
+	__syncthreads();
+	factor_row(cs, K, k.y, ex, fb);
+	__syncthreads();
+	for (int t = threadIdx.x; t < LP; t += blockDim.x) {
+		const int d = t - off;
+		double acc = 0;
+		const int mlo = max(0, d), mhi = min(L - 1, L - 1 + d);
+		for (int m = mlo; m <= mhi; ++m) { acc += fa[m] * fb[m - d]; }
+		ps[t] = (fabs(acc) > tol) ? acc : 0.0;
+	}
+	__syncthreads();
+	const int shift = 3 * k.z - off;
+	const int tlo = max(0, -M - shift), thi = min(LP - 1, M - shift);
+	for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+		double acc[4] = { 0, 0, 0, 0 };
+		int t = tlo;
+		for (; t + 16 <= thi + 1; t += 16) {
+			double g[16];
+#pragma unroll
+			for (int u = 0; u < 16; ++u) { g[u] = gw[(int64_t)abs(t + u + shift) * rows + r]; }
+#pragma unroll
+			for (int u = 0; u < 16; ++u) { acc[u & 3] = fma(ps[t + u], g[u], acc[u & 3]); }
+		}
+		for (; t <= thi; ++t) { acc[0] = fma(ps[t], gw[(int64_t)abs(t + shift) * rows + r], acc[0]); }
+		GU[(int64_t)r * Cp + key] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+	}
+}
+
 // V[k1 <= k2 <= k3] = sum_q U[q][k3] G[q][k2] G[q][k1], tetrahedral store (the rank-Q tile kernel; a tile's
 // rows p = k3 Cp + k2 share k3, its columns run over k1)
 static void mode_product_cp(const ModeArgs& a)
@@ -2242,15 +2296,27 @@ static void separable_gu(const EriSetup& s, const double* P, const double* gw, i
 	GLI_HIP(hipGetLastError());
 }
 
-// The key tensor of the separable method: the rank-Q factors G and U from the pair products, then the
-// rank-Q product into V.  Queued on the library stream.
-static void key_tensor_separable(const EriSetup& s, EriWorkspace& w)
+// The key tensor of the separable method, queued on the library stream: the rank-Q factors G and U
+// straight from the coefficients (d_coeffs on the device, or h_coeffs of at most kArgCoeffs entries as a
+// kernel argument) in one k_separable_tables launch, then the rank-Q product into V.  No pair-product
+// launch precedes it.
+static void key_tensor_separable(const EriSetup& s, EriWorkspace& w, const double* d_coeffs, const double* h_coeffs,
+                                 double tol)
 {
+	GLI_REQUIRE(tol >= 0, "tolerance must be non-negative");
 	separable_factors(w, s);
 	const int64_t Qp = w.sep_Qp, Cp = s.Cp;
 	w.GU.reserve((size_t)(2 * Qp * Cp));
-	separable_gu(s, w.P.p, w.sep_gw.p, (int)Qp, w.GU.p);
-	phase_mark("separable_factors");
+	CoeffArg ca{};
+	if (d_coeffs == nullptr) {
+		GLI_REQUIRE(h_coeffs != nullptr && s.K <= kArgCoeffs, "coefficients by argument: at most 64 of them");
+		for (int q = 0; q < s.K; ++q) { ca.c[q] = h_coeffs[q]; }
+	}
+	const size_t shm = (size_t)(s.K + 3 * (2 * s.K - 1) + (4 * s.K - 3)) * sizeof(double);
+	k_separable_tables<<<(unsigned)Cp, 256, shm, ctx().stream>>>(d_coeffs, ca, s.K, s.R, tol, s.d_keys, s.C1, s.M,
+	                                                             w.sep_gw.p, (int)(2 * Qp), (int)Cp, w.GU.p);
+	GLI_HIP(hipGetLastError());
+	phase_mark("tables");
 	const RankQProblem pr{w.GU.p, w.GU.p + Qp * Cp, w.V.p, (int)Cp, s.C1, 0};
 	if (!rankq_tet_launch(&pr, 1, w.sep_Q, (int)Qp)) {
 		ModeArgs a{w.GU.p, Cp, Cp * Cp, w.GU.p, Cp, s.Cp, (int)Qp, w.V.p, 0, s.Cp, 1, 1, 0, s.C1};
@@ -2318,6 +2384,12 @@ static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t gr
 		upload_async(w.coeffs.p, coeffs, K * sizeof(double));
 		d_coeffs = w.coeffs.p;
 	}
+	if (method == GLI_ERI_SEPARABLE_FP64 && any) {
+		// one launch for the pair products and the rank-Q factors (k_separable_tables), then the rank-Q product
+		w.V.reserve((size_t)tet_index(0, 0, s.C1));
+		key_tensor_separable(s, w, d_coeffs, coeffs, tol);
+		return ep;
+	}
 	launch_tables(s, d_coeffs, tol, w.P.p, method == GLI_ERI_SEPARABLE_FP64 ? nullptr : w.B.p, coeffs);
 	phase_mark("tables");
 	if (!any) { return ep; }
@@ -2335,11 +2407,6 @@ static EriPrepared eri_prepare(const double* coeffs, int64_t K, const int64_t gr
 		}
 		ep.direct_shm = 3 * (size_t)s.Mp * sizeof(double) + DIRECT_CHUNK * (sizeof(double) + sizeof(int));
 		GLI_REQUIRE(ep.direct_shm <= 65536, "grid too long for the per-element path: 6(n-1) + 2(K-1) must stay below 1700");
-		return ep;
-	}
-	if (method == GLI_ERI_SEPARABLE_FP64) {
-		w.V.reserve((size_t)tet_index(0, 0, s.C1));
-		key_tensor_separable(s, w);
 		return ep;
 	}
 	const bool mixed = (method == GLI_ERI_TENSOR_MIXED || method == GLI_ERI_TENSOR_BF16X2);
