@@ -64,8 +64,8 @@ int gli_synchronize(void);
  *                 8192 consecutive elements (keeping each one's place), so every replay reads the key tensor
  *                 in ascending order within a window and stores the values back in list order through LDS;
  *                 0: positions in list order
- *   "gather_order" 0 (default): workgroups take consecutive pieces of the list; 1: each XCD takes one
- *                 contiguous chunk of the list (its L2 then holds the key-tensor entries it reuses); 2: 1
+ *   "gather_order" 0: workgroups take consecutive pieces of the list; 1: each XCD takes one contiguous
+ *                 chunk of the list (its L2 then holds the key-tensor entries it reuses); 2 (default): 1
  *                 while the key tensor is at most 128 MB (11^3: 71 MB), else 0 (13^3: 299 MB)
  *   "gather_bucketed" the single-grid gather decodes a window of 8 x gather_threads elements, counting-sorts
  *                 the window's key-tensor positions by bucket in LDS and reads the key tensor in that order
