@@ -2180,13 +2180,13 @@ __global__ __launch_bounds__(256) void k_separable_gu(const double* __restrict__
 // |P| <= tol) and the column's [G; U] (the k_separable_gu sums), all in LDS, with the same terms in the same
 // order as those two kernels.  The 2R - 1 or so keys of one shift pair recompute the same P', which costs
 // less than a second launch.  Dynamic LDS: K coefficients, 2K - 1 Gaussians, two factor rows, 4K - 3 products.
-__global__ __launch_bounds__(256) void k_separable_tables(const double* __restrict__ c, CoeffArg ca, int K, int R,
-                                                          double tol, const int4* __restrict__ keys, int C1, int M,
-                                                          const double* __restrict__ gw, int rows, int Cp,
-                                                          double* __restrict__ GU)
+// The coefficients come from device memory c, or from ca when c is null.
+__device__ inline void separable_tables_column(const double* __restrict__ c, const CoeffArg& ca, int K, int R,
+                                               double tol, const int4* __restrict__ keys, int C1, int M,
+                                               const double* __restrict__ gw, int rows, int Cp,
+                                               double* __restrict__ GU, int key)
 {
 	extern __shared__ double lds[];
-	const int key = blockIdx.x;
 	if (key >= C1) {
 		for (int r = threadIdx.x; r < rows; r += blockDim.x) { GU[(int64_t)r * Cp + key] = 0.0; }
 		return;
@@ -2227,6 +2227,40 @@ __global__ __launch_bounds__(256) void k_separable_tables(const double* __restri
 		for (; t <= thi; ++t) { acc[0] = fma(ps[t], gw[(int64_t)abs(t + shift) * rows + r], acc[0]); }
 		GU[(int64_t)r * Cp + key] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 	}
+}
+
+__global__ __launch_bounds__(256) void k_separable_tables(const double* __restrict__ c, CoeffArg ca, int K, int R,
+                                                          double tol, const int4* __restrict__ keys, int C1, int M,
+                                                          const double* __restrict__ gw, int rows, int Cp,
+                                                          double* __restrict__ GU)
+{
+	separable_tables_column(c, ca, K, R, tol, keys, C1, M, gw, rows, Cp, GU, (int)blockIdx.x);
+}
+
+// The same for every grid of a batch plan in one launch: problem g owns workgroups wg_begin .. wg_begin + Cp - 1
+// (one per key column); its coefficients are in device memory.
+struct SepTablesProblem {
+	const double* c;
+	const int4* keys;
+	const double* gw;
+	double* GU;
+	double tol;
+	int K, R, C1, M, rows, Cp;
+	int wg_begin;
+};
+struct SepTablesGroup {
+	SepTablesProblem p[kMaxGroup];
+	int n;
+};
+
+__global__ __launch_bounds__(256) void k_separable_tables_grouped(SepTablesGroup grp)
+{
+	int g = 0;
+	while (g + 1 < grp.n && (int)blockIdx.x >= grp.p[g + 1].wg_begin) { ++g; }
+	const SepTablesProblem& pr = grp.p[g];
+	const CoeffArg none{};
+	separable_tables_column(pr.c, none, pr.K, pr.R, pr.tol, pr.keys, pr.C1, pr.M, pr.gw, pr.rows, pr.Cp, pr.GU,
+	                        (int)blockIdx.x - pr.wg_begin);
 }
 
 // V[k1 <= k2 <= k3] = sum_q U[q][k3] G[q][k2] G[q][k1], tetrahedral store (the rank-Q tile kernel; a tile's
@@ -2702,25 +2736,37 @@ static ModeArgs batch_sep_args(const BatchGrid& bg)
 	return a;
 }
 
-// separable plans: every grid's factors [G; U] from its pair products, then one grouped rank-Q product
-// over all grids (the single-grid path's kernels, so bit-identical to it; the tile order applies to the
-// tile kernel only)
+// separable plans: every grid's pair products and factors [G; U] in one grouped launch, then one grouped
+// rank-Q product over all grids (the single-grid path's sums, so bit-identical to it; the tile order applies
+// to the tile kernel only)
 static void batch_separable(BatchPlan& plan, int order)
 {
 	ModeArgs a[kMaxGroup];
 	RankQProblem pr[kMaxGroup];
-	int n = 0, Q = -1, Qp = 0;
+	SepTablesGroup tg;
+	int n = 0, Q = -1, Qp = 0, wgs = 0;
+	size_t shm = 0;
 	bool same_q = true;
+	GLI_REQUIRE(plan.grids.size() <= (size_t)kMaxGroup, "too many grids in one plan");
 	for (auto& g : plan.grids) {
 		const EriSetup& s = g->s;
 		const int64_t Qg = g->sep_Qp;
-		separable_gu(s, g->P.p, g->sep_gw, (int)Qg, g->GU.p);
+		tg.p[n] = SepTablesProblem{plan.d_params.p + g->coeff_off, s.d_keys, g->sep_gw, g->GU.p, plan.tol, s.K, s.R, s.C1,
+		                           s.M, (int)(2 * Qg), s.Cp, wgs};
+		wgs += s.Cp;
+		shm = std::max(shm, (size_t)(s.K + 3 * (2 * s.K - 1) + (4 * s.K - 3)) * sizeof(double));
 		a[n] = batch_sep_args(*g);
 		pr[n] = RankQProblem{g->GU.p, g->GU.p + Qg * s.Cp, g->V.p, s.Cp, s.C1, 0};
 		same_q = same_q && (Q < 0 || g->sep_Q == Q);
 		Q = g->sep_Q;
 		Qp = (n == 0) ? (int)Qg : std::min(Qp, (int)Qg);
 		++n;
+	}
+	tg.n = n;
+	if (wgs > 0) {
+		// every grid's pair products and [G; U] in one launch (the single-grid path's k_separable_tables sums)
+		k_separable_tables_grouped<<<(unsigned)wgs, 256, shm, ctx().stream>>>(tg);
+		GLI_HIP(hipGetLastError());
 	}
 	// the tetrahedral kernel takes one node count per launch; the tile kernel any mix
 	if (!(same_q && rankq_tet_launch(pr, n, Q, Qp))) { mode_product_grouped(a, n, 0, order); }
@@ -2843,7 +2889,9 @@ static void batch_compute_branches(BatchPlan& plan, const BatchChoice& ch)
 // everything after the parameter upload
 static void batch_compute(BatchPlan& plan, const BatchChoice& ch)
 {
-	for (auto& g : plan.grids) { batch_tables(plan, *g); }
+	if (plan.method != GLI_ERI_SEPARABLE_FP64) {   // separable plans build their tables in batch_separable
+		for (auto& g : plan.grids) { batch_tables(plan, *g); }
+	}
 	const bool any_split = ch.split[0] || ch.split[1] || ch.split[2];
 	if (plan.method == GLI_ERI_SEPARABLE_FP64) {
 		batch_separable(plan, ch.gemm_order[2]);
@@ -2904,7 +2952,9 @@ static BatchChoice batch_autotune(BatchPlan& plan)
 {
 	BatchChoice ch;
 	std::string rep = "{\"modes\": [";
-	for (auto& g : plan.grids) { batch_tables(plan, *g); }
+	if (plan.method != GLI_ERI_SEPARABLE_FP64) {
+		for (auto& g : plan.grids) { batch_tables(plan, *g); }
+	}
 	// a mixed plan always runs the split-precision products (the autotuner picks among their
 	// variants only), so that the method a plan reports is the arithmetic it executes
 	const bool mixed = plan.method == GLI_ERI_TENSOR_MIXED;
