@@ -1662,7 +1662,7 @@ static void launch_tables(const EriSetup& s, const double* d_coeffs, double tol,
 		for (int q = 0; q < s.K; ++q) { ca.c[q] = h_coeffs[q]; }
 	}
 	k_products<<<(unsigned)(s.R * s.R), 256, (size_t)(s.K + 3 * L) * sizeof(double), st>>>(d_coeffs, ca, s.K, s.R, tol, P);
-	if (B != nullptr) {   // the separable method folds the Gaussian factors instead (k_separable_gu)
+	if (B != nullptr) {   // the separable method needs no folded columns (k_separable_tables)
 		k_fold<<<dim3((unsigned)ceil_div(s.Cp, 128), s.Mp), 128, 0, st>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, s.Cp, B);
 	}
 	GLI_HIP(hipGetLastError());
@@ -2136,51 +2136,16 @@ static void separable_factors(EriWorkspace& w, const EriSetup& s)
 	w.sep_Q = Q;
 }
 
-// [G; U][r][key] = sum_mu gw[mu][r] B[mu][key] straight from the pair products: B[mu][key] sums the P'
-// entries t of the key's shift pair (a, b) with |t - (2K-2) + 3 cd| = mu (k_fold), so
-//     [G; U][r][key] = sum_{t < 4K-3} P'_ab[t] gw[|t - (2K-2) + 3 cd|][r]   (terms with that index above M
-// vanish).  One workgroup per key column (Cp of them; padding columns are zero), one thread per row r of
-// [g | W g]; the key's P' row is staged in LDS.  About 8 MFLOP at 11^3: the launch is latency-bound, so
-// each step issues 16 independent loads before its FMAs (four accumulators, added at the end).
-__global__ __launch_bounds__(256) void k_separable_gu(const double* __restrict__ P, const int4* __restrict__ keys, int C1,
-                                                      int K, int R, int M, const double* __restrict__ gw, int rows,
-                                                      int Cp, double* __restrict__ GU)
-{
-	extern __shared__ double ps[];   // 4K - 3 doubles
-	const int key = blockIdx.x;
-	const int LP = 4 * K - 3, off = 2 * K - 2;
-	if (key >= C1) {
-		for (int r = threadIdx.x; r < rows; r += blockDim.x) { GU[(int64_t)r * Cp + key] = 0.0; }
-		return;
-	}
-	const int4 k = keys[key];
-	const double* p = P + ((int64_t)k.x * R + k.y) * LP;
-	for (int t = threadIdx.x; t < LP; t += blockDim.x) { ps[t] = p[t]; }
-	__syncthreads();
-	// the terms with |t + shift| <= M form one interval of t
-	const int shift = 3 * k.z - off;
-	const int tlo = max(0, -M - shift), thi = min(LP - 1, M - shift);
-	for (int r = threadIdx.x; r < rows; r += blockDim.x) {
-		double acc[4] = { 0, 0, 0, 0 };
-		int t = tlo;
-		for (; t + 16 <= thi + 1; t += 16) {
-			double g[16];
-#pragma unroll
-			for (int u = 0; u < 16; ++u) { g[u] = gw[(int64_t)abs(t + u + shift) * rows + r]; }
-#pragma unroll
-			for (int u = 0; u < 16; ++u) { acc[u & 3] = fma(ps[t + u], g[u], acc[u & 3]); }
-		}
-		for (; t <= thi; ++t) { acc[0] = fma(ps[t], gw[(int64_t)abs(t + shift) * rows + r], acc[0]); }
-		GU[(int64_t)r * Cp + key] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-	}
-}
-
 // The separable method's tables in one launch: one workgroup per key column (a, b, cd) computes the two
 // factor rows f_a and f_b (factor_row), its shift pair's products P'_ab (the k_products sums, zeroed where
-// |P| <= tol) and the column's [G; U] (the k_separable_gu sums), all in LDS, with the same terms in the same
-// order as those two kernels.  The 2R - 1 or so keys of one shift pair recompute the same P', which costs
-// less than a second launch.  Dynamic LDS: K coefficients, 2K - 1 Gaussians, two factor rows, 4K - 3 products.
-// The coefficients come from device memory c, or from ca when c is null.
+// |P| <= tol) and the column's rank-Q factors
+//     [G; U][r][key] = sum_{t < 4K-3} P'_ab[t] gw[|t - (2K-2) + 3 cd|][r]   (terms with that index above M vanish),
+// all in LDS.  gw = [g | W g] holds the Gaussians of the separable rule per folded index mu; B[mu][key] (k_fold)
+// sums the P' entries t with |t - (2K-2) + 3 cd| = mu, so this is [g | W g]^T B without forming B.  The 2R - 1
+// or so keys of one shift pair recompute the same P', which costs less than a second launch.  Each row's sum
+// issues 16 independent loads per step before its FMAs (four accumulators, added at the end): the launch is
+// latency-bound.  Dynamic LDS: K coefficients, 2K - 1 Gaussians, two factor rows, 4K - 3 products.  The
+// coefficients come from device memory c, or from ca when c is null.
 __device__ inline void separable_tables_column(const double* __restrict__ c, const CoeffArg& ca, int K, int R,
                                                double tol, const int4* __restrict__ keys, int C1, int M,
                                                const double* __restrict__ gw, int rows, int Cp,
@@ -2320,15 +2285,6 @@ static bool rankq_tet_launch(const RankQProblem* probs, int n, int Q, int Qp)
 	}
 	GLI_HIP(hipGetLastError());
 	return true;
-}
-
-// [G; U] of one grid from its pair products P' (library stream)
-static void separable_gu(const EriSetup& s, const double* P, const double* gw, int Qp, double* GU)
-{
-	const int rows = 2 * Qp;
-	k_separable_gu<<<(unsigned)s.Cp, (unsigned)std::min(256, (int)round_up(rows, 64)), (size_t)(4 * s.K - 3) * sizeof(double),
-	                 ctx().stream>>>(P, s.d_keys, s.C1, s.K, s.R, s.M, gw, rows, s.Cp, GU);
-	GLI_HIP(hipGetLastError());
 }
 
 // The key tensor of the separable method, queued on the library stream: the rank-Q factors G and U
