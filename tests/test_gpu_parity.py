@@ -593,6 +593,31 @@ def test_rankq_tet_bit_identical_to_tile_kernel(L, n, K, mirror):
         L.gli_set_tuning(b"rankq_tet", saved)
 
 
+@pytest.mark.parametrize("n,K", [(1, 3), (3, 301)])
+def test_rankq_tet_smallest_and_largest_rules(L, orc, n, K):
+    """the shortest and the longest node rules the tetrahedral rank-Q kernel is built for: the one-point grid with
+    K = 3 (folded range 4, Q = 34, S = 9) and 3^3 with K = 301 (range 612, Q = 114, S = 29, coefficients from device
+    memory).  Bit-identical to the tile kernel, and within 1e-13 relative of the oracle's reference sums"""
+    say("rankq tet rules", n, K)
+    c = _coeffs_sym(K, 5 + K)
+    d, num = eri_enumerate_device(n)
+    idx = d.to_numpy(np.int64, num)
+    sample = idx[:: max(1, num // 64)]
+    saved = L.gli_get_tuning(b"rankq_tet")
+    try:
+        assert L.gli_set_tuning(b"rankq_tet", 0) == 0
+        tile = eri_values_device(c, sym(n), idx, 1e-16, 6)
+        assert L.gli_set_tuning(b"rankq_tet", 1) == 0
+        tet = eri_values_device(c, sym(n), idx, 1e-16, 6)
+        assert np.array_equal(tet.view(np.uint64), tile.view(np.uint64)), float(np.max(np.abs(tet - tile)))
+        ref = orc.eri_sparse(c, sym(n), sample, 1e-16)
+        got = eri_values_device(c, sym(n), sample, 1e-16, 6)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * max(1.0, float(np.max(np.abs(ref))))
+    finally:
+        L.gli_clear_error()
+        L.gli_set_tuning(b"rankq_tet", saved)
+
+
 # ---------------------------------------------------------------------------
 # batch plans: several grids, grouped launches, autotuner, one replayed hipGraph
 
