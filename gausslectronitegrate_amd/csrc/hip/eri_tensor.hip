@@ -161,7 +161,7 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 {
 	constexpr int TM = BM / 32, TN = BN / 32;
 	const int tid = threadIdx.x;
-	const int lane = tid & 63, wave = tid >> 6;
+	const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
 	const int64_t p0 = (int64_t)rt * BM;
 	const int k0 = ct * BN;
 
@@ -193,6 +193,25 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 	for (int i = 0; i < TM; ++i) {
 #pragma unroll
 		for (int j = 0; j < TN; ++j) { acc[i][j] = d4{0, 0, 0, 0}; }
+	}
+	// 16 x 16 blocks of this wave that hold no stored entry: with p = outer Cp + r, tri_cols keeps k <= r
+	// and tri_rows r <= outer (the tetrahedral store keeps both), so a block whose smallest column exceeds
+	// its largest r, or whose smallest r exceeds outer, is never stored and its MFMAs are skipped
+	// (wave-uniform; the stored blocks are unchanged)
+	bool need[TM][TN];
+	{
+		const int64_t gp = a.p_base + p0;
+		const int64_t outer = (a.tri_cols || a.tri_rows) ? gp / a.Cp : 0;
+		const int64_t rlo = (a.tri_cols || a.tri_rows) ? gp - outer * a.Cp : 0;
+#pragma unroll
+		for (int i = 0; i < TM; ++i) {
+			const int64_t rmin = rlo + wm * (BM / 2) + i * 16, rmax = rmin + 15;
+#pragma unroll
+			for (int j = 0; j < TN; ++j) {
+				const int64_t cmin = k0 + wn * (BN / 2) + j * 16;
+				need[i][j] = (!a.tri_cols || cmin <= rmax) && (!a.tri_rows || rmin <= outer);
+			}
+		}
 	}
 
 	// software pipeline: the next mu-slice is loaded into registers while the current one
@@ -235,37 +254,51 @@ __device__ inline void mode_product_tile(const ModeArgs& a, int rt, int ct, doub
 			rb[r] = a.B[okb[r] ? (unsigned)((mu0 + mu) * ldb + k0 + k) : 0u];
 		}
 	};
-	gload(0);
-	for (int mu0 = 0; mu0 < a.Kd; mu0 += GBK) {
+	// the mu loop in two versions: every block needed (most tiles: the plain unrolled MFMA sequence), or
+	// a wave-uniform test per block (tiles on the faces of the stored region)
+	auto mu_loop = [&](auto every) {
+		gload(0);
+		for (int mu0 = 0; mu0 < a.Kd; mu0 += GBK) {
 #pragma unroll
-		for (int r = 0; r < PA; ++r) {
-			const int id = tid + 256 * r;
-			As[id / BM][id % BM] = oka[r] ? (kCp ? ra[r] * su[kCp ? r : 0] : ra[r]) : 0.0;
-		}
+			for (int r = 0; r < PA; ++r) {
+				const int id = tid + 256 * r;
+				As[id / BM][id % BM] = oka[r] ? (kCp ? ra[r] * su[kCp ? r : 0] : ra[r]) : 0.0;
+			}
 #pragma unroll
-		for (int r = 0; r < PB; ++r) {
-			const int id = tid + 256 * r;
-			Bs[id / BN][id % BN] = okb[r] ? rb[r] : 0.0;
-		}
-		__syncthreads();
-		gload(mu0 + GBK);   // past the last slice every element is masked off (reads offset 0)
+			for (int r = 0; r < PB; ++r) {
+				const int id = tid + 256 * r;
+				Bs[id / BN][id % BN] = okb[r] ? rb[r] : 0.0;
+			}
+			__syncthreads();
+			gload(mu0 + GBK);   // past the last slice every element is masked off (reads offset 0)
 #pragma unroll
-		for (int kk = 0; kk < GBK; kk += 4) {
-			double av[TM], bv[TN];
+			for (int kk = 0; kk < GBK; kk += 4) {
+				double av[TM], bv[TN];
 #pragma unroll
-			for (int i = 0; i < TM; ++i) { av[i] = As[kk + lane / 16][wm * (BM / 2) + i * 16 + lane % 16]; }
+				for (int i = 0; i < TM; ++i) { av[i] = As[kk + lane / 16][wm * (BM / 2) + i * 16 + lane % 16]; }
 #pragma unroll
-			for (int j = 0; j < TN; ++j) { bv[j] = Bs[kk + lane / 16][wn * (BN / 2) + j * 16 + lane % 16]; }
+				for (int j = 0; j < TN; ++j) { bv[j] = Bs[kk + lane / 16][wn * (BN / 2) + j * 16 + lane % 16]; }
 #pragma unroll
-			for (int i = 0; i < TM; ++i) {
+				for (int i = 0; i < TM; ++i) {
 #pragma unroll
-				for (int j = 0; j < TN; ++j) {
-					acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+					for (int j = 0; j < TN; ++j) {
+						if (decltype(every)::value || need[i][j]) {
+							acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+						}
+					}
 				}
 			}
+			__syncthreads();
 		}
-		__syncthreads();
+	};
+	bool every = true;
+#pragma unroll
+	for (int i = 0; i < TM; ++i) {
+#pragma unroll
+		for (int j = 0; j < TN; ++j) { every = every && need[i][j]; }
 	}
+	if (every) { mu_loop(std::true_type()); }
+	else { mu_loop(std::false_type()); }
 
 	// per-tile constants, so that no output element needs a division: the rows of a tile share the
 	// largest key tk3 (tetrahedral store; Cp and p_base are multiples of BM) and start at (m2, m3) =
