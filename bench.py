@@ -248,12 +248,14 @@ def mixed_vs_fp64(compare, method, value):
     return out
 
 
-def batch_measure(L, args, stream):
+def batch_measure(L, args, stream, method_name=None):
     """One step = every unique four-index of every grid of the batch, from the coefficients to the
     scaled values in HBM, as one replay of the plan's captured graph.  The error is taken over every
-    element against the single-grid FP64 path (times 1/a).  Returns the measurement as a dict."""
+    element against the single-grid FP64 path (times 1/a).  method_name overrides the bench's method.
+    Returns the measurement as a dict."""
     import torch
-    method = METHODS[args.method] if args.method in ("tensor", "mixed", "separable") else 0
+    name = method_name or args.method
+    method = METHODS[name] if name in ("tensor", "mixed", "separable") else 0
     n_g = len(BATCH)
     coeffs, grids, idxs, nums, vals = [], [], [], [], []
     enum_ms = 0.0
@@ -755,6 +757,15 @@ def main():
     if not args.no_batch:
         batch_line = batch_measure(L, args, stream)
         batch_line.pop("autotune", None)
+        if args.method != "mixed":
+            # the same batch with the split-precision numerics of the precision study (5-digit BF16 slices
+            # on MFMA, exact FP32 group sums, FP64 combination)
+            bm = batch_measure(L, args, stream, "mixed")
+            batch_line["mixed"] = {"method": "mixed", "dtype": "f64+split-bf16", "ms_per_step": bm["ms_per_step"],
+                                   "value": bm["value"], "plan_create_ms": bm["plan_create_ms"],
+                                   "max_abs_err_vs_fp64": max(e["max_abs_err_vs_fp64"] for e in bm["errors"]),
+                                   "note": "slower than the FP64 plan: the split products and their digit passes cost "
+                                           "more than FP64 MFMA on MI355X"}
 
     # the reference entry points with host arrays: indices in and values out over PCIe, the host
     # copies included (reported beside `value`, never as it)
