@@ -2287,6 +2287,14 @@ static void mode_product_cp(const ModeArgs& a)
 // The tetrahedral rank-Q kernel over n problems with Q nodes (G and U hold Qp >= 4 ceil(Q / 4) rows).
 // Returns false, having launched nothing, when the knob selects the tile kernel or no instance covers
 // S = ceil(Q / 4) steps; the caller then runs the tile kernel.
+// the node counts the tetrahedral kernel has instances for (S = ceil(Q / 4) steps), with the knob on
+static bool rankq_tet_supported(int Q)
+{
+	const int S = (Q + 3) / 4;
+	return ctx().rankq_tet != 0 && Q >= 1 &&
+	       (S == 2 || S == 3 || S == 5 || S == 9 || S == 13 || S == 17 || S == 21 || S == 25 || S == 29);
+}
+
 static bool rankq_tet_launch(const RankQProblem* probs, int n, int Q, int Qp)
 {
 	if (ctx().rankq_tet == 0 || n < 1 || n > kMaxGroup) { return false; }
@@ -2733,9 +2741,9 @@ static void batch_separable(BatchPlan& plan, int order)
 	ModeArgs a[kMaxGroup];
 	RankQProblem pr[kMaxGroup];
 	SepTablesGroup tg;
-	int n = 0, Q = -1, Qp = 0, wgs = 0;
+	int n = 0, wgs = 0;
 	size_t shm = 0;
-	bool same_q = true;
+	bool tet = true;   // every grid's node count has a tetrahedral instance
 	GLI_REQUIRE(plan.grids.size() <= (size_t)kMaxGroup, "too many grids in one plan");
 	for (auto& g : plan.grids) {
 		const EriSetup& s = g->s;
@@ -2746,9 +2754,7 @@ static void batch_separable(BatchPlan& plan, int order)
 		shm = std::max(shm, (size_t)(s.K + 3 * (2 * s.K - 1) + (4 * s.K - 3)) * sizeof(double));
 		a[n] = batch_sep_args(*g);
 		pr[n] = RankQProblem{g->GU.p, g->GU.p + Qg * s.Cp, g->V.p, s.Cp, s.C1, 0};
-		same_q = same_q && (Q < 0 || g->sep_Q == Q);
-		Q = g->sep_Q;
-		Qp = (n == 0) ? (int)Qg : std::min(Qp, (int)Qg);
+		tet = tet && rankq_tet_supported(g->sep_Q);
 		++n;
 	}
 	tg.n = n;
@@ -2757,8 +2763,25 @@ static void batch_separable(BatchPlan& plan, int order)
 		k_separable_tables_grouped<<<(unsigned)wgs, 256, shm, ctx().stream>>>(tg);
 		GLI_HIP(hipGetLastError());
 	}
-	// the tetrahedral kernel takes one node count per launch; the tile kernel any mix
-	if (!(same_q && rankq_tet_launch(pr, n, Q, Qp))) { mode_product_grouped(a, n, 0, order); }
+	if (!tet) {   // the tile kernel takes any mix of node counts in one launch
+		mode_product_grouped(a, n, 0, order);
+		return;
+	}
+	// the tetrahedral kernel: one grouped launch per distinct node count
+	bool done[kMaxGroup] = {};
+	for (int g0 = 0; g0 < n; ++g0) {
+		if (done[g0]) { continue; }
+		const int Q = plan.grids[g0]->sep_Q;
+		RankQProblem sub[kMaxGroup];
+		int m = 0, Qp = 1 << 30;
+		for (int g = g0; g < n; ++g) {
+			if (done[g] || plan.grids[g]->sep_Q != Q) { continue; }
+			sub[m++] = pr[g];
+			Qp = std::min(Qp, plan.grids[g]->sep_Qp);
+			done[g] = true;
+		}
+		GLI_REQUIRE(rankq_tet_launch(sub, m, Q, Qp), "rank-Q product: no tetrahedral instance for this node count");
+	}
 }
 
 // the gather of a batch plan: from the cached positions when the plan holds them
