@@ -781,10 +781,19 @@ __device__ inline void wave_positions(const GatherDesc& d, int64_t wave_base, co
 	const GridDev& g = d.g;
 	const DecodeConsts& dc = d.dc;
 	int64_t t[GATHER_EPT];
+	if (wave_base + 64 * GATHER_EPT <= d.num) {
+		// a wave wholly inside the list loads without per-element bounds (a wave-uniform test instead of a
+		// 64-bit compare and an exec mask per load: 11^3 gather 1.128 -> 1.107 ms)
+		const int64_t* __restrict__ src = d.idx + wave_base + lane;
 #pragma unroll
-	for (int u = 0; u < GATHER_EPT; ++u) {
-		const int64_t e = wave_base + u * 64 + lane;
-		t[u] = (e < d.num) ? d.idx[e] : -1;
+		for (int u = 0; u < GATHER_EPT; ++u) { t[u] = src[u * 64]; }
+	}
+	else {
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) {
+			const int64_t e = wave_base + u * 64 + lane;
+			t[u] = (e < d.num) ? d.idx[e] : -1;
+		}
 	}
 	// the wave's pair (i, j) from its first element
 	const int W = 4 * g.R - 3, off = 2 * (g.R - 1);
@@ -876,10 +885,17 @@ __device__ inline void gather_wave(const GatherDesc& d, int64_t wave_base, const
 #pragma unroll
 		for (int u = 0; u < GATHER_EPT; ++u) { v[u] *= sc; }
 	}
+	if (wave_base + 64 * GATHER_EPT <= d.num) {
+		double* __restrict__ dst = d.out + wave_base + lane;
 #pragma unroll
-	for (int u = 0; u < GATHER_EPT; ++u) {
-		const int64_t e = wave_base + u * 64 + lane;
-		if (e < d.num) { d.out[e] = v[u]; }
+		for (int u = 0; u < GATHER_EPT; ++u) { dst[u * 64] = v[u]; }
+	}
+	else {
+#pragma unroll
+		for (int u = 0; u < GATHER_EPT; ++u) {
+			const int64_t e = wave_base + u * 64 + lane;
+			if (e < d.num) { d.out[e] = v[u]; }
+		}
 	}
 }
 
@@ -1309,10 +1325,17 @@ __device__ inline void gather_wave_classed(const GatherDesc& d, int64_t wave_bas
 #pragma unroll
 		for (int u = 0; u < E; ++u) { v[u] *= sc; }
 	}
+	if (wave_base + 64 * E <= d.num) {
+		double* __restrict__ dst = d.out + wave_base + lane;
 #pragma unroll
-	for (int u = 0; u < E; ++u) {
-		const int64_t e = wave_base + u * 64 + lane;
-		if (e < d.num) { d.out[e] = v[u]; }
+		for (int u = 0; u < E; ++u) { dst[u * 64] = v[u]; }
+	}
+	else {
+#pragma unroll
+		for (int u = 0; u < E; ++u) {
+			const int64_t e = wave_base + u * 64 + lane;
+			if (e < d.num) { d.out[e] = v[u]; }
+		}
 	}
 }
 
