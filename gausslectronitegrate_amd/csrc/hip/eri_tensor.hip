@@ -990,7 +990,11 @@ __global__ __launch_bounds__(256) void k_gather_cached_grouped(const GatherDesc*
 	}
 	double v[GATHER_EPT];
 #pragma unroll
-	for (int u = 0; u < GATHER_EPT; ++u) { v[u] = (q[u] != kNoPosition) ? d.V[q[u]] : __builtin_nan(""); }
+	for (int u = 0; u < GATHER_EPT; ++u) {
+		const bool ok = (q[u] != kNoPosition);   // a select on the address keeps the load out of a branch
+		const double x = d.V[ok ? q[u] : 0u];
+		v[u] = ok ? x : __builtin_nan("");
+	}
 	if (d.scale != nullptr) {
 		const double sc = *d.scale;
 #pragma unroll
@@ -1051,17 +1055,32 @@ __global__ __launch_bounds__(kSortThreads) void k_gather_sorted_grouped(const Ga
 	if (piece < d.piece_begin || base >= d.num) { return; }   // whole workgroup idle
 	const int cnt = (int)min((int64_t)kSortWin, d.num - base);
 	constexpr int E = kSortWin / kSortThreads;
+	// a whole window (every workgroup but a list's last) loads and stores without per-element bounds
+	const bool full = (cnt == kSortWin);
 	uint32_t q[E];
 	uint16_t p[E];
+	if (full) {
 #pragma unroll
-	for (int r = 0; r < E; ++r) {
-		const int i = r * kSortThreads + threadIdx.x;
-		q[r] = (i < cnt) ? d.pos[base + i] : kNoPosition;
-		p[r] = (i < cnt) ? d.perm[base + i] : 0;
+		for (int r = 0; r < E; ++r) {
+			q[r] = d.pos[base + r * kSortThreads + threadIdx.x];
+			p[r] = d.perm[base + r * kSortThreads + threadIdx.x];
+		}
+	}
+	else {
+#pragma unroll
+		for (int r = 0; r < E; ++r) {
+			const int i = r * kSortThreads + threadIdx.x;
+			q[r] = (i < cnt) ? d.pos[base + i] : kNoPosition;
+			p[r] = (i < cnt) ? d.perm[base + i] : 0;
+		}
 	}
 	double v[E];
 #pragma unroll
-	for (int r = 0; r < E; ++r) { v[r] = (q[r] != kNoPosition) ? d.V[q[r]] : __builtin_nan(""); }
+	for (int r = 0; r < E; ++r) {
+		const bool ok = (q[r] != kNoPosition);   // a select on the address keeps the load out of a branch
+		const double x = d.V[ok ? q[r] : 0u];
+		v[r] = ok ? x : __builtin_nan("");
+	}
 	const double sc = (d.scale != nullptr) ? *d.scale : 1.0;
 #pragma unroll
 	for (int r = 0; r < E; ++r) {
@@ -1069,10 +1088,16 @@ __global__ __launch_bounds__(kSortThreads) void k_gather_sorted_grouped(const Ga
 		if (i < cnt) { so[p[r]] = (d.scale != nullptr) ? v[r] * sc : v[r]; }
 	}
 	__syncthreads();
+	if (full) {
 #pragma unroll
-	for (int r = 0; r < E; ++r) {
-		const int i = r * kSortThreads + threadIdx.x;
-		if (i < cnt) { d.out[base + i] = so[i]; }
+		for (int r = 0; r < E; ++r) { d.out[base + r * kSortThreads + threadIdx.x] = so[r * kSortThreads + threadIdx.x]; }
+	}
+	else {
+#pragma unroll
+		for (int r = 0; r < E; ++r) {
+			const int i = r * kSortThreads + threadIdx.x;
+			if (i < cnt) { d.out[base + i] = so[i]; }
+		}
 	}
 }
 
