@@ -748,6 +748,40 @@ def test_batch_plan_matches_single_grids(L, method, branches, positions, sorted_
         L.gli_eri_batch_destroy(plan)
 
 
+@pytest.mark.parametrize("method", [0, 6])
+def test_batch_plan_indices_outside_grid_in_whole_windows(L, method):
+    """indices outside the grid inside whole windows of 8192 (and in the partial last one) of a plan with
+    cached, window-sorted positions give NaN there and the single-call values everywhere else"""
+    say("batch plan, indices outside the grid, method", method)
+    saved = L.gli_get_tuning(b"batch_positions"), L.gli_get_tuning(b"batch_sorted")
+    assert L.gli_set_tuning(b"batch_positions", 1) == 0
+    assert L.gli_set_tuning(b"batch_sorted", 1) == 0
+    N4 = 5 ** 12
+    idx = golden("golden_enum_n5.npz")["eri_indices"][:3 * 8192 + 777].astype(np.int64).copy()
+    bad_at = np.array([0, 1, 4000, 8191, 8192, 12345, 16383, 20000, len(idx) - 1])
+    idx[bad_at] = np.array([-1, N4, -5, N4 + 7, 2 ** 40, -2 ** 40, N4, -1, N4 + 1])
+    entries = _batch_setup([(sym(5), golden("coeffs_K29_s0.npy"), idx)])
+    cptr, Ks, grids, dptr, nums, vptr = _batch_arrays(entries)
+    plan = C.c_void_p()
+    try:
+        _lib.check(L.gli_eri_batch_create(1, cptr, Ks.ctypes.data_as(_abi.P_glong), grids.ctypes.data_as(_abi.P_glong),
+                                          dptr, nums.ctypes.data_as(_abi.P_glong), 1e-16, method, vptr, 1,
+                                          C.byref(plan)), "gli_eri_batch_create")
+    finally:
+        L.gli_set_tuning(b"batch_positions", saved[0])
+        L.gli_set_tuning(b"batch_sorted", saved[1])
+    try:
+        ref = eri_values_device(entries[0][1], entries[0][0], idx, 1e-16, method)
+        assert np.all(np.isnan(ref[bad_at])) and np.isnan(ref).sum() == len(bad_at)
+        for rep in range(2):
+            _lib.check(L.gli_eri_batch_run(plan, None, None), "gli_eri_batch_run")
+            assert L.gli_synchronize() == 0
+            got = entries[0][4].to_numpy(np.float64, len(idx))
+            assert np.array_equal(got, ref, equal_nan=True), rep
+    finally:
+        L.gli_eri_batch_destroy(plan)
+
+
 def test_batch_one_shot_entry(L):
     """gli_eri_values_batch_device (a plan without autotuning, run once) equals the single-grid path"""
     say("batch one shot")
